@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain BSGS sieve on one MI355X.
+
+Workload (R745-shaped, the reference's headline case): base 745, 22 odd k
+drawn in [3, 10^7] by seed, n in [180000, 250000] kept when no odd prime
+q < 10^5 divides k*745^n-1, and p = every prime in [1e12, 1e12 + K*2^30).
+One step = one 2^30-wide p-chunk (about 3.9e7 primes) through the device
+sieve (set-up kernel + BSGS kernel); the chunk's prime offsets are resident
+in HBM before timing starts (host prime generation is outside the timed
+region and reported separately).
+
+value = p-range advanced per second (the reference's "p/sec",
+kernel_v0.24.6.cu:1552-1559, :1759), the unit of its published 360e6 p/s
+on an A100 (README.md:20); primes/s is reported beside it.
+
+roofline: the BSGS kernel is bound by VALU 32-bit integer multiply-adds
+(neither HBM nor MFMA).  achieved = algorithmic v_mad_u64_u32-class
+multiplies per launch / mean launch time (HIP events on the sieve's own
+stream), peak = the same instruction's rate measured on this device.
+cpu_baseline: the CPU oracle (a C++ restatement of the reference
+algorithm, OpenMP over primes) on a bounded sample of chunk 0.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "cuda-riesel-sieve_amd"), os.path.join(ROOT, "oracle")]
+
+A100_PUBLISHED = 360e6  # README.md:20, -b 5 -m 8 -s 192 -Q 18, R745.ABCD
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=0, help="baby steps (0 = cost model)")
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--chunk-log2", type=int, default=30)
+    ap.add_argument("--max-batch", type=int, default=1 << 22)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(p-chunks shard across GPUs with no exchange; see the sharding design)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import pyrsieve as rs
+
+    if rs.device_count() < 1:
+        log("bench.py: no HIP device visible")
+        sys.exit(1)
+    K, W = args.steps, args.warmup
+    chunk = 1 << args.chunk_log2
+    p0 = 10**12
+    t = time.time()
+    abcd = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=args.seed)
+    plan = rs.Plan(abcd)
+    log(f"job: Q={plan.Q} rows={plan.nrows} n=[{plan.minN},{plan.maxN}] "
+        f"({time.time() - t:.1f}s)")
+    t = time.time()
+    gen = rs.PrimeGen(p0 + K * chunk + 1, threads=min(16, os.cpu_count() or 1))
+    bufs, counts = [], []
+    for i in range(K):
+        lo = p0 + i * chunk
+        offs = gen.range(lo, lo + chunk)
+        counts.append(len(offs))
+        bufs.append(rs.DeviceBuffer(0, offs))
+        if i == 0:
+            sample_offs = offs
+    gen_s = time.time() - t
+    nprimes = sum(counts)
+    log(f"primes: {nprimes} in [{p0}, {p0 + K * chunk}) generated+uploaded in {gen_s:.1f}s")
+    sv = rs.Sieve(plan, m=args.m, max_batch=args.max_batch, hit_capacity=1 << 22)
+    log(f"sieve: m={sv.m} T={sv.T} slots={sv.slots} lds/wave={sv.lds_bytes}B")
+    for w in range(W):
+        sv.submit_device(p0 + (w % K) * chunk, bufs[w % K])
+        sv.collect()
+        log(f"warmup step {w} done")
+    sv.timing()  # reset the event accumulators
+    t0 = time.perf_counter()
+    for i in range(K):
+        sv.submit_device(p0 + i * chunk, bufs[i])
+    hits = sv.collect()
+    t1 = time.perf_counter()
+    tm = sv.timing()
+    elapsed = t1 - t0
+    log(f"timed: {K} steps in {elapsed:.3f}s, {len(hits)} factors, "
+        f"bsgs {tm['bsgs_ms']:.1f} ms, prep {tm['prep_ms']:.1f} ms")
+    prange_rate = K * chunk / elapsed
+    primes_rate = nprimes / elapsed
+
+    # algorithmic multiplies per prime in the BSGS kernel: m baby steps and
+    # Q*T Y products at 11 32-bit multiplies each (full 64x64 Montgomery),
+    # A*T giant steps at 5 (32x64 Montgomery), A = rows/2 expected Jacobi
+    # survivors
+    A = plan.nrows / 2.0
+    muls_per_prime = 11 * sv.m + 11 * plan.Q * sv.T + 5 * A * sv.T
+    launches = math.ceil(counts[0] / args.max_batch) * K
+    per_launch_primes = nprimes / launches
+    bsgs_s_per_launch = tm["bsgs_ms"] / 1e3 / launches
+    achieved = muls_per_prime * per_launch_primes / bsgs_s_per_launch
+    peak = rs.dev_mad_rate()
+    roof = {"bound": "valu", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "Gmul/s",
+            "frac": achieved / peak, "traffic": None,
+            "kernel": "rs_bsgs_kernel", "muls_per_prime": muls_per_prime,
+            "primes_per_launch": per_launch_primes,
+            "mean_launch_ms": bsgs_s_per_launch * 1e3}
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        rk, rr, _ = plan.rows()
+        job = oracle.Job(plan.base, plan.Q, sv.m, plan.minN, plan.maxN, rk, rr)
+        n = 2000
+        while True:
+            ts = time.perf_counter()
+            oracle.bsgs_many(job, p0, sample_offs[:n], threads=args.cpu_threads)
+            dt = time.perf_counter() - ts
+            if dt > args.cpu_seconds / 4 or n >= len(sample_offs):
+                break
+            n = min(len(sample_offs), int(n * max(2.0, args.cpu_seconds / 4 / max(dt, 1e-3))))
+        span = int(sample_offs[n - 1]) + 1
+        cpu = {"value": span / dt, "unit": "p/sec", "cores": args.cpu_threads, "kind": "port",
+               "primes_per_sec": n / dt,
+               "sample": f"first {n} primes of [1e12, 1e12+2^30) (p-range {span}), "
+                         f"same job and m, {dt:.2f}s"}
+        log(f"cpu: {n} primes in {dt:.2f}s on {args.cpu_threads} threads")
+
+    out = {
+        "metric": "primes/sec sieved (22 k, n 180k-250k) plain vs batched-serve; p99 job latency",
+        "value": prange_rate,
+        "unit": "p/sec (p-range advanced per second, the reference's unit)",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": prange_rate / A100_PUBLISHED,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "R745-shaped plain BSGS: base 745, 22 odd k in [3,1e7] "
+                               f"(seed {args.seed}), n in [180000,250000] presieved by odd "
+                               f"primes < 1e5, p in [1e12, 1e12+{K}*2^{args.chunk_log2})",
+                   "Q": plan.Q, "rows": plan.nrows, "m": sv.m, "T": sv.T,
+                   "primes": nprimes, "chunk": chunk, "global_batch": counts[0],
+                   "parallelism": "single GPU, wavefront per prime"},
+        "primes_per_sec": primes_rate,
+        "factors": len(hits),
+        "kernel_ms": {"bsgs": tm["bsgs_ms"], "prep": tm["prep_ms"]},
+        "host_primegen_s": gen_s,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

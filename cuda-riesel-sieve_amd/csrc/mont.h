@@ -1,0 +1,118 @@
+// mont.h -- exact 64-bit modular arithmetic for gfx950.
+//
+// Residues are Montgomery forms x*R mod p with R = 2^64, for odd
+// 3 <= p < 2^63.  Products are built from 32x32+64 -> 64 multiply-adds,
+// which the AMDGPU backend selects as v_mad_u64_u32, so every carry is
+// exact (the reference's PTX version, kernel_v0.24.6.cu:205-241, loses the
+// limb-1 carry at :226).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+// a*b + c, exact in 64 bits for 32-bit a, b, c (one v_mad_u64_u32).
+__device__ __forceinline__ u64 mad32(u32 a, u32 b, u64 c) {
+    return (u64)a * (u64)b + c;
+}
+
+__device__ __forceinline__ u32 lo32(u64 x) { return (u32)x; }
+__device__ __forceinline__ u32 hi32(u64 x) { return (u32)(x >> 32); }
+
+// -p^-1 mod 2^64 (p odd): Newton steps x <- x(2 - p x), four in 32 bits
+// (3 -> 48 correct bits) and one in 64 bits.
+__device__ __forceinline__ u64 mont_mprime(u64 p) {
+    u32 p0 = lo32(p);
+    u32 x = p0;
+    x *= 2u - p0 * x;
+    x *= 2u - p0 * x;
+    x *= 2u - p0 * x;
+    x *= 2u - p0 * x;
+    u64 y = x;
+    y *= 2ull - p * y;
+    return 0ull - y;
+}
+
+// Montgomery product a*b*2^-64 mod p.  Needs b < p and a < 2^64 (a may be
+// an unreduced integer such as k or b): then a*b + m*p < 2^128 and the
+// result before the final subtraction is < 2p.
+//
+// T = a*b as four partial products (t3:t2.lo:t0.lo), m = lo64(T)*mp mod 2^64
+// (three products), then hi64(T + m*p) column by column (four products);
+// the low 64 bits of T + m*p are zero by construction.
+__device__ __forceinline__ u64 montmul(u64 a, u64 b, u64 p, u64 mp) {
+    const u32 a0 = lo32(a), a1 = hi32(a), b0 = lo32(b), b1 = hi32(b);
+    const u64 t0 = mad32(a0, b0, 0);
+    const u64 t1 = mad32(a0, b1, hi32(t0));
+    const u64 t2 = mad32(a1, b0, lo32(t1));
+    const u64 t3 = mad32(a1, b1, hi32(t1)) + hi32(t2);
+    const u32 l0 = lo32(t0), l1 = lo32(t2);
+    const u32 mp0 = lo32(mp), mp1 = hi32(mp);
+    const u64 q0 = mad32(l0, mp0, 0);
+    const u32 m0 = lo32(q0);
+    const u32 m1 = hi32(q0) + l0 * mp1 + l1 * mp0;
+    const u32 p0 = lo32(p), p1 = hi32(p);
+    const u64 s0 = mad32(m0, p0, l0);               // low word is 0
+    const u64 s1 = mad32(m0, p1, hi32(s0));
+    const u64 s2 = mad32(m1, p0, lo32(s1)) + l1;    // low word is 0
+    const u64 s3 = mad32(m1, p1, hi32(s1)) + hi32(s2);
+    u64 r = t3 + s3;
+    return r >= p ? r - p : r;
+}
+
+// k*y*2^-32 mod p for 32-bit k and y < p: one 32-bit Montgomery step.
+// k*y < 2^32 p and m*p < 2^32 p, so (k*y + m*p)/2^32 < 2p fits in 64 bits.
+__device__ __forceinline__ u64 halfmul(u32 k, u64 y, u64 p, u32 mp0) {
+    const u64 a0 = mad32(k, lo32(y), 0);
+    const u64 a1 = mad32(k, hi32(y), hi32(a0));
+    const u32 m = lo32(a0) * mp0;
+    const u64 s0 = mad32(m, lo32(p), lo32(a0));     // low word is 0
+    const u64 s1 = mad32(m, hi32(p), hi32(s0));
+    u64 r = a1 + s1;
+    return r >= p ? r - p : r;
+}
+
+// x^e in Montgomery form (x and one = R mod p in Montgomery form).
+__device__ __forceinline__ u64 mont_pow(u64 x, u32 e, u64 one, u64 p, u64 mp) {
+    u64 r = one;
+    while (e) {
+        if (e & 1) r = montmul(r, x, p, mp);
+        e >>= 1;
+        if (e) x = montmul(x, x, p, mp);
+    }
+    return r;
+}
+
+// Jacobi symbol (a/p) for 32-bit a and odd p > 1, in {-1, 0, 1}.
+// Binary algorithm after one reduction p mod a.
+__device__ __forceinline__ int jacobi32(u32 a, u64 p) {
+    int t = 1;
+    u32 n;
+    if (p <= 0xffffffffull && a >= (u32)p) a %= (u32)p;
+    if (a == 0) return p == 1 ? 1 : 0;
+    const u32 z = __builtin_ctz(a);
+    a >>= z;
+    const u32 p8 = lo32(p) & 7;
+    if ((z & 1) && (p8 == 3 || p8 == 5)) t = -t;
+    if (a == 1) return t;
+    // reciprocity: (a/p) = (p/a) (-1)^((a-1)/2 (p-1)/2), a odd > 1
+    if ((a & 3) == 3 && (p8 & 3) == 3) t = -t;
+    n = a;
+    a = (u32)(p % n);
+    while (a != 0) {
+        const u32 s = __builtin_ctz(a);
+        a >>= s;
+        const u32 n8 = n & 7;
+        if ((s & 1) && (n8 == 3 || n8 == 5)) t = -t;
+        if (a < n) {
+            const u32 tmp = a;
+            a = n;
+            n = tmp;
+            if ((a & 3) == 3 && (n & 3) == 3) t = -t;
+        }
+        a -= n;
+    }
+    return n == 1 ? t : 0;
+}

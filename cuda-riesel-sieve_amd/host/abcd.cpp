@@ -1,0 +1,277 @@
+// abcd.cpp -- ABCD sieve-file input/output and the synthetic input
+// generator.
+//
+// Reading follows countKs (kernel_v0.24.6.cu:113-137, FileInput.cpp:12-36)
+// and the parser in main (kernel_v0.24.6.cu:938-1083): the token before the
+// first space decides the line type; "ABCD k*b^$a-1 [n0]" starts a k, the
+// base is taken from the first header only (:1009-1016), and every other
+// line is a positive delta added to the previous n (:1049-1054).  Two
+// deliberate differences: malformed lines are an error instead of a stoi
+// exception or silent garbage, and minN/maxN cover every n (the reference
+// raises maxN only on delta lines).
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <thread>
+#include <unordered_set>
+
+#include "host.h"
+
+namespace {
+
+// Parses a non-negative decimal integer at s (leading blanks allowed);
+// returns false if there is no digit or it does not fit in int32.
+bool parse_int(const char* s, long long* out) {
+    while (*s == ' ' || *s == '\t') ++s;
+    if (!isdigit((unsigned char)*s)) return false;
+    long long v = 0;
+    while (isdigit((unsigned char)*s)) {
+        v = v * 10 + (*s - '0');
+        if (v > 0x7fffffffLL) return false;
+        ++s;
+    }
+    *out = v;
+    return true;
+}
+
+void finish(rs_abcd* a) {
+    a->kstart.push_back((int32_t)a->ns.size());
+    int lo = 0x7fffffff, hi = 0;
+    for (int32_t n : a->ns) {
+        lo = std::min(lo, (int)n);
+        hi = std::max(hi, (int)n);
+    }
+    a->minN = a->ns.empty() ? 0 : lo;
+    a->maxN = a->ns.empty() ? 0 : hi;
+}
+
+}  // namespace
+
+extern "C" rs_status rs_abcd_read(const char* path, rs_abcd** out) {
+    if (!path || !out) return RS_ERR_ARG;
+    *out = nullptr;
+    std::ifstream f(path);
+    if (!f.is_open()) return RS_ERR_IO;
+    rs_abcd* a = new rs_abcd();
+    std::string line;
+    long long total = 0;
+    bool have = false;
+    while (std::getline(f, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        size_t sp = line.find(' ');
+        std::string tok = line.substr(0, sp);
+        if (tok == "ABCD") {
+            if (sp == std::string::npos) { delete a; return RS_ERR_FORMAT; }
+            const std::string rest = line.substr(sp + 1);
+            size_t st = rest.find('*'), car = rest.find('^');
+            size_t lb = rest.find('['), rb = rest.find(']');
+            if (st == std::string::npos || car == std::string::npos || car < st ||
+                lb == std::string::npos || rb == std::string::npos || rb < lb) {
+                delete a;
+                return RS_ERR_FORMAT;
+            }
+            // only Riesel (-1) sequences: "...^$a-1"
+            if (rest.find("-1", car) == std::string::npos || rest.find("-1", car) > lb) {
+                delete a;
+                return RS_ERR_FORMAT;
+            }
+            long long k, b, n0;
+            if (!parse_int(rest.substr(0, st).c_str(), &k) || k < 1 ||
+                !parse_int(rest.substr(st + 1, car - st - 1).c_str(), &b) || b < 2 ||
+                !parse_int(rest.substr(lb + 1, rb - lb - 1).c_str(), &n0)) {
+                delete a;
+                return RS_ERR_FORMAT;
+            }
+            if (a->base == 0) a->base = (uint32_t)b;
+            a->kstart.push_back((int32_t)a->ns.size());
+            a->ks.push_back((int32_t)k);
+            a->ns.push_back((int32_t)n0);
+            total = n0;
+            have = true;
+        } else {
+            long long d;
+            if (!have || !parse_int(tok.c_str(), &d) || d <= 0 ||
+                total + d > 0x7fffffffLL) {
+                delete a;
+                return RS_ERR_FORMAT;
+            }
+            total += d;
+            a->ns.push_back((int32_t)total);
+        }
+    }
+    finish(a);
+    *out = a;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_abcd_from_lists(uint32_t base, int nk, const int32_t* ks,
+                                        const int32_t* kstart, const int32_t* ns,
+                                        rs_abcd** out) {
+    if (!out || nk < 0 || base < 2 || (nk > 0 && (!ks || !kstart || !ns)))
+        return RS_ERR_ARG;
+    *out = nullptr;
+    rs_abcd* a = new rs_abcd();
+    a->base = base;
+    for (int i = 0; i < nk; ++i) {
+        if (ks[i] < 1 || kstart[i + 1] <= kstart[i]) { delete a; return RS_ERR_ARG; }
+        a->kstart.push_back((int32_t)a->ns.size());
+        a->ks.push_back(ks[i]);
+        for (int j = kstart[i]; j < kstart[i + 1]; ++j) {
+            if (ns[j] < 0 || (j > kstart[i] && ns[j] <= ns[j - 1])) {
+                delete a;
+                return RS_ERR_ARG;
+            }
+            a->ns.push_back(ns[j]);
+        }
+    }
+    finish(a);
+    *out = a;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_abcd_info(const rs_abcd* a, uint32_t* base, int* nk, int* nn,
+                                  int* minN, int* maxN) {
+    if (!a) return RS_ERR_ARG;
+    if (base) *base = a->base;
+    if (nk) *nk = (int)a->ks.size();
+    if (nn) *nn = (int)a->ns.size();
+    if (minN) *minN = a->minN;
+    if (maxN) *maxN = a->maxN;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_abcd_lists(const rs_abcd* a, int32_t* ks, int32_t* kstart,
+                                   int32_t* ns) {
+    if (!a) return RS_ERR_ARG;
+    if (ks) std::copy(a->ks.begin(), a->ks.end(), ks);
+    if (kstart) std::copy(a->kstart.begin(), a->kstart.end(), kstart);
+    if (ns) std::copy(a->ns.begin(), a->ns.end(), ns);
+    return RS_OK;
+}
+
+extern "C" rs_status rs_abcd_write(const rs_abcd* a, const char* path) {
+    if (!a || !path) return RS_ERR_ARG;
+    FILE* f = fopen(path, "w");
+    if (!f) return RS_ERR_IO;
+    for (size_t i = 0; i < a->ks.size(); ++i) {
+        int s = a->kstart[i], e = a->kstart[i + 1];
+        fprintf(f, "ABCD %d*%u^$a-1 [%d]\n", a->ks[i], a->base, a->ns[s]);
+        for (int j = s + 1; j < e; ++j) fprintf(f, "%d\n", a->ns[j] - a->ns[j - 1]);
+    }
+    return fclose(f) == 0 ? RS_OK : RS_ERR_IO;
+}
+
+extern "C" void rs_abcd_free(rs_abcd* a) { delete a; }
+
+// ------------------------------------------------------------ synthetic input
+
+namespace {
+
+uint64_t inv_mod(uint64_t a, uint64_t q) {  // q prime, a != 0 mod q
+    return rsh::powmod(a % q, q - 2, q);
+}
+
+}  // namespace
+
+// Presieve: n survives iff no odd prime q < bound (q ∤ b) divides k*b^n-1.
+// For each q the discrete-log table of b mod q gives, per k, the residue
+// class n == dlog(k^-1) (mod ord_q(b)) that q removes.  Work is split over
+// q across threads, which mark one shared removal byte per (k, n).
+extern "C" rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_t kmax,
+                                   int odd_k, int nmin, int nmax,
+                                   uint32_t presieve_bound, uint64_t seed,
+                                   rs_abcd** out) {
+    if (!out || base < 2 || nk < 1 || kmin < 1 || kmax < kmin || nmin < 1 ||
+        nmax < nmin || kmax > 0x7fffffffu)
+        return RS_ERR_ARG;
+    *out = nullptr;
+    const uint64_t span = (uint64_t)(kmax - kmin) + 1;
+    const uint64_t avail = odd_k ? (span + (kmin & 1)) / 2 : span;
+    if ((uint64_t)nk > avail) return RS_ERR_ARG;
+    std::vector<uint32_t> qs;
+    for (uint32_t q : rsh::primes_upto(presieve_bound ? presieve_bound - 1 : 0))
+        if (q != 2 && base % q != 0) qs.push_back(q);
+    const int width = nmax - nmin + 1;
+    rsh::SplitMix64 rng(seed);
+    std::vector<uint32_t> chosen;
+    std::vector<std::vector<int32_t>> lists;
+    std::unordered_set<uint32_t> used;
+    const int nthreads = rsh::hw_threads(0);
+    int draws = 0;
+    while ((int)chosen.size() < nk) {
+        // draw a batch of candidate k and presieve them together
+        std::vector<uint32_t> batch;
+        while ((int)(chosen.size() + batch.size()) < nk) {
+            if (++draws > 64 * nk + 1024) return RS_ERR_ARG;  // cannot fill
+            uint32_t k = kmin + (uint32_t)rng.below(span);
+            if (odd_k && (k & 1) == 0) { if (k + 1 > kmax) continue; k += 1; }
+            if (used.count(k)) continue;
+            used.insert(k);
+            batch.push_back(k);
+        }
+        const size_t B = batch.size();
+        // one shared removal byte per (k, n); marks are idempotent, written
+        // with relaxed atomic stores
+        std::vector<uint8_t> removed(B * (size_t)width, 0);
+        std::vector<std::thread> th;
+        for (int t = 0; t < nthreads; ++t) {
+            th.emplace_back([&, t]() {
+                std::vector<int32_t> dlog;
+                uint8_t* rm = removed.data();
+                for (size_t qi = t; qi < qs.size(); qi += nthreads) {
+                    const uint32_t q = qs[qi];
+                    const uint64_t bq = base % q;
+                    dlog.assign(q, -1);
+                    uint64_t v = 1;
+                    uint32_t ord = 0;
+                    do {
+                        dlog[v] = (int32_t)ord;
+                        v = v * bq % q;
+                        ++ord;
+                    } while (v != 1);
+                    for (size_t i = 0; i < B; ++i) {
+                        const uint64_t k = batch[i] % q;
+                        if (k == 0) continue;  // k*b^n-1 == -1 mod q
+                        const int32_t d = dlog[inv_mod(k, q)];
+                        if (d < 0) continue;
+                        // first n >= nmin with n == d (mod ord)
+                        long long n = d;
+                        if (n < nmin) n += ((nmin - n + ord - 1) / ord) * (long long)ord;
+                        uint8_t* row = rm + i * (size_t)width;
+                        for (; n <= nmax; n += ord)
+                            __atomic_store_n(&row[n - nmin], (uint8_t)1, __ATOMIC_RELAXED);
+                    }
+                }
+            });
+        }
+        for (auto& x : th) x.join();
+        for (size_t i = 0; i < B; ++i) {
+            std::vector<int32_t> ns;
+            const uint8_t* row = removed.data() + i * (size_t)width;
+            for (int n = nmin; n <= nmax; ++n)
+                if (!row[n - nmin]) ns.push_back(n);
+            if (ns.empty()) continue;  // covered k: redraw
+            chosen.push_back(batch[i]);
+            lists.push_back(std::move(ns));
+        }
+    }
+    // headers in ascending k, as srsieve writes them
+    std::vector<int> order(nk);
+    for (int i = 0; i < nk; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](int x, int y) { return chosen[x] < chosen[y]; });
+    rs_abcd* a = new rs_abcd();
+    a->base = base;
+    for (int i : order) {
+        a->kstart.push_back((int32_t)a->ns.size());
+        a->ks.push_back((int32_t)chosen[i]);
+        a->ns.insert(a->ns.end(), lists[i].begin(), lists[i].end());
+    }
+    finish(a);
+    *out = a;
+    return RS_OK;
+}

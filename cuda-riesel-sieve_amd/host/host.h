@@ -1,0 +1,55 @@
+// host.h -- internal types of the host side of librsieve (ABCD input,
+// planner, prime generator).  The C ABI over them is include/rsieve.h.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "rsieve.h"
+
+struct rs_abcd {
+    uint32_t base = 0;
+    std::vector<int32_t> ks;      // one entry per header line
+    std::vector<int32_t> kstart;  // CSR offsets into ns, size ks.size() + 1
+    std::vector<int32_t> ns;      // strictly increasing per k
+    int minN = 0, maxN = 0;
+};
+
+struct rs_plan {
+    uint32_t base = 0;
+    int Q = 0;
+    int minN = 0, maxN = 0;
+    int x0 = 0, x1 = 0;           // floor(minN/Q), floor(maxN/Q)
+    int words = 0;                // 32-bit words per row bitmap
+    std::vector<int32_t> rows_k;  // sorted by (k, r)
+    std::vector<int32_t> rows_r;
+    std::vector<uint32_t> bitmaps;  // rows * words, MSB-first
+};
+
+namespace rsh {
+
+// 64-bit helpers shared by the host code.
+typedef unsigned __int128 u128;
+
+inline uint64_t mulmod(uint64_t a, uint64_t b, uint64_t p) {
+    return (uint64_t)((u128)a * b % p);
+}
+uint64_t powmod(uint64_t b, uint64_t e, uint64_t p);
+bool is_prime_u64(uint64_t n);  // deterministic Miller-Rabin
+std::vector<uint32_t> primes_upto(uint32_t n);  // all primes <= n
+
+// splitmix64 stream used for seeded draws.
+struct SplitMix64 {
+    uint64_t s;
+    explicit SplitMix64(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    uint64_t below(uint64_t n) { return next() % n; }
+};
+
+int hw_threads(int requested);
+
+}  // namespace rsh

@@ -1,0 +1,240 @@
+// main.cpp -- rsieve, the command-line host: a drop-in for the reference
+// program's main (kernel_v0.24.6.cu:688-1807) over librsieve.
+//
+// Flags as at :705-786: -i FILE (ABCD input, default sr_745.abcd), -m M
+// (baby steps m = 2 << M, :931; without -m the cost model picks m), -p P0,
+// -P P1, -Q Q (even), -O FILE (factor file, default factors.txt), -b/-t
+// SCALE (primes per launch = 128*2^(b-1) * 128*2^(t-1), :817-818, :849-851),
+// -s SCALE (at least 4096*s sieving primes, :1274), -v (no verification:
+// datless output), -h.  Unlike :705, -h and -v take no argument.  With no
+// -p/-P the default test range of :837-839 is used.  The sieved range is
+// clipped to [P0, P1] (the reference runs past P1 with its last batch).
+// Output lines keep the reference's formats: factors "p | k*b^n-1"
+// (:1833) and a progress line every 20 launches (:1759).
+#include <getopt.h>
+
+#include <chrono>
+#include <cinttypes>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <vector>
+
+#include "rsieve.h"
+
+static void usage() {
+    printf(
+        "\nrsieve -- a BSGS sieve for multiple sequences of the form k*b^n-1 (MI355X)\n\n"
+        "-i FILE  : Read in abcd sieve file called FILE.\n"
+        "-m m     : Use 2<<m baby steps (hash table elements); default: cost model.\n"
+        "-p P0    : Start sieving from P0. Must have a corresponding P1.\n"
+        "-P P1    : Finish sieving at P1. Must have a corresponding P0.\n"
+        "                If no -p and -P arguments then we default to 102297149770000 for testing.\n"
+        "-Q Q     : Override subsequence value Q (even). Sieve k*b^n-1 as (k*b^d)*(b^Q)^m-1.\n"
+        "-O FILE  : Write factors to a file called FILE.\n"
+        "-b SCALE : Scale the primes per launch by 2^(SCALE-1) (blocks).\n"
+        "-t SCALE : Scale the primes per launch by 2^(SCALE-1) (threads).\n"
+        "-s SCALE : Use at least 4096*SCALE small primes for the prime generator. Default 256.\n"
+        "-v       : Don't verify factors. This implies running datless.\n"
+        "-d DEV   : GPU device index (default 0).\n"
+        "-h       : Prints this help\n");
+}
+
+static int fail(const char* what, rs_status st) {
+    fprintf(stderr, "%s failed: %s (%d)\n", what, rs_strerror(st), st);
+    return 1;
+}
+
+int main(int argc, char* argv[]) {
+    int blockScale = 1, threadScale = 1, Qin = 0, hashFlag = 0, verify = 1;
+    int smallPrimeScale = 256, device = 0;
+    const char* abcdFile = "sr_745.abcd";
+    const char* factorFile = "factors.txt";
+    unsigned long long low = 0, high = 0;
+    int opt;
+    while ((opt = getopt(argc, argv, "b:hi:m:p:P:Q:O:s:t:vd:")) != -1) {
+        switch (opt) {
+            case 'b': blockScale = (int)strtol(optarg, nullptr, 0); break;
+            case 'h': usage(); return 0;
+            case 'i': abcdFile = optarg; break;
+            case 'm': hashFlag = (int)strtol(optarg, nullptr, 0); break;
+            case 'p': low = strtoull(optarg, nullptr, 0); break;
+            case 'P': high = strtoull(optarg, nullptr, 0); break;
+            case 'Q':
+                Qin = (int)strtol(optarg, nullptr, 0);
+                if (Qin < 1 || Qin % 2 == 1) {
+                    printf("Bad input parameter : Q must be an even integer greater than 0.\n");
+                    return 1;
+                }
+                break;
+            case 'O': factorFile = optarg; break;
+            case 's': smallPrimeScale = (int)strtol(optarg, nullptr, 0); break;
+            case 't': threadScale = (int)strtol(optarg, nullptr, 0); break;
+            case 'v': verify = 0; break;
+            case 'd': device = (int)strtol(optarg, nullptr, 0); break;
+            default: usage(); return 1;
+        }
+    }
+    if (blockScale < 1 || blockScale > 31 || threadScale < 1 || threadScale > 31) {
+        printf("Bad input parameter : 1 <= BlockScale, ThreadScale <= 31.\n");
+        return 1;
+    }
+    if (hashFlag < 0 || hashFlag > 14) {
+        printf("Bad input parameter : -m must be in [0, 14] (m = 2 << M).\n");
+        return 1;
+    }
+    if (smallPrimeScale < 1 || smallPrimeScale > 4096) {
+        printf("Bad input parameter : -s must be in [1, 4096].\n");
+        return 1;
+    }
+    printf("BlockScale = %d\nThreadScale = %d\nLow = %llu\nHigh = %llu\nQ_In = %d\n", blockScale,
+           threadScale, low, high, Qin);
+    printf("HashTableSize = %d\nVerify = %d\nS = %d\nABCD = %s\nFactorFile = %s\n\n", hashFlag,
+           verify, smallPrimeScale, abcdFile, factorFile);
+    const long long bscale = 1ll << (blockScale - 1), tscale = 1ll << (threadScale - 1);
+    if (low == 0 && high == 0) {  // :821-841
+        low = 102297149770000ull;
+        high = low + 2500000ull * (unsigned long long)(bscale * tscale);
+    }
+    if (high < low) {
+        printf("Bad input parameters : P1 < P0. We want to sieve primes p in the range P0 <= p <= P1.\n");
+        return 1;
+    }
+    const long long arraySize = 128 * bscale * 128 * tscale;
+    if (arraySize > (1ll << 30)) {
+        printf("Bad input parameter : primes per launch must stay below 2^30.\n");
+        return 1;
+    }
+
+    rs_abcd* abcd = nullptr;
+    rs_status st = rs_abcd_read(abcdFile, &abcd);
+    if (st == RS_ERR_IO) {
+        printf("Unable to open file %s\n", abcdFile);
+        return 1;
+    }
+    if (st != RS_OK) return fail("reading the ABCD file", st);
+    uint32_t base;
+    int nk, nn, minN, maxN;
+    rs_abcd_info(abcd, &base, &nk, &nn, &minN, &maxN);
+    printf("The base is %u\n%d k-values, %d candidates\nMinN = %d and MaxN = %d\n", base, nk, nn,
+           minN, maxN);
+    rs_plan* plan = nullptr;
+    if ((st = rs_plan_create(abcd, Qin, &plan)) != RS_OK) return fail("planning", st);
+    int Q, nrows, words;
+    rs_plan_info(plan, nullptr, &Q, &nrows, &words, nullptr, nullptr);
+    const int m = hashFlag ? (2 << hashFlag) : 0;
+    if (m != 0 && m < 64) {
+        printf("Bad input parameter : this build needs m >= 64 baby steps (-m 5 or more).\n");
+        return 1;
+    }
+    printf("Q=%d. Subsequences= %d. Range = %d\n", Q, nrows, maxN / Q - minN / Q + 1);
+    if (low < 3) low = 3;
+    if (low % 2 == 0) low -= 1;  // :1515-1519
+    if (low <= base) {
+        printf("Bad input parameters : P0 must be larger than the base %u.\n", base);
+        return 1;
+    }
+
+    rs_sieve* sv = nullptr;
+    if ((st = rs_sieve_create(device, plan, m, arraySize, 1 << 20, &sv)) != RS_OK)
+        return fail("device set-up", st);
+    int mUsed, T, slots, ldsw;
+    rs_sieve_info(sv, &mUsed, &T, &slots, &ldsw);
+    printf("Each prime uses %d baby steps, %d giant steps per row, a %d-slot LDS table "
+           "(%d bytes of LDS per wavefront)\n\n", mUsed, T, slots, ldsw);
+    rs_primegen* gen = nullptr;
+    if ((st = rs_primegen_create(smallPrimeScale, high + 1, 0, &gen)) != RS_OK)
+        return fail("prime generator set-up", st);
+
+    const unsigned long long startLow = low;
+    std::vector<uint32_t> cur((size_t)arraySize), nxt((size_t)arraySize);
+    std::vector<rs_factor> hits(1 << 20);
+    // one batch: the first arraySize primes >= lo (lo odd), clipped to <= high
+    auto make_batch = [&](uint64_t lo, std::vector<uint32_t>& buf, long long* n,
+                          uint64_t* nextLow) -> rs_status {
+        rs_status s2 = rs_primes_batch(gen, lo, buf.data(), arraySize, nextLow);
+        if (s2 != RS_OK) return s2;
+        long long c = arraySize;
+        while (c > 0 && lo + buf[c - 1] > high) --c;
+        *n = c;
+        return RS_OK;
+    };
+    long long curN = 0, nxtN = 0;
+    uint64_t curLow = low, nextLow = 0;
+    if ((st = make_batch(curLow, cur, &curN, &nextLow)) != RS_OK) return fail("prime generation", st);
+    int kernelCount = 0;
+    long long factorCount = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    while (true) {
+        auto tb = std::chrono::steady_clock::now();
+        ++kernelCount;
+        if (curN > 0 && (st = rs_sieve_submit(sv, curLow, cur.data(), curN)) != RS_OK)
+            return fail("submit", st);
+        const unsigned long long progress = curN > 0 ? cur[curN - 1] - cur[0] : 0;
+        const bool more = nextLow <= high && nextLow > curLow;
+        const uint64_t nLow = nextLow;
+        if (more && (st = make_batch(nLow, nxt, &nxtN, &nextLow)) != RS_OK)
+            return fail("prime generation", st);
+        int64_t got = 0, lost = 0;
+        st = rs_sieve_collect(sv, hits.data(), (int64_t)hits.size(), &got, &lost);
+        if (st == RS_ERR_OVERFLOW) {
+            hits.resize((size_t)got);
+            st = rs_sieve_collect(sv, hits.data(), (int64_t)hits.size(), &got, &lost);
+        }
+        if (st != RS_OK) return fail("collect", st);
+        if (lost > 0) fprintf(stderr, "Warning: %" PRId64 " hits dropped (append buffer full)\n", lost);
+        FILE* ff = nullptr;
+        for (int64_t i = 0; i < got; ++i) {
+            const rs_factor& f = hits[i];
+            if (verify) {
+                if (!rs_plan_contains(plan, f.k, f.n)) continue;              // :1620-1647
+                if (!rs_verify_factor(f.p, f.k, base, f.n)) {
+                    fprintf(stderr, "Error: device reported %llu | %u*%u^%u-1, which fails "
+                            "verification\n", (unsigned long long)f.p, f.k, base, f.n);
+                    return 1;
+                }
+            }
+            char line[128];
+            rs_format_factor(&f, base, line, sizeof line);
+            fputs(line, stdout);
+            if (!ff) ff = fopen(factorFile, "a");
+            if (!ff) {
+                printf("Unable to open factor file %s\n", factorFile);
+                return 1;
+            }
+            fputs(line, ff);
+            ++factorCount;
+        }
+        if (ff) fclose(ff);
+        auto te = std::chrono::steady_clock::now();
+        const double dt = std::chrono::duration<double>(te - tb).count();
+        if (kernelCount % 20 == 0) {
+            const double done = 100.0 * (double)(nLow - startLow) / (double)(high - startLow + 1);
+            const double soFar = std::chrono::duration<double>(te - t0).count();
+            const double left = done > 0 ? soFar * (100.0 / done - 1.0) : 0.0;
+            time_t eta = time(nullptr) + (time_t)left;
+            char etaBuf[64];
+            strftime(etaBuf, sizeof etaBuf, "%a %b %d %H:%M:%S %Y", localtime(&eta));
+            printf("p=%llu, %.0f p/sec,  %.2f%% done, %lld factors found, ETA %s \n",
+                   (unsigned long long)nLow,
+                   progress / dt, done > 100 ? 100.0 : done, factorCount, etaBuf);
+            fflush(stdout);
+        }
+        if (!more) break;
+        std::swap(cur, nxt);
+        curN = nxtN;
+        curLow = nLow;
+    }
+    const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const unsigned long long swept = high + 1 - startLow;
+    printf("\nTime taken: %gs\n", total);
+    printf("Factors found: %lld\n", factorCount);
+    printf("Time per kernel: %g\n", total / kernelCount);
+    printf("Progress = %llu at %.0f p/sec\n\n", swept, swept / total);
+    rs_primegen_free(gen);
+    rs_sieve_destroy(sv);
+    rs_plan_free(plan);
+    rs_abcd_free(abcd);
+    return 0;
+}

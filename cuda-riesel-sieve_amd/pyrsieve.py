@@ -1,0 +1,382 @@
+"""ctypes binding of librsieve.so (include/rsieve.h) for tests and benchmarks.
+
+Every call goes through the C ABI; a missing library or a failing call raises
+(there is no fallback path).  The objects mirror the reference program's
+pieces: Abcd (the ABCD reader, kernel_v0.24.6.cu:113-137, :938-1083), Plan
+(the subsequence planner, :1086-1255), PrimeGen (the prime-array generator,
+:1838-1987) and Sieve (device set-up, launch and result collection,
+:1367-1611).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librsieve.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "rsieve.h")
+
+u64, u32, i32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int64
+vp = ctypes.c_void_p
+P = ctypes.POINTER
+
+RS_OK = 0
+RS_ERR_ARG = -1
+RS_ERR_HIP = -2
+RS_ERR_NOMEM = -3
+RS_ERR_IO = -4
+RS_ERR_FORMAT = -5
+RS_ERR_OVERFLOW = -6
+RS_ERR_BUDGET = -7
+RS_ERR_STATE = -8
+RS_ERR_RANGE = -9
+
+
+class Factor(ctypes.Structure):
+    _fields_ = [("p", u64), ("k", u32), ("n", u32)]
+
+
+FACTOR_DTYPE = np.dtype([("p", np.uint64), ("k", np.uint32), ("n", np.uint32)])
+
+_SIGS = {
+    "rs_strerror": (ctypes.c_char_p, [i32]),
+    "rs_abcd_read": (i32, [ctypes.c_char_p, P(vp)]),
+    "rs_abcd_from_lists": (i32, [u32, i32, P(i32), P(i32), P(i32), P(vp)]),
+    "rs_abcd_info": (i32, [vp, P(u32), P(i32), P(i32), P(i32), P(i32)]),
+    "rs_abcd_lists": (i32, [vp, P(i32), P(i32), P(i32)]),
+    "rs_abcd_write": (i32, [vp, ctypes.c_char_p]),
+    "rs_abcd_free": (None, [vp]),
+    "rs_abcd_synth": (i32, [u32, i32, u32, u32, i32, i32, i32, u32, u64, P(vp)]),
+    "rs_plan_create": (i32, [vp, i32, P(vp)]),
+    "rs_plan_info": (i32, [vp, P(u32), P(i32), P(i32), P(i32), P(i32), P(i32)]),
+    "rs_plan_rows": (i32, [vp, P(i32), P(i32), P(u32)]),
+    "rs_plan_contains": (i32, [vp, u32, u32]),
+    "rs_plan_tminmax": (i32, [vp, i32, P(i32), P(i32)]),
+    "rs_plan_auto_m": (i32, [vp]),
+    "rs_plan_free": (None, [vp]),
+    "rs_primegen_create": (i32, [i32, u64, i32, P(vp)]),
+    "rs_primes_range": (i32, [vp, u64, u64, P(u32), i64, P(i64)]),
+    "rs_primes_batch": (i32, [vp, u64, P(u32), i64, P(u64)]),
+    "rs_primegen_free": (None, [vp]),
+    "rs_verify_factor": (i32, [u64, u32, u32, u32]),
+    "rs_format_factor": (i32, [P(Factor), u32, ctypes.c_char_p, i32]),
+    "rs_sieve_create": (i32, [i32, vp, i32, i64, i64, P(vp)]),
+    "rs_sieve_info": (i32, [vp, P(i32), P(i32), P(i32), P(i32)]),
+    "rs_sieve_submit": (i32, [vp, u64, P(u32), i64]),
+    "rs_sieve_submit_device": (i32, [vp, u64, vp, i64]),
+    "rs_sieve_collect": (i32, [vp, P(Factor), i64, P(i64), P(i64)]),
+    "rs_sieve_sync": (i32, [vp]),
+    "rs_sieve_timing": (i32, [vp, P(ctypes.c_double), P(ctypes.c_double), P(i64)]),
+    "rs_sieve_destroy": (None, [vp]),
+    "rs_dev_montmul": (i32, [i32, P(u64), P(u64), P(u64), P(u64), i64]),
+    "rs_dev_halfmul": (i32, [i32, P(u32), P(u64), P(u64), P(u64), i64]),
+    "rs_dev_jacobi": (i32, [i32, P(u32), P(u64), P(i32), i64]),
+    "rs_dev_alloc": (i32, [i32, i64, P(vp)]),
+    "rs_dev_upload": (i32, [i32, vp, vp, i64]),
+    "rs_dev_free": (i32, [i32, vp]),
+    "rs_dev_mad_rate": (i32, [i32, i32, i32, P(ctypes.c_double)]),
+    "rs_device_count": (i32, []),
+}
+
+_lib = None
+
+
+class RsError(RuntimeError):
+    def __init__(self, what, status):
+        self.status = status
+        msg = lib().rs_strerror(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{what}: {msg} ({status})")
+
+
+def lib():
+    """Loads librsieve.so (built by __graft_entry__.build()); raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _chk(what, st):
+    if st != RS_OK:
+        raise RsError(what, st)
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(P(ct))
+
+
+def device_count():
+    return lib().rs_device_count()
+
+
+def strerror(st):
+    return lib().rs_strerror(st).decode()
+
+
+class Abcd:
+    """An ABCD sieve file image (base, k list, n lists)."""
+
+    h = vp()
+
+    def __init__(self, handle):
+        self.h = vp(handle)
+
+    @classmethod
+    def read(cls, path):
+        h = vp()
+        _chk(f"rs_abcd_read({path})", lib().rs_abcd_read(str(path).encode(), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_lists(cls, base, kn):
+        """kn: dict k -> increasing list of n (insertion order kept)."""
+        ks = np.array(list(kn.keys()), np.int32)
+        counts = [len(v) for v in kn.values()]
+        kstart = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        ns = np.array([n for v in kn.values() for n in v], np.int32)
+        h = vp()
+        _chk("rs_abcd_from_lists", lib().rs_abcd_from_lists(
+            base, len(ks), _p(ks, i32), _p(kstart, i32), _p(ns, i32), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def synth(cls, base, nk, kmin, kmax, nmin, nmax, presieve_bound, seed, odd_k=True):
+        h = vp()
+        _chk("rs_abcd_synth", lib().rs_abcd_synth(base, nk, kmin, kmax, int(odd_k), nmin, nmax,
+                                                  presieve_bound, seed, ctypes.byref(h)))
+        return cls(h.value)
+
+    def info(self):
+        b, nk, nn, lo, hi = u32(), i32(), i32(), i32(), i32()
+        _chk("rs_abcd_info", lib().rs_abcd_info(self.h, ctypes.byref(b), ctypes.byref(nk),
+                                                ctypes.byref(nn), ctypes.byref(lo),
+                                                ctypes.byref(hi)))
+        return dict(base=b.value, nk=nk.value, nn=nn.value, minN=lo.value, maxN=hi.value)
+
+    def lists(self):
+        inf = self.info()
+        ks = np.zeros(inf["nk"], np.int32)
+        kstart = np.zeros(inf["nk"] + 1, np.int32)
+        ns = np.zeros(inf["nn"], np.int32)
+        _chk("rs_abcd_lists", lib().rs_abcd_lists(self.h, _p(ks, i32), _p(kstart, i32),
+                                                  _p(ns, i32)))
+        return ks, kstart, ns
+
+    def write(self, path):
+        _chk("rs_abcd_write", lib().rs_abcd_write(self.h, str(path).encode()))
+
+    def __del__(self):
+        if _lib is not None and self.h:
+            _lib.rs_abcd_free(self.h)
+            self.h = vp()
+
+
+class Plan:
+    """Subsequence plan: Q, rows (k, r) sorted, bitmaps, n window."""
+
+    h = vp()
+
+    def __init__(self, abcd, Qin=0):
+        h = vp()
+        _chk("rs_plan_create", lib().rs_plan_create(abcd.h, Qin, ctypes.byref(h)))
+        self.h = h
+        b, Q, nr, w, lo, hi = u32(), i32(), i32(), i32(), i32(), i32()
+        _chk("rs_plan_info", lib().rs_plan_info(self.h, ctypes.byref(b), ctypes.byref(Q),
+                                                ctypes.byref(nr), ctypes.byref(w),
+                                                ctypes.byref(lo), ctypes.byref(hi)))
+        self.base, self.Q, self.nrows, self.words = b.value, Q.value, nr.value, w.value
+        self.minN, self.maxN = lo.value, hi.value
+
+    def rows(self):
+        rk = np.zeros(self.nrows, np.int32)
+        rr = np.zeros(self.nrows, np.int32)
+        bm = np.zeros(self.nrows * self.words, np.uint32)
+        _chk("rs_plan_rows", lib().rs_plan_rows(self.h, _p(rk, i32), _p(rr, i32), _p(bm, u32)))
+        return rk, rr, bm.reshape(self.nrows, self.words)
+
+    def contains(self, k, n):
+        return bool(lib().rs_plan_contains(self.h, k, n))
+
+    def tminmax(self, m):
+        a, b = i32(), i32()
+        _chk("rs_plan_tminmax", lib().rs_plan_tminmax(self.h, m, ctypes.byref(a),
+                                                      ctypes.byref(b)))
+        return a.value, b.value
+
+    def auto_m(self):
+        return lib().rs_plan_auto_m(self.h)
+
+    def __del__(self):
+        if _lib is not None and self.h:
+            _lib.rs_plan_free(self.h)
+            self.h = vp()
+
+
+class PrimeGen:
+    """True-prime generator (uint32 offsets from a uint64 base)."""
+
+    h = vp()
+
+    def __init__(self, hi, small_prime_scale=256, threads=0):
+        h = vp()
+        _chk("rs_primegen_create", lib().rs_primegen_create(small_prime_scale, hi, threads,
+                                                            ctypes.byref(h)))
+        self.h = h
+
+    def range(self, lo, hi):
+        n = i64()
+        est = int((hi - lo) / max(1.0, np.log(max(hi, 3)) - 1.2) * 1.1) + 1024
+        buf = np.zeros(est, np.uint32)
+        st = lib().rs_primes_range(self.h, lo, hi, _p(buf, u32), len(buf), ctypes.byref(n))
+        if st == RS_ERR_OVERFLOW:
+            buf = np.zeros(n.value, np.uint32)
+            st = lib().rs_primes_range(self.h, lo, hi, _p(buf, u32), len(buf), ctypes.byref(n))
+        _chk("rs_primes_range", st)
+        return buf[: n.value]
+
+    def batch(self, low, count):
+        buf = np.zeros(count, np.uint32)
+        nxt = u64()
+        _chk("rs_primes_batch", lib().rs_primes_batch(self.h, low, _p(buf, u32), count,
+                                                      ctypes.byref(nxt)))
+        return buf, nxt.value
+
+    def __del__(self):
+        if _lib is not None and self.h:
+            _lib.rs_primegen_free(self.h)
+            self.h = vp()
+
+
+def verify_factor(p, k, b, n):
+    return bool(lib().rs_verify_factor(p, k, b, n))
+
+
+def format_factor(p, k, b, n):
+    f = Factor(p, k, n)
+    buf = ctypes.create_string_buffer(128)
+    r = lib().rs_format_factor(ctypes.byref(f), b, buf, 128)
+    if r < 0:
+        raise RsError("rs_format_factor", r)
+    return buf.value.decode()
+
+
+class DeviceBuffer:
+    """Device memory holding uint32 prime offsets (HBM-resident input)."""
+
+    ptr = vp()
+
+    def __init__(self, device, host_array):
+        a = np.ascontiguousarray(host_array)
+        self.device, self.nbytes, self.count = device, a.nbytes, a.size
+        self.ptr = vp()
+        _chk("rs_dev_alloc", lib().rs_dev_alloc(device, max(a.nbytes, 4), ctypes.byref(self.ptr)))
+        if a.nbytes:
+            _chk("rs_dev_upload", lib().rs_dev_upload(device, self.ptr, a.ctypes.data_as(vp),
+                                                      a.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().rs_dev_free(self.device, self.ptr)
+            self.ptr = vp()
+
+    def __del__(self):
+        if _lib is not None and self.ptr:
+            self.free()
+
+
+class Sieve:
+    """A sieve job resident on one GPU."""
+
+    h = vp()
+
+    def __init__(self, plan, m=0, max_batch=1 << 20, hit_capacity=1 << 20, device=0):
+        h = vp()
+        _chk("rs_sieve_create", lib().rs_sieve_create(device, plan.h, m, max_batch, hit_capacity,
+                                                      ctypes.byref(h)))
+        self.h = h
+        self.plan = plan
+        mm, T, slots, lds = i32(), i32(), i32(), i32()
+        _chk("rs_sieve_info", lib().rs_sieve_info(self.h, ctypes.byref(mm), ctypes.byref(T),
+                                                  ctypes.byref(slots), ctypes.byref(lds)))
+        self.m, self.T, self.slots, self.lds_bytes = mm.value, T.value, slots.value, lds.value
+
+    def submit(self, low, offsets):
+        offs = np.ascontiguousarray(offsets, np.uint32)
+        _chk("rs_sieve_submit", lib().rs_sieve_submit(self.h, low, _p(offs, u32), len(offs)))
+
+    def submit_device(self, low, dbuf, start=0, count=None):
+        count = dbuf.count - start if count is None else count
+        ptr = vp(dbuf.ptr.value + 4 * start)
+        _chk("rs_sieve_submit_device", lib().rs_sieve_submit_device(self.h, low, ptr, count))
+
+    def collect(self):
+        """All hits since the last collect as a sorted list of (p, k, n)."""
+        cnt, lost = i64(), i64()
+        cap = 4096
+        while True:
+            buf = (Factor * cap)()
+            st = lib().rs_sieve_collect(self.h, buf, cap, ctypes.byref(cnt), ctypes.byref(lost))
+            if st == RS_ERR_OVERFLOW:
+                cap = cnt.value
+                continue
+            _chk("rs_sieve_collect", st)
+            if lost.value:
+                raise RuntimeError(f"device hit buffer overflowed: {lost.value} hits lost")
+            return [(int(f.p), int(f.k), int(f.n)) for f in buf[: cnt.value]]
+
+    def sync(self):
+        _chk("rs_sieve_sync", lib().rs_sieve_sync(self.h))
+
+    def timing(self):
+        a, b, n = ctypes.c_double(), ctypes.c_double(), i64()
+        _chk("rs_sieve_timing", lib().rs_sieve_timing(self.h, ctypes.byref(a), ctypes.byref(b),
+                                                      ctypes.byref(n)))
+        return dict(bsgs_ms=a.value, prep_ms=b.value, primes=n.value)
+
+    def close(self):
+        if self.h:
+            lib().rs_sieve_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        if _lib is not None and self.h:
+            self.close()
+
+
+def dev_montmul(a, b, p, device=0):
+    a, b, p = (np.ascontiguousarray(x, np.uint64) for x in (a, b, p))
+    out = np.zeros_like(a)
+    _chk("rs_dev_montmul", lib().rs_dev_montmul(device, _p(a, u64), _p(b, u64), _p(p, u64),
+                                                _p(out, u64), len(a)))
+    return out
+
+
+def dev_halfmul(k, y, p, device=0):
+    k = np.ascontiguousarray(k, np.uint32)
+    y, p = (np.ascontiguousarray(x, np.uint64) for x in (y, p))
+    out = np.zeros_like(y)
+    _chk("rs_dev_halfmul", lib().rs_dev_halfmul(device, _p(k, u32), _p(y, u64), _p(p, u64),
+                                                _p(out, u64), len(k)))
+    return out
+
+
+def dev_jacobi(a, p, device=0):
+    a = np.ascontiguousarray(a, np.uint32)
+    p = np.ascontiguousarray(p, np.uint64)
+    out = np.zeros(len(a), np.int32)
+    _chk("rs_dev_jacobi", lib().rs_dev_jacobi(device, _p(a, u32), _p(p, u64), _p(out, i32),
+                                              len(a)))
+    return out
+
+
+def dev_mad_rate(blocks=8192, iters=4096, device=0):
+    r = ctypes.c_double()
+    _chk("rs_dev_mad_rate", lib().rs_dev_mad_rate(device, blocks, iters, ctypes.byref(r)))
+    return r.value

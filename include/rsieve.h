@@ -1,0 +1,202 @@
+/* rsieve.h -- C ABI of the MI355X (gfx950) Riesel BSGS sieve, librsieve.so.
+ *
+ * Sieves candidates k*b^n-1: for every prime p of a chunk it finds every
+ * (k, n) with n in [minN, maxN], (k, n mod Q) a planned row and
+ * k*b^n == 1 (mod p).  Plain integers and pointers only; every call returns
+ * an rs_status (0 = RS_OK, negative = error) and never throws.  Handles are
+ * opaque; one caller thread per handle.
+ *
+ * Reference interface each group replaces (rob147147/CUDA-Riesel-Sieve
+ * v0.24.6, file "CUDA Riesel Sieve/kernel_v0.24.6.cu" unless noted):
+ *   ABCD input ........ countKs :113-137 (FileInput.cpp:12-36, FileInput.h:1)
+ *                       and the parser in main :938-1083
+ *   planner ........... Q choice :1086-1146, tMin/tMax :1148-1158,
+ *                       row matrix + bitmaps :1160-1209, row sort :1219-1255
+ *   prime generator ... small primes :1271-1303, wheel masks :1306-1364,
+ *                       generateGPUPrimes :1838-1987
+ *   device sieve ...... device set-up :1367-1512 (cudaMalloc/cudaMemcpy/
+ *                       cudaMemcpyToSymbol of base, tMin, tMax, Q, m, rows),
+ *                       per-batch launch :1537-1598 (lowGPU, KernelP copy,
+ *                       memsets, addKernel1<<<>>> at :1578),
+ *                       result copy + scan :1599-1611
+ *   verification ...... bitmap lookup + __int128 modpow :1612-1719,
+ *                       writeFactorToFile :1830-1835
+ */
+#ifndef RSIEVE_H
+#define RSIEVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int rs_status;
+#define RS_OK 0
+#define RS_ERR_ARG (-1)      /* bad argument or unsupported value */
+#define RS_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define RS_ERR_NOMEM (-3)    /* host or device allocation failed */
+#define RS_ERR_IO (-4)       /* file cannot be opened or written */
+#define RS_ERR_FORMAT (-5)   /* malformed ABCD input */
+#define RS_ERR_OVERFLOW (-6) /* a caller buffer is too small (count says how big) */
+#define RS_ERR_BUDGET (-7)   /* serving: job does not fit the memory budget */
+#define RS_ERR_STATE (-8)    /* call out of order (e.g. collect with nothing submitted) */
+#define RS_ERR_RANGE (-9)    /* prime window has too few primes (reference exit(1), :1975-1979) */
+
+/* Text for a status code (static string). */
+const char* rs_strerror(rs_status s);
+
+/* One factor: p | k*b^n-1. */
+typedef struct rs_factor {
+    uint64_t p;
+    uint32_t k;
+    uint32_t n;
+} rs_factor;
+
+/* ------------------------------------------------------------ ABCD input */
+/* Replaces countKs (:113-137) + the parser (:938-1083): a line whose first
+ * space-separated token is "ABCD" starts a k ("ABCD k*b^$a-1 [n0]"), the
+ * base comes from the first header, other lines are positive n deltas. */
+typedef struct rs_abcd rs_abcd;
+
+rs_status rs_abcd_read(const char* path, rs_abcd** out);
+/* Builds a file image from explicit lists: ks[nk], kstart[nk+1] CSR offsets
+ * into ns (strictly increasing per k). */
+rs_status rs_abcd_from_lists(uint32_t base, int nk, const int32_t* ks,
+                             const int32_t* kstart, const int32_t* ns,
+                             rs_abcd** out);
+rs_status rs_abcd_info(const rs_abcd* a, uint32_t* base, int* nk, int* nn,
+                       int* minN, int* maxN);
+/* Copies the lists out (ks[nk], kstart[nk+1], ns[nn]). */
+rs_status rs_abcd_lists(const rs_abcd* a, int32_t* ks, int32_t* kstart,
+                        int32_t* ns);
+/* Writes the ABCD text form (srsieve layout, one header per k). */
+rs_status rs_abcd_write(const rs_abcd* a, const char* path);
+void rs_abcd_free(rs_abcd* a);
+
+/* Synthetic input (no reference counterpart: the reference only reads ABCD
+ * made by srsieve, sr_745.abcd:1).  Draws nk distinct k uniformly from
+ * [kmin, kmax] (odd only if odd_k) with a seeded splitmix64 stream, keeps
+ * n in [nmin, nmax] such that no odd prime q < presieve_bound with q ∤ b
+ * divides k*b^n-1, and redraws a k whose list comes out empty. */
+rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_t kmax,
+                        int odd_k, int nmin, int nmax, uint32_t presieve_bound,
+                        uint64_t seed, rs_abcd** out);
+
+/* ------------------------------------------------------------ planner */
+/* Replaces :1086-1255.  Q is the even value in [2, 24) (or Qin, even) with
+ * the least rows*R work, R = maxN/Q - minN/Q + 1; rows are the distinct
+ * (k, n mod Q) sorted by (k, r), each with an MSB-first bitmap over
+ * x' = (n-r)/Q - floor(minN/Q).  Unlike :1093, Q never falls back to 1. */
+typedef struct rs_plan rs_plan;
+
+rs_status rs_plan_create(const rs_abcd* a, int Qin, rs_plan** out);
+rs_status rs_plan_info(const rs_plan* pl, uint32_t* base, int* Q, int* nrows,
+                       int* words, int* minN, int* maxN);
+/* rows_k[nrows], rows_r[nrows], bitmaps[nrows*words] (any may be NULL). */
+rs_status rs_plan_rows(const rs_plan* pl, int32_t* rows_k, int32_t* rows_r,
+                       uint32_t* bitmaps);
+/* 1 if n is in k's list (the bitmap test of :1620-1647), else 0. */
+int rs_plan_contains(const rs_plan* pl, uint32_t k, uint32_t n);
+/* tMin/tMax of :1148-1158 for m = 2^s (reported for the -m table). */
+rs_status rs_plan_tminmax(const rs_plan* pl, int m, int* tmin, int* tmax);
+/* Table size m picked by the cost model for this plan (multiple of 64). */
+int rs_plan_auto_m(const rs_plan* pl);
+void rs_plan_free(rs_plan* pl);
+
+/* ------------------------------------------------------------ primes */
+/* True primes (the reference's generator is a partial sieve, :1274 and
+ * :1890, whose survivors are composite past (largest small prime)^2).
+ * small_prime_scale keeps the meaning of -s: at least 4096*s sieving primes
+ * are used; more are added up to sqrt(hi), and survivors above the sieving
+ * bound squared are proved by deterministic Miller-Rabin.  threads <= 0
+ * uses the hardware count. */
+typedef struct rs_primegen rs_primegen;
+
+rs_status rs_primegen_create(int small_prime_scale, uint64_t hi, int threads,
+                             rs_primegen** out);
+/* Every prime in [lo, hi) as offsets p - lo (hi - lo <= 2^32).  If cap is
+ * too small returns RS_ERR_OVERFLOW with *count = the number needed. */
+rs_status rs_primes_range(rs_primegen* g, uint64_t lo, uint64_t hi,
+                          uint32_t* offsets, int64_t cap, int64_t* count);
+/* generateGPUPrimes (:1838-1987) replacement: the first count primes >= low
+ * (low made odd as at :1515-1519) as offsets from low, plus the next low
+ * (last prime + 2, :1561).  RS_ERR_RANGE if 2^32 offsets do not hold count. */
+rs_status rs_primes_batch(rs_primegen* g, uint64_t low, uint32_t* offsets,
+                          int64_t count, uint64_t* next_low);
+void rs_primegen_free(rs_primegen* g);
+
+/* ------------------------------------------------------------ verification */
+/* 1 if p | k*b^n - 1 (128-bit square-and-multiply, :1656-1671). */
+int rs_verify_factor(uint64_t p, uint32_t k, uint32_t b, uint32_t n);
+/* Formats "p | k*b^n-1\n" (writeFactorToFile, :1833) into buf; returns the
+ * length or a negative status. */
+int rs_format_factor(const rs_factor* f, uint32_t b, char* buf, int cap);
+
+/* ------------------------------------------------------------ device sieve */
+/* One sieve job on one GPU: base, Q, table size m (multiple of 64, or 0 for
+ * the cost model's pick), rows and n window from a plan.  Replaces the
+ * set-up at :1367-1512.  max_batch bounds the primes of one launch (the
+ * reference's arraySize, :851); hit_capacity bounds the device append
+ * buffer (the reference keeps one slot per prime, :615-621). */
+typedef struct rs_sieve rs_sieve;
+
+rs_status rs_sieve_create(int device, const rs_plan* pl, int m,
+                          int64_t max_batch, int64_t hit_capacity,
+                          rs_sieve** out);
+rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
+                        int* lds_bytes_per_wave);
+/* Primes p = low + offsets[i] from HOST memory (p odd, 3 <= p < 2^63, p ∤ b).
+ * Copied through pinned double buffers on a copy stream, then sieved on the
+ * compute stream; returns once the copy of this call's data is issued.
+ * Replaces :1543-1578. */
+rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* offsets,
+                          int64_t count);
+/* Same, with offsets already in device memory (HBM-resident input). */
+rs_status rs_sieve_submit_device(rs_sieve* s, uint64_t low,
+                                 const uint32_t* offsets_dev, int64_t count);
+/* Waits for everything submitted and returns all hits found since the last
+ * collect, sorted by (p, k, n).  If cap is too small: RS_ERR_OVERFLOW with
+ * *count = hits held (kept for the next call).  *lost (optional) > 0 means
+ * the device append buffer overflowed and hits were dropped (raise
+ * hit_capacity).  Replaces :1596-1611. */
+rs_status rs_sieve_collect(rs_sieve* s, rs_factor* out, int64_t cap,
+                           int64_t* count, int64_t* lost);
+/* Waits for all submitted work. */
+rs_status rs_sieve_sync(rs_sieve* s);
+/* Device time of the sieve kernels since the last call, in ms, and the
+ * number of primes processed (from HIP events on the compute stream). */
+rs_status rs_sieve_timing(rs_sieve* s, double* bsgs_ms, double* prep_ms,
+                          int64_t* primes);
+void rs_sieve_destroy(rs_sieve* s);
+
+/* Device-side helpers for tests (exact arithmetic on the GPU): out[i] =
+ * montmul(a[i], b[i]) mod p[i] (R = 2^64) and out[i] = k[i]*y[i]*2^-32
+ * mod p[i]; count elements, host buffers. */
+rs_status rs_dev_montmul(int device, const uint64_t* a, const uint64_t* b,
+                         const uint64_t* p, uint64_t* out, int64_t count);
+rs_status rs_dev_halfmul(int device, const uint32_t* k, const uint64_t* y,
+                         const uint64_t* p, uint64_t* out, int64_t count);
+/* out[i] = Jacobi (a[i] / p[i]) in {-1, 0, 1} computed on the device. */
+rs_status rs_dev_jacobi(int device, const uint32_t* a, const uint64_t* p,
+                        int32_t* out, int64_t count);
+
+/* Device buffers for HBM-resident inputs (plain hipMalloc/hipMemcpy; no
+ * reference counterpart beyond the cudaMalloc/cudaMemcpy of :1387-1512). */
+rs_status rs_dev_alloc(int device, int64_t bytes, void** ptr);
+rs_status rs_dev_upload(int device, void* dst, const void* src, int64_t bytes);
+rs_status rs_dev_free(int device, void* ptr);
+
+/* Measured rate of the 32x32+64 -> 64 multiply-add (v_mad_u64_u32) on the
+ * device, in lane-operations per second, from a dependent-chain kernel of
+ * `iters` mads per lane over `blocks` x 256 lanes. */
+rs_status rs_dev_mad_rate(int device, int blocks, int iters, double* ops_per_s);
+
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int rs_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSIEVE_H */

@@ -1,0 +1,281 @@
+"""Parity of the HIP sieve (through the C ABI) with the CPU oracle.
+
+Bar: bit-exact.  Every case compares the sorted (p, k, n) hit list of the
+device with the oracle's BSGS restatement and, for tiny inputs, with an
+exhaustive search; every device factor is re-verified by modpow.
+"""
+import os
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import R745, ROOT
+
+pytestmark = pytest.mark.gpu
+
+KNOWN_P = 102297149781479  # divides 18300*745^242670-1 (datless: n not in sr_745.abcd)
+
+
+def py_montmul(a, b, p):
+    return a * b * pow(2, -64, p) % p
+
+
+def is_prime(n):
+    if n < 2:
+        return False
+    for q in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % q == 0:
+            return n == q
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def run_device(rs, plan, low, offs, m=0, max_batch=1 << 16, device=0):
+    sv = rs.Sieve(plan, m=m, max_batch=max_batch, device=device)
+    sv.submit(low, offs)
+    hits = sv.collect()
+    sv.close()
+    return hits
+
+
+def run_oracle(orc, plan, m, low, offs):
+    rk, rr, _ = plan.rows()
+    job = orc.Job(plan.base, plan.Q, m, plan.minN, plan.maxN, rk, rr)
+    hits, _ = orc.bsgs_many(job, low, offs, threads=16)
+    return hits
+
+
+# ------------------------------------------------------------ arithmetic
+
+
+def test_dev_montmul_random_and_edges(rs, gpu):
+    rng = random.Random(1)
+    a, b, p = [], [], []
+    for bits in (3, 20, 31, 32, 33, 40, 50, 61, 62, 63):
+        for _ in range(300):
+            q = rng.getrandbits(bits) | 1 | (1 << (bits - 1))
+            q = max(q, 3)
+            if q >= 1 << 63:
+                q = (1 << 63) - 25
+            a.append(rng.randrange(q))
+            b.append(rng.randrange(q))
+            p.append(q)
+    # special operands: 0, 1, p-1, a >= p (unreduced), p near 2^63
+    for q in ((1 << 63) - 25, (1 << 62) + 135, 4294967311, 3, 5):
+        for x, y in ((0, 1), (1, 1), (q - 1, q - 1), (1 << 32, q - 1), ((1 << 64) - 1, q - 1)):
+            a.append(x % (1 << 64))
+            b.append(y % q)
+            p.append(q)
+    # the reference's lost-carry case: bits 32..63 of a*b all ones
+    for _ in range(500):
+        q = (1 << 63) - 2 * rng.randrange(1, 1 << 40) - 1
+        bb = rng.randrange(2, 1 << 31)
+        h = rng.randrange(1, bb)
+        aa = ((h << 64) + (1 << 64) - 1) // bb
+        assert aa < 1 << 64
+        assert ((aa * bb) >> 32) & 0xFFFFFFFF == 0xFFFFFFFF
+        a.append(aa)
+        b.append(bb)
+        p.append(q)
+    got = rs.dev_montmul(a, b, p)
+    want = [py_montmul(x, y, q) for x, y, q in zip(a, b, p)]
+    assert [int(v) for v in got] == want
+
+
+def test_dev_halfmul(rs, gpu):
+    rng = random.Random(2)
+    k, y, p = [], [], []
+    for bits in (5, 32, 33, 45, 62, 63):
+        for _ in range(500):
+            q = max(3, rng.getrandbits(bits) | 1 | (1 << (bits - 1)))
+            q = min(q, (1 << 63) - 25)
+            k.append(rng.getrandbits(32))
+            y.append(rng.randrange(q))
+            p.append(q)
+    for q in ((1 << 63) - 25, 3):
+        k += [0, 1, (1 << 32) - 1]
+        y += [q - 1, q - 1, q - 1]
+        p += [q, q, q]
+    got = rs.dev_halfmul(k, y, p)
+    want = [kk * yy * pow(2, -32, q) % q for kk, yy, q in zip(k, y, p)]
+    assert [int(v) for v in got] == want
+
+
+def test_dev_jacobi_vs_euler(rs, gpu):
+    rng = random.Random(3)
+    ps = [3, 5, 7, 11, 13, 101, 65537, 4294967311, 1000000007]
+    while len(ps) < 60:
+        q = rng.getrandbits(rng.choice((33, 40, 50, 62))) | 1
+        if is_prime(q):
+            ps.append(q)
+    a, p = [], []
+    for q in ps:
+        for x in [0, 1, 2, 4, 8, 3, 745, 18300, 29586, (1 << 32) - 1] + [
+                rng.getrandbits(32) for _ in range(40)]:
+            a.append(x)
+            p.append(q)
+    got = rs.dev_jacobi(a, p)
+    for x, q, j in zip(a, p, got):
+        e = pow(x, (q - 1) // 2, q)
+        want = 0 if x % q == 0 else (1 if e == 1 else -1)
+        assert j == want, (x, q)
+
+
+# ------------------------------------------------------------ BSGS parity
+
+
+def tiny_job(rs, base, kn):
+    return rs.Plan(rs.Abcd.from_lists(base, kn))
+
+
+def test_tiny_primes_vs_exhaustive(rs, orc, gpu):
+    """Small p (orders of b^Q below m, p | k, p | b) against brute force."""
+    plan = tiny_job(rs, 6, {5: list(range(100, 700, 3)), 11: list(range(101, 650, 2)),
+                            12: [100, 150, 200, 640], 35: list(range(300, 700))})
+    primes = [q for q in range(5, 60000, 2) if is_prime(q)]
+    low = 3
+    offs = np.array([q - low for q in primes], np.uint32)
+    rk, rr, _ = plan.rows()
+    for m in (64, 192, 1024):
+        got = run_device(rs, plan, low, offs, m=m)
+        job = orc.Job(plan.base, plan.Q, m, plan.minN, plan.maxN, rk, rr)
+        brute = []
+        for q in primes:
+            brute += orc.brute_prime(job, q)
+        assert got == brute, f"m={m}"
+    # independent Python check on the smallest primes
+    small = [h for h in got if h[0] < 400]
+    rows = set(zip(rk.tolist(), rr.tolist()))
+    want = []
+    for q in primes:
+        if q >= 400:
+            break
+        for k, r in sorted(rows):
+            for n in range(plan.minN, plan.maxN + 1):
+                if n % plan.Q == r and k * pow(plan.base, n, q) % q == 1:
+                    want.append((q, k, n))
+    assert small == sorted(want)
+
+
+def test_config1_full(rs, orc, gpu):
+    """b=745, 4 odd k in [3, 1e6], n in [10000, 20000]
+    presieved by odd primes < 1000, p = all primes in [1e9, 1e9 + 1e7]."""
+    abcd = rs.Abcd.synth(745, 4, 3, 10**6, 10000, 20000, 1000, seed=1)
+    plan = rs.Plan(abcd)
+    low = 10**9
+    offs = rs.PrimeGen(low + 10**7).range(low, low + 10**7 + 1)
+    assert len(offs) == 482449
+    for m in (0, 512):
+        got = run_device(rs, plan, low, offs, m=m, max_batch=1 << 17)
+        mm = m or plan.auto_m()
+        want = run_oracle(orc, plan, mm, low, offs)
+        assert got == want
+    for p, k, n in got:
+        assert orc.verify(p, k, plan.base, n)
+    assert len(got) > 0
+
+
+def test_r745_known_answer_and_window(rs, orc, gpu):
+    """sr_745.abcd on the reference's default test range (:837-839)."""
+    plan = rs.Plan(rs.Abcd.read(R745))
+    low = 102297149770000 + 1
+    offs = rs.PrimeGen(low + 2500000).range(low, low + 2500000)
+    for m in (512, 0, 2048):
+        got = run_device(rs, plan, low, offs, m=m)
+        assert (KNOWN_P, 18300, 242670) in got
+        want = run_oracle(orc, plan, m or plan.auto_m(), low, offs)
+        assert got == want
+    # with verification (dat mode) the datless hit is dropped
+    assert not plan.contains(18300, 242670)
+
+
+def test_r745_shaped_window(rs, orc, gpu):
+    """BASELINE config 2 shape: 22 odd k in [3, 1e7], n in [180000, 250000]
+    presieved by odd primes < 1e5, a 2^22 window of p at 1e12."""
+    abcd = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2)
+    plan = rs.Plan(abcd)
+    low = 10**12 + 1
+    offs = rs.PrimeGen(low + 2**22).range(low, low + 2**22)
+    got = run_device(rs, plan, low, offs, m=0, max_batch=1 << 15)
+    want = run_oracle(orc, plan, plan.auto_m(), low, offs)
+    assert got == want
+
+
+def test_large_p_near_2_62_and_2_63(rs, orc, gpu):
+    plan = rs.Plan(rs.Abcd.synth(2, 12, 3, 10**9, 1000000, 1003000, 1000, seed=3))
+    for low in ((1 << 62) - (1 << 18), (1 << 63) - (1 << 19)):
+        offs = rs.PrimeGen(low + (1 << 18), threads=16).range(low, low + (1 << 18))
+        assert all(is_prime(low + int(o)) for o in offs[:50])
+        got = run_device(rs, plan, low, offs, m=1024)
+        want = run_oracle(orc, plan, 1024, low, offs)
+        assert got == want
+
+
+def test_many_hits_small_p_datless(rs, orc, gpu):
+    """p near 1e6: many hits per prime, exercises the append buffer."""
+    plan = rs.Plan(rs.Abcd.synth(5, 10, 3, 10**5, 2000, 60000, 200, seed=4, odd_k=False))
+    low = 10**6 + 1
+    offs = rs.PrimeGen(low + 10**5).range(low, low + 10**5)
+    got = run_device(rs, plan, low, offs, m=256)
+    want = run_oracle(orc, plan, 256, low, offs)
+    assert got == want and len(got) > 1000
+
+
+def test_device_resident_and_repeat(rs, gpu):
+    plan = rs.Plan(rs.Abcd.synth(745, 8, 3, 10**6, 50000, 80000, 1000, seed=5))
+    low = 10**11 + 1
+    offs = rs.PrimeGen(low + 2**21).range(low, low + 2**21)
+    a = run_device(rs, plan, low, offs, m=384, max_batch=5000)
+    sv = rs.Sieve(plan, m=384, max_batch=1 << 14)
+    buf = rs.DeviceBuffer(0, offs)
+    sv.submit_device(low, buf)
+    b = sv.collect()
+    sv.submit_device(low, buf, start=0, count=len(offs) // 2)
+    sv.submit_device(low, buf, start=len(offs) // 2)
+    c = sv.collect()
+    t = sv.timing()
+    sv.close()
+    buf.free()
+    assert a == b == c
+    assert t["primes"] == 2 * len(offs) and t["bsgs_ms"] > 0
+
+
+def test_hit_buffer_overflow_is_reported(rs, gpu):
+    plan = rs.Plan(rs.Abcd.synth(5, 10, 3, 10**5, 2000, 60000, 200, seed=4, odd_k=False))
+    low = 10**6 + 1
+    offs = rs.PrimeGen(low + 10**5).range(low, low + 10**5)
+    sv = rs.Sieve(plan, m=256, hit_capacity=16)
+    sv.submit(low, offs)
+    with pytest.raises(RuntimeError, match="overflowed"):
+        sv.collect()
+    sv.close()
+
+
+def test_cli_default_range_known_answer(tmp_path, gpu):
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+    out = tmp_path / "f.txt"
+    r = subprocess.run([exe, "-i", R745, "-O", str(out), "-v", "-m", "8"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    line = f"{KNOWN_P} | 18300*745^242670-1"
+    assert line in r.stdout and line in out.read_text()
+    out2 = tmp_path / "g.txt"
+    r = subprocess.run([exe, "-i", R745, "-O", str(out2), "-m", "8"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert line not in r.stdout

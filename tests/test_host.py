@@ -1,0 +1,170 @@
+"""Host side of librsieve (reader, planner, prime generator, synthetic
+input, verification, formats, error paths) against the oracle, through the
+C ABI.  CPU only."""
+import os
+import random
+import re
+
+import numpy as np
+import pytest
+
+from conftest import R745
+
+KNOWN_P = 102297149781479
+
+
+def test_reader_matches_oracle(rs, orc):
+    a = rs.Abcd.read(R745)
+    d = orc.read_abcd(R745)
+    info = a.info()
+    assert (info["base"], info["nk"], info["nn"], info["minN"], info["maxN"]) == (
+        745, 22, 65111, 182644, 250000)
+    ks, kstart, ns = a.lists()
+    assert np.array_equal(ks, d["ks"]) and np.array_equal(kstart, d["kstart"])
+    assert np.array_equal(ns, d["ns"])
+
+
+def test_planner_matches_oracle(rs, orc):
+    a = rs.Abcd.read(R745)
+    d = orc.read_abcd(R745)
+    for Qin in (0, 2, 6, 18, 22):
+        pl = rs.Plan(a, Qin)
+        op = orc.plan(d["ks"], d["kstart"], d["ns"], d["minN"], d["maxN"], Qin)
+        rk, rr, bm = pl.rows()
+        assert pl.Q == op["Q"]
+        assert np.array_equal(rk, op["rows_k"]) and np.array_equal(rr, op["rows_r"])
+        assert np.array_equal(bm, op["bitmaps"])
+        for m in (32, 512, 2048, 16384):
+            assert pl.tminmax(m) == orc.tminmax(d["minN"], d["maxN"], pl.Q, m)
+
+
+def test_plan_contains_is_the_file(rs, orc):
+    a = rs.Abcd.read(R745)
+    pl = rs.Plan(a)
+    ks, kstart, ns = a.lists()
+    rng = random.Random(1)
+    for i in range(len(ks)):
+        lst = set(ns[kstart[i]:kstart[i + 1]].tolist())
+        for n in rng.sample(sorted(lst), 50):
+            assert pl.contains(int(ks[i]), n)
+        for n in rng.sample(range(182644, 250001), 200):
+            assert pl.contains(int(ks[i]), n) == (n in lst)
+    assert not pl.contains(18300, 242670)
+    assert not pl.contains(1161, 182711)
+    assert not pl.contains(1160, 182643) and not pl.contains(1160, 250001)
+
+
+def test_writer_round_trip(rs, tmp_path):
+    a = rs.Abcd.read(R745)
+    out = tmp_path / "r.abcd"
+    a.write(out)
+    b = rs.Abcd.read(out)
+    for x, y in zip(a.lists(), b.lists()):
+        assert np.array_equal(x, y)
+    first = out.read_text().splitlines()[0]
+    assert first == "ABCD 1160*745^$a-1 [182711]"
+
+
+def test_reader_errors(rs, tmp_path):
+    with pytest.raises(rs.RsError) as e:
+        rs.Abcd.read(tmp_path / "missing.abcd")
+    assert e.value.status == rs.RS_ERR_IO
+    bad = tmp_path / "bad.abcd"
+    for text in ("12\n", "ABCD 5*2^$a-1 [10]\nxyz\n", "ABCD 5*2^$a-1\n",
+                 "ABCD 5*2^$a+1 [10]\n3\n", "ABCD 5*2^$a-1 [10]\n0\n"):
+        bad.write_text(text)
+        with pytest.raises(rs.RsError) as e:
+            rs.Abcd.read(bad)
+        assert e.value.status == rs.RS_ERR_FORMAT, text
+
+
+def test_plan_rejects_odd_q(rs):
+    a = rs.Abcd.read(R745)
+    with pytest.raises(rs.RsError):
+        rs.Plan(a, 3)
+
+
+def test_primes_range_matches_oracle(rs, orc):
+    g = rs.PrimeGen(10**12 + 2**24, threads=4)
+    for lo, hi in ((0, 100000), (1, 3), (2, 3), (999, 1000), (10**9 + 7, 10**9 + 2**20),
+                   (10**12, 10**12 + 2**22)):
+        got = g.range(lo, hi)
+        want = orc.true_primes(lo, hi)
+        assert np.array_equal(got, want), (lo, hi)
+
+
+def test_primes_batch_is_reference_layout(rs, orc):
+    g = rs.PrimeGen(10**12 + 2**24, threads=4)
+    low = 10**12 + 1
+    offs, nxt = g.batch(low, 50000)
+    want = orc.true_primes(low, low + 4 * 10**6)[:50000]
+    assert np.array_equal(offs, want)
+    assert nxt == low + int(offs[-1]) + 2
+    # even low is moved down to the odd below it (kernel_v0.24.6.cu:1515-1519)
+    offs2, _ = g.batch(low + 1, 10)
+    assert np.array_equal(offs2, offs[:10])
+
+
+def test_primes_equal_reference_generator_below_qmax_squared(rs, orc):
+    s = 4
+    low = 10**9 + 1
+    g = rs.PrimeGen(low + 10**6, small_prime_scale=s)
+    ours, _ = g.batch(low, 4096)
+    ref = orc.ref_primes(low, s, 4096, 4096 * orc.test_array_factor(s))
+    assert np.array_equal(ours, ref)
+
+
+def test_primes_true_above_qmax_squared(rs):
+    low = (1 << 62) + 1
+    g = rs.PrimeGen(low + 2**16, small_prime_scale=4, threads=4)
+    offs = g.range(low, low + 2**16)
+    vals = [low + int(o) for o in offs]
+    for v in vals:
+        assert all(pow(a, v - 1, v) == 1 for a in (2, 3, 5, 7))
+    # and nothing prime is missing: check every odd number in a slice
+    for v in range(low, low + 4000, 2):
+        isp = all(pow(a, v - 1, v) == 1 for a in (2, 3, 5, 7, 11, 13))
+        assert (v in set(vals)) == isp
+
+
+def test_synth_presieve_exact(rs):
+    a = rs.Abcd.synth(745, 5, 3, 10**6, 10000, 12000, 1000, seed=11)
+    ks, kstart, ns = a.lists()
+    qs = [q for q in range(3, 1000, 2) if all(q % d for d in range(3, int(q**0.5) + 1, 2))
+          and 745 % q]
+    assert len(ks) == 5 and np.all(ks % 2 == 1) and np.all(np.diff(ks) > 0)
+    for i, k in enumerate(ks.tolist()):
+        kept = set(ns[kstart[i]:kstart[i + 1]].tolist())
+        assert kept
+        for n in range(10000, 12001):
+            survives = all((k * pow(745, n, q) - 1) % q for q in qs)
+            assert (n in kept) == survives, (k, n)
+
+
+def test_synth_is_seeded(rs):
+    a = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2)
+    b = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2)
+    c = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=3)
+    for x, y in zip(a.lists(), b.lists()):
+        assert np.array_equal(x, y)
+    assert not np.array_equal(a.lists()[0], c.lists()[0])
+
+
+def test_verify_and_format(rs, orc):
+    assert rs.verify_factor(KNOWN_P, 18300, 745, 242670)
+    assert not rs.verify_factor(KNOWN_P, 18300, 745, 242671)
+    assert not rs.verify_factor(KNOWN_P + 2, 18300, 745, 242670)
+    line = rs.format_factor(KNOWN_P, 18300, 745, 242670)
+    assert line == "102297149781479 | 18300*745^242670-1\n"
+    assert re.fullmatch(r"\d+ \| \d+\*\d+\^\d+-1\n", line)
+    rng = random.Random(3)
+    for _ in range(500):
+        p = rng.getrandbits(62) | 1
+        k, b, n = rng.getrandbits(31), rng.randint(2, 2000), rng.getrandbits(20)
+        assert rs.verify_factor(p, k, b, n) == bool(orc.verify(p, k, b, n))
+
+
+def test_auto_m_is_multiple_of_64(rs):
+    pl = rs.Plan(rs.Abcd.read(R745))
+    m = pl.auto_m()
+    assert m % 64 == 0 and 256 <= m <= 2048
