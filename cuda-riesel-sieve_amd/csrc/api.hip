@@ -138,15 +138,16 @@ static rs_status make_job(const rs_plan* pl, int m, JobDev* J, std::vector<uint3
     J->maxN = pl->maxN;
     J->nrows = nrows;
     J->nk = (int)kvals->size();
-    int logM = 0;
-    while ((1 << logM) < 2 * m) ++logM;
-    J->logM = logM;
-    J->jbits = 32 - __builtin_clz((unsigned)m);
+    // buckets: average load between 1.5 and 3 of the 8 slots
+    int logNB = 0;
+    while ((1 << logNB) * 3 < m) ++logNB;
+    J->logNB = std::max(logNB, 2);
+    J->invT = J->T == 1 ? 0u : (uint32_t)((0x100000000ull + J->T - 1) / J->T);
+    if (J->Q * J->T >= 65536 || J->T > 65535) return RS_ERR_ARG;
     J->W = rec_words(J->Q, J->T, J->nk);
     J->off_gt = REC_BH + J->Q;
     J->off_j = J->off_gt + J->T;
-    int words = (1 << logM) + 2 * J->Q * J->T + nrows;
-    words = (words + 3) & ~3;
+    const int words = lds_words_per_wave(J->logNB, J->Q, J->T, nrows, J->W);
     J->lds_words = words;
     return RS_OK;
 }
@@ -228,7 +229,7 @@ extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table
     if (!s) return RS_ERR_ARG;
     if (m) *m = s->J.m;
     if (T) *T = s->J.T;
-    if (table_slots) *table_slots = 1 << s->J.logM;
+    if (table_slots) *table_slots = 8 << s->J.logNB;
     if (lds_bytes_per_wave) *lds_bytes_per_wave = s->J.lds_words * 4;
     return RS_OK;
 }

@@ -24,12 +24,12 @@ struct JobDev {
     int minN, maxN;
     int nrows;
     int nk;       // distinct k
-    int logM;     // table slots M = 2^logM >= 2m
-    int jbits;    // bit length of m: width of the j field of a table entry
+    int logNB;    // baby table: NB = 2^logNB buckets of 8 16-bit fingerprints
+    uint32_t invT;  // ceil(2^32 / T) for T >= 2, else 0: idx / T for idx < 2^16
     int W;        // record words
     int off_gt;   // REC_BH + Q
     int off_j;    // off_gt + T
-    int lds_words;  // 32-bit LDS words per wave (table, Y, active list)
+    int lds_words;  // 32-bit LDS words per wave (table, counts, Y, rows, record)
     const uint32_t* kvals;    // distinct k, ascending [nk]
     const uint32_t* rowinfo;  // per row: kidx | r << 16
     const uint32_t* rowk;     // per row: k
@@ -43,4 +43,16 @@ struct HitDev {
 
 static inline int rec_words(int Q, int T, int nk) {
     return REC_BH + Q + T + 2 * ((nk + 63) / 64);
+}
+
+// LDS layout of one wavefront, in 32-bit words from its base:
+//   [0, 4 NB)                 table: NB buckets x 8 uint16 fingerprints
+//   [4 NB, 4 NB + NB/2)       bucket fill counts, two uint16 per word
+//   then 2 Q T                Y products (uint64)
+//   then 2 nrows              active rows (k, r T << 8 | r)
+//   then 2 W                  the prime's record (uint64)
+static inline int lds_words_per_wave(int logNB, int Q, int T, int nrows, int W) {
+    const int NB = 1 << logNB;
+    int w = 4 * NB + NB / 2 + 2 * Q * T + 2 * nrows + 2 * W;
+    return (w + 3) & ~3;
 }

@@ -27,8 +27,6 @@
 #include "kernels.h"
 #include "mont.h"
 
-#define EMPTY_SLOT 0xffffffffu
-
 // Orders LDS traffic between the lanes of one wavefront.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -99,6 +97,25 @@ rs_prep_kernel(JobDev J, uint64_t low, const uint32_t* __restrict__ offs,
     R[REC_G64] = g64;
 }
 
+__device__ __forceinline__ u64 readlane64(u64 x, int l) {
+    const u32 lo = __builtin_amdgcn_readlane(lo32(x), l);
+    const u32 hi = __builtin_amdgcn_readlane(hi32(x), l);
+    return ((u64)hi << 32) | lo;
+}
+
+// idx / T for idx < 2^16: umulhi with invT = ceil(2^32 / T) when T >= 2
+// (invT is 0 for T == 1, where the quotient is idx itself).
+__device__ __forceinline__ int div_T(u32 idx, u32 invT) {
+    return invT ? (int)__umulhi(idx, invT) : (int)idx;
+}
+
+// 16-bit table fingerprint of a residue: the bits above the bucket index;
+// 0xffff marks an empty slot, so that value is folded onto 0xfffe.
+__device__ __forceinline__ u32 fingerprint(u64 v, int logNB) {
+    const u32 raw = lo32(v >> logNB) & 0xffffu;
+    return raw - (raw == 0xffffu);
+}
+
 extern "C" __global__ void __launch_bounds__(256)
 rs_bsgs_kernel(JobDev J, const uint64_t* __restrict__ rec, long long count,
                HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count,
@@ -107,94 +124,138 @@ rs_bsgs_kernel(JobDev J, const uint64_t* __restrict__ rec, long long count,
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wpb = blockDim.x >> 6;
-    const u32 M = 1u << J.logM;
-    const u32 jmask = (1u << J.jbits) - 1;
-    const int Q = J.Q, T = J.T, m = J.m;
-    u32* tab = lds + wv * J.lds_words;
-    u64* Y = (u64*)(tab + M);
-    u32* list = (u32*)(Y + Q * T);
+    const u32 NB = 1u << J.logNB;
+    const int Q = J.Q, T = J.T, m = J.m, W = J.W;
+    u32* const wbase = lds + wv * J.lds_words;
+    uint4* const tab = (uint4*)wbase;
+    uint16_t* const tab16 = (uint16_t*)wbase;
+    u32* const cnt = wbase + 4 * NB;
+    u64* const Y = (u64*)(cnt + NB / 2);
+    uint2* const list = (uint2*)(Y + Q * T);
+    u64* const R = (u64*)(list + J.nrows);
     const int L = m >> 6;
     const int dA = 64 / T, dT = 64 - dA * T;
+    const long long stride = (long long)gridDim.x * wpb;
 
-    for (long long i = (long long)blockIdx.x * wpb + wv; i < count;
-         i += (long long)gridDim.x * wpb) {
-        const u64* R = rec + i * (long long)J.W;
-        const u64 p = R[REC_P], mp = R[REC_MP], one = R[REC_ONE], g64 = R[REC_G64];
+    long long i = (long long)blockIdx.x * wpb + wv;
+    // record words 0..63 of the next prime travel in one register per lane
+    u64 pre = (i < count && lane < W) ? rec[i * W + lane] : 0;
+    for (; i < count; i += stride) {
+        if (lane < W) R[lane] = pre;
+        for (int w = 64 + lane; w < W; w += 64) R[w] = rec[i * W + w];
+        const u64 p = readlane64(pre, REC_P), mp = readlane64(pre, REC_MP);
+        const u64 g64 = readlane64(pre, REC_G64);
         const u32 mp0 = lo32(mp);
+        const long long ni = i + stride;
+        pre = (ni < count && lane < W) ? rec[ni * W + lane] : 0;
 
-        // clear the table
-        for (u32 s = lane * 4; s < M; s += 256)
-            *(uint4*)(tab + s) = make_uint4(EMPTY_SLOT, EMPTY_SLOT, EMPTY_SLOT, EMPTY_SLOT);
+        // clear table and counts
+        for (u32 b = lane; b < NB; b += 64) tab[b] = make_uint4(~0u, ~0u, ~0u, ~0u);
+        for (u32 c = lane; c < NB / 2; c += 64) cnt[c] = 0;
         wave_sync();
 
-        // baby steps: lane l inserts g^j for j = l + 64 s
-        u64 v = montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp);
+        // baby steps: lane l inserts g^j, j = l + 64 s, as a fingerprint in the
+        // first free slot of its bucket (or of the next non-full bucket)
+        const u64 vstart = montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp);
+        u64 v = vstart;
         for (int s = 0; s < L; ++s) {
-            const u32 j = (u32)(lane + 64 * s);
-            u32 slot = lo32(v) & (M - 1);
-            const u32 entry = (lo32(v >> J.logM) << J.jbits) | j;
-            while (atomicCAS(&tab[slot], EMPTY_SLOT, entry) != EMPTY_SLOT)
-                slot = (slot + 1) & (M - 1);
+            u32 b = lo32(v) & (NB - 1);
+            const u32 fp = fingerprint(v, J.logNB);
+            while (true) {
+                const u32 sh = (b & 1) * 16;
+                const u32 c = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
+                if (c < 8) {
+                    tab16[b * 8 + c] = (uint16_t)fp;
+                    break;
+                }
+                b = (b + 1) & (NB - 1);
+            }
             if (s + 1 < L) v = montmul(v, g64, p, mp);
         }
 
         // Y[r*T + t] = b^r H G^t 2^32 (Montgomery form)
         for (int idx = lane; idx < Q * T; idx += 64) {
-            const int r = idx / T, t = idx - r * T;
+            const int r = div_T((u32)idx, J.invT), t = idx - r * T;
             Y[idx] = montmul(R[REC_BH + r], R[J.off_gt + t], p, mp);
         }
 
-        // rows passing the Jacobi filter, compacted in order
+        // rows passing the Jacobi filter, compacted in order as (k, rT << 8 | r)
         int A = 0;
         for (int c = 0; c < J.nrows; c += 64) {
             const int row = c + lane;
             bool pass = false;
+            u32 k = 0, r = 0;
             if (row < J.nrows) {
                 const u32 info = J.rowinfo[row];
-                const u32 kidx = info & 0xffffu, r = info >> 16;
+                const u32 kidx = info & 0xffffu;
+                r = info >> 16;
+                k = J.rowk[row];
                 const u64 w = R[J.off_j + 2 * (kidx >> 6) + (r & 1)];
                 pass = (w >> (kidx & 63)) & 1;
             }
             const u64 mask = __ballot(pass);
-            if (pass) list[A + __popcll(mask & ((1ull << lane) - 1))] = (u32)row;
+            if (pass)
+                list[A + __popcll(mask & ((1ull << lane) - 1))] =
+                    make_uint2(k, ((r * (u32)T) << 8) | r);
             A += __popcll(mask);
         }
         wave_sync();
 
-        // giant steps: item (a, t) of the A*T grid, lanes stride by 64
+        // giant steps: item (a, t) of the A*T grid, lanes stride by 64; a probe
+        // reads one bucket (8 fingerprints) and follows full buckets.  The
+        // loop is wave-uniform: the rescan below needs every lane.
         const int total = A * T;
-        int a = lane / T, t = lane - (lane / T) * T;
-        for (int it = lane; it < total; it += 64) {
-            const u32 row = list[a];
-            const u32 k = J.rowk[row];
-            const int r = (int)(J.rowinfo[row] >> 16);
-            const u64 s = halfmul(k, Y[r * T + t], p, mp0);
-            u32 slot = lo32(s) & (M - 1);
-            const u32 fp = lo32(s >> J.logM) << J.jbits;
-            while (true) {
-                const u32 e = tab[slot];
-                if (e == EMPTY_SLOT) break;
-                if ((e & ~jmask) == fp) {
-                    // exact check: g^j == s
-                    const u32 j = e & jmask;
-                    u64 gj = montmul(R[REC_GP + (j & 7)], R[REC_G8P + ((j >> 3) & 7)], p, mp);
-                    if (j >> 6) gj = montmul(gj, mont_pow(g64, j >> 6, one, p, mp), p, mp);
-                    if (gj == s) {
-                        const long long xx = (long long)J.x0 + (long long)t * m + (m - 1 - (int)j);
-                        const long long n = (long long)Q * xx + r;
+        int a = div_T((u32)lane, J.invT), t = lane - a * T;
+        for (int it0 = 0; it0 < total; it0 += 64) {
+            const bool act = it0 + lane < total;
+            bool match = false;
+            u64 s = 0;
+            uint2 row = make_uint2(0, 0);
+            if (act) {
+                row = list[a];
+                s = halfmul(row.x, Y[(row.y >> 8) + t], p, mp0);
+                u32 b = lo32(s) & (NB - 1);
+                const u32 fp2 = fingerprint(s, J.logNB) * 0x10001u;
+                while (true) {
+                    const uint4 e = tab[b];
+                    const u32 x0 = e.x ^ fp2, x1 = e.y ^ fp2, x2 = e.z ^ fp2, x3 = e.w ^ fp2;
+                    // a 16-bit half of x is zero iff that slot holds the fingerprint
+                    const u32 z = ((x0 - 0x10001u) & ~x0) | ((x1 - 0x10001u) & ~x1) |
+                                  ((x2 - 0x10001u) & ~x2) | ((x3 - 0x10001u) & ~x3);
+                    match |= (z & 0x80008000u) != 0;
+                    if ((e.w >> 16) == 0xffffu) break;  // slot 7 free: chain ends here
+                    b = (b + 1) & (NB - 1);
+                }
+            }
+            // fingerprint match: all lanes rescan the baby steps (lane l checks
+            // j = l + 64 u) for exact equalities with the matching lane's s
+            u64 mb = __ballot(match);
+            while (mb) {
+                const int src = __builtin_ctzll(mb);
+                mb &= mb - 1;
+                const u64 target = readlane64(s, src);
+                const u32 kk = __builtin_amdgcn_readlane(row.x, src);
+                const u32 ry = __builtin_amdgcn_readlane(row.y, src);
+                const int tt = __builtin_amdgcn_readlane(t, src);
+                u64 w = vstart;
+                for (int u = 0; u < L; ++u) {
+                    if (w == target) {
+                        const int j = lane + 64 * u;
+                        const long long xx = (long long)J.x0 + (long long)tt * m + (m - 1 - j);
+                        const long long n = (long long)Q * xx + (int)(ry & 0xffu);
                         if (n >= J.minN && n <= J.maxN) {
                             const u32 at = atomicAdd(hit_count, 1u);
                             if (at < hit_cap) {
                                 HitDev h;
                                 h.p = p;
-                                h.k = k;
+                                h.k = kk;
                                 h.n = (u32)n;
                                 hits[at] = h;
                             }
                         }
                     }
+                    w = montmul(w, g64, p, mp);
                 }
-                slot = (slot + 1) & (M - 1);
             }
             t += dT;
             a += dA;
@@ -227,21 +288,23 @@ extern "C" __global__ void rs_test_jacobi(const uint32_t* a, const uint64_t* p, 
     out[i] = jacobi32(a[i], p[i]);
 }
 
-// Throughput probe for v_mad_u64_u32: 8 independent dependent chains per
-// lane, iters steps each (8*iters mads per lane).  The result is stored so
-// the chains are not removed.
+// Throughput probe for v_mad_u64_u32: 8 independent chains per lane, each
+// step acc = lo32(acc) * x + c with a loop-invariant 64-bit addend c, so the
+// loop body is mads only (8*iters per lane).  The result is stored so the
+// chains are not removed.
 extern "C" __global__ void __launch_bounds__(256) rs_mad_probe(uint64_t* out, int iters) {
     u64 acc[8];
-    u32 a = threadIdx.x * 2654435761u + blockIdx.x;
+    const u32 a = threadIdx.x * 2654435761u + blockIdx.x;
+    const u64 c = ((u64)a << 17) ^ 0x9E3779B97F4A7C15ull;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = (u64)(a + c) * 0x9E3779B9u;
+    for (int k = 0; k < 8; ++k) acc[k] = (u64)(a + k) * 0x9E3779B9u;
     for (int i = 0; i < iters; ++i) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[c] = mad32(lo32(acc[c]), a ^ (u32)c, hi32(acc[c]));
+        for (int k = 0; k < 8; ++k) acc[k] = mad32(lo32(acc[k]), a ^ (u32)k, c);
     }
     u64 s = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) s ^= acc[c];
+    for (int k = 0; k < 8; ++k) s ^= acc[k];
     out[(long long)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
