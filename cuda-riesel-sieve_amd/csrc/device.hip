@@ -1,0 +1,629 @@
+// device.hip -- C ABI of the device sieve: single jobs (rs_sieve_*) and
+// batched launches of many jobs (rs_batch_*), both around the persistent
+// kernel rs_sieve_kernel.
+//
+// Replaces the reference's device set-up and launch sequence
+// (kernel_v0.24.6.cu:1367-1512 allocations and constant copies,
+// :1537-1611 per-batch copy, memsets, addKernel1 launch and result copy):
+//  * job constants and the row table are copied once at create;
+//  * a chunk of primes runs as one persistent launch whose wavefronts take
+//    64-prime groups from a device work counter (no per-launch batch limit,
+//    no hash-table memsets: the tables live in LDS);
+//  * host prime batches go through two pinned buffers on a copy stream, so
+//    the host may refill one while the other is copied or sieved (the
+//    reference reuses one pinned buffer, racing :1573 against :1592);
+//  * hits go to a device append buffer (every hit, not one per prime) and
+//    are returned sorted at collect;
+//  * batched launches take job and chunk tables from device memory and are
+//    captured once in a hipGraph, then replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../host/host.h"
+#include "kernels.h"
+
+extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
+                                           const ChunkDev* chunks, int nchunks,
+                                           long long ngroups, int lds_words, int scratch_words,
+                                           uint64_t* scratch, unsigned long long* work,
+                                           HitDev* hits, uint32_t* hit_count, uint32_t hit_cap);
+extern "C" __global__ void rs_zero_u32(uint32_t*, int);
+
+#define HIPCHK(x)                                 \
+    do {                                          \
+        if ((x) != hipSuccess) return RS_ERR_HIP; \
+    } while (0)
+
+namespace {
+
+const int kMaxWavesPerBlock = 4;
+
+// Job constants and device row tables built from a plan.
+struct DevJob {
+    JobDev J{};
+    uint32_t* d_kvals = nullptr;
+    uint32_t* d_rowinfo = nullptr;
+    uint32_t* d_rowk = nullptr;
+};
+
+void free_job(DevJob& j) {
+    (void)hipFree(j.d_kvals);
+    (void)hipFree(j.d_rowinfo);
+    (void)hipFree(j.d_rowk);
+    j.d_kvals = j.d_rowinfo = j.d_rowk = nullptr;
+}
+
+// Fills the job constants from a plan and table size m (0: cost model).
+rs_status make_job(const rs_plan* pl, int m, DevJob* dj) {
+    if (!pl) return RS_ERR_ARG;
+    if (m <= 0) m = rs_plan_auto_m(pl);
+    if (m < 64 || m % 64 != 0 || m > (1 << 16)) return RS_ERR_ARG;
+    if (pl->Q < 2 || pl->Q % 2 != 0 || pl->minN < 1 || pl->rows_k.empty()) return RS_ERR_ARG;
+    const int nrows = (int)pl->rows_k.size();
+    std::vector<uint32_t> kvals, rowinfo(nrows), rowk(nrows);
+    for (int32_t k : pl->rows_k)
+        if (kvals.empty() || kvals.back() != (uint32_t)k) kvals.push_back((uint32_t)k);
+    if (kvals.size() > 65535) return RS_ERR_ARG;
+    for (int i = 0; i < nrows; ++i) {
+        const uint32_t kidx =
+            (uint32_t)(std::lower_bound(kvals.begin(), kvals.end(), (uint32_t)pl->rows_k[i]) -
+                       kvals.begin());
+        rowinfo[i] = kidx | ((uint32_t)pl->rows_r[i] << 16);
+        rowk[i] = (uint32_t)pl->rows_k[i];
+    }
+    JobDev& J = dj->J;
+    J.base = pl->base;
+    J.Q = pl->Q;
+    J.m = m;
+    J.x0 = pl->x0;
+    J.T = (pl->x1 - pl->x0 + 1 + m - 1) / m;
+    J.minN = pl->minN;
+    J.maxN = pl->maxN;
+    J.nrows = nrows;
+    J.nk = (int)kvals.size();
+    // buckets: average load between 1.5 and 3 of the 8 slots
+    int logNB = 0;
+    while ((1 << logNB) * 3 < m) ++logNB;
+    J.logNB = std::max(logNB, 2);
+    if (J.Q * J.T >= 65536 || J.T > 65535) return RS_ERR_ARG;
+    J.invT = J.T == 1 ? 0u : (uint32_t)((0x100000000ull + J.T - 1) / J.T);
+    J.W = rec_words(J.Q, J.T, J.nk);
+    J.off_gt = REC_BH + J.Q;
+    J.off_j = J.off_gt + J.T;
+    J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.T, nrows, J.W);
+    HIPCHK(hipMalloc(&dj->d_kvals, kvals.size() * 4));
+    HIPCHK(hipMalloc(&dj->d_rowinfo, rowinfo.size() * 4));
+    HIPCHK(hipMalloc(&dj->d_rowk, rowk.size() * 4));
+    HIPCHK(hipMemcpy(dj->d_kvals, kvals.data(), kvals.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dj->d_rowinfo, rowinfo.data(), rowinfo.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dj->d_rowk, rowk.data(), rowk.size() * 4, hipMemcpyHostToDevice));
+    J.kvals = dj->d_kvals;
+    J.rowinfo = dj->d_rowinfo;
+    J.rowk = dj->d_rowk;
+    return RS_OK;
+}
+
+// Resources of the persistent kernel on one device: grid shape, per-wave
+// record scratch, work counter and hit buffer.
+struct Launcher {
+    int device = 0;
+    int grid = 0;
+    int wpb = 1;
+    int lds_words = 0;
+    size_t lds_bytes = 0;
+    int scratch_words = 0;  // per wavefront: 64 records
+    uint64_t* scratch = nullptr;
+    unsigned long long* work = nullptr;
+    HitDev* hits = nullptr;
+    uint32_t* hit_count = nullptr;
+    uint32_t hit_cap = 0;
+};
+
+void free_launcher(Launcher& L) {
+    (void)hipFree(L.scratch);
+    (void)hipFree(L.work);
+    (void)hipFree(L.hits);
+    (void)hipFree(L.hit_count);
+    L.scratch = nullptr;
+    L.work = nullptr;
+    L.hits = nullptr;
+    L.hit_count = nullptr;
+}
+
+// Sizes the grid for lds_words per wave and record size W (both maxima over
+// the jobs of the launch) and allocates scratch, counter and hits.
+rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity) {
+    L->device = device;
+    int max_lds = 0, cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const size_t per_wave = (size_t)lds_words * 4;
+    if (per_wave > (size_t)max_lds) return RS_ERR_ARG;
+    // waves of a block are independent (one table each); up to 4 per block
+    L->wpb = (int)std::min<size_t>(kMaxWavesPerBlock, (size_t)max_lds / per_wave);
+    L->lds_words = lds_words;
+    L->lds_bytes = per_wave * L->wpb;
+    if (L->lds_bytes > 65536)
+        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(rs_sieve_kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)L->lds_bytes));
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(rs_sieve_kernel), 64 * L->wpb, L->lds_bytes));
+    L->grid = std::max(1, cus * std::max(1, per_cu));
+    L->scratch_words = 64 * W;
+    const size_t waves = (size_t)L->grid * L->wpb;
+    if (hipMalloc(&L->scratch, waves * L->scratch_words * 8) != hipSuccess) return RS_ERR_NOMEM;
+    HIPCHK(hipMalloc(&L->work, 8));
+    L->hit_cap = (uint32_t)hit_capacity;
+    if (hipMalloc(&L->hits, (size_t)hit_capacity * sizeof(HitDev)) != hipSuccess)
+        return RS_ERR_NOMEM;
+    HIPCHK(hipMalloc(&L->hit_count, 4));
+    HIPCHK(hipMemset(L->hit_count, 0, 4));
+    return RS_OK;
+}
+
+// Zeroes the work counter and launches the persistent kernel on `stream`.
+// Kernel-only (no memset nodes), so it can be captured in a graph.
+rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const ChunkDev& c0,
+                 const JobDev* jobs, const ChunkDev* chunks, int nchunks, long long ngroups) {
+    hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)L.work, 2);
+    HIPCHK(hipGetLastError());
+    const int grid = (int)std::min<long long>(L.grid, (ngroups + L.wpb - 1) / L.wpb);
+    hipLaunchKernelGGL(rs_sieve_kernel, dim3(std::max(1, grid)), dim3(64 * L.wpb), L.lds_bytes,
+                       stream, j0, c0, jobs, chunks, nchunks, ngroups, L.lds_words,
+                       L.scratch_words, L.scratch, L.work, L.hits, L.hit_count, L.hit_cap);
+    HIPCHK(hipGetLastError());
+    return RS_OK;
+}
+
+// Copies the hits out (appending to `out`) and zeroes the device counter.
+rs_status drain_hits(const Launcher& L, hipStream_t stream, std::vector<HitDev>* out,
+                     int64_t* lost) {
+    HIPCHK(hipStreamSynchronize(stream));
+    uint32_t n = 0;
+    HIPCHK(hipMemcpy(&n, L.hit_count, 4, hipMemcpyDeviceToHost));
+    const uint32_t got = std::min(n, L.hit_cap);
+    if (n > L.hit_cap) *lost += (int64_t)(n - L.hit_cap);
+    if (got) {
+        const size_t old = out->size();
+        out->resize(old + got);
+        HIPCHK(hipMemcpy(out->data() + old, L.hits, (size_t)got * sizeof(HitDev),
+                         hipMemcpyDeviceToHost));
+    }
+    hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, L.hit_count, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    return RS_OK;
+}
+
+bool hit_less(const HitDev& a, const HitDev& b) {
+    if (a.job != b.job) return a.job < b.job;
+    if (a.p != b.p) return a.p < b.p;
+    if (a.k != b.k) return a.k < b.k;
+    return a.n < b.n;
+}
+
+struct Timing {
+    hipEvent_t a, b;
+    long long primes;
+};
+
+}  // namespace
+
+extern "C" int rs_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+// ------------------------------------------------------------ single job
+
+struct rs_sieve {
+    int device = 0;
+    DevJob job;
+    Launcher L;
+    int64_t max_batch = 0;
+    hipStream_t stream = nullptr, copy_stream = nullptr;
+    uint32_t* h_pin[2] = {nullptr, nullptr};
+    uint32_t* d_offs[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {nullptr, nullptr};
+    hipEvent_t consumed[2] = {nullptr, nullptr};
+    bool consumed_valid[2] = {false, false};
+    int buf = 0;
+    std::vector<Timing> timings;
+    double acc_ms = 0;
+    long long acc_primes = 0;
+    std::vector<HitDev> held;
+    int64_t lost = 0;
+};
+
+static void free_sieve(rs_sieve* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
+    free_job(s->job);
+    free_launcher(s->L);
+    for (int b = 0; b < 2; ++b) {
+        if (s->h_pin[b]) (void)hipHostFree(s->h_pin[b]);
+        (void)hipFree(s->d_offs[b]);
+        if (s->copied[b]) (void)hipEventDestroy(s->copied[b]);
+        if (s->consumed[b]) (void)hipEventDestroy(s->consumed[b]);
+    }
+    for (auto& t : s->timings) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+    delete s;
+}
+
+extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int64_t max_batch,
+                                     int64_t hit_capacity, rs_sieve** out) {
+    if (!pl || !out || max_batch < 1 || hit_capacity < 1 || hit_capacity > 0x7fffffff)
+        return RS_ERR_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return RS_ERR_HIP;
+    rs_sieve* s = new rs_sieve();
+    s->device = device;
+    s->max_batch = max_batch;
+    rs_status st = make_job(pl, m, &s->job);
+    if (st == RS_OK) st = init_launcher(&s->L, device, s->job.J.lds_words, s->job.J.W, hit_capacity);
+    if (st == RS_OK &&
+        (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+         hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess))
+        st = RS_ERR_HIP;
+    for (int b = 0; b < 2 && st == RS_OK; ++b)
+        if (hipEventCreateWithFlags(&s->copied[b], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->consumed[b], hipEventDisableTiming) != hipSuccess)
+            st = RS_ERR_HIP;
+    if (st != RS_OK) {
+        free_sieve(s);
+        return st;
+    }
+    *out = s;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
+                                   int* lds_bytes_per_wave) {
+    if (!s) return RS_ERR_ARG;
+    if (m) *m = s->job.J.m;
+    if (T) *T = s->job.J.T;
+    if (table_slots) *table_slots = 8 << s->job.J.logNB;
+    if (lds_bytes_per_wave) *lds_bytes_per_wave = s->job.J.lds_words * 4;
+    return RS_OK;
+}
+
+// One persistent launch over count primes (offsets in device memory).
+static rs_status sieve_launch(rs_sieve* s, uint64_t low, const uint32_t* d_offs, int64_t count) {
+    Timing t{};
+    HIPCHK(hipEventCreate(&t.a));
+    HIPCHK(hipEventCreate(&t.b));
+    t.primes = count;
+    HIPCHK(hipEventRecord(t.a, s->stream));
+    ChunkDev c0{};
+    c0.low = low;
+    c0.offs = d_offs;
+    c0.count = count;
+    c0.group0 = 0;
+    c0.job = 0;
+    rs_status st = launch(s->L, s->stream, s->job.J, c0, nullptr, nullptr, 1, (count + 63) / 64);
+    if (st != RS_OK) return st;
+    HIPCHK(hipEventRecord(t.b, s->stream));
+    s->timings.push_back(t);
+    return RS_OK;
+}
+
+extern "C" rs_status rs_sieve_submit_device(rs_sieve* s, uint64_t low,
+                                            const uint32_t* offsets_dev, int64_t count) {
+    if (!s || count < 0 || (count > 0 && !offsets_dev)) return RS_ERR_ARG;
+    if (count == 0) return RS_OK;
+    // offsets are not read back on this path: bound p by the widest offset
+    if (low < 3 || low >= (1ull << 63) - (1ull << 32)) return RS_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    return sieve_launch(s, low, offsets_dev, count);
+}
+
+extern "C" rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* offsets,
+                                     int64_t count) {
+    if (!s || count < 0 || (count > 0 && !offsets)) return RS_ERR_ARG;
+    if (count == 0) return RS_OK;
+    uint32_t mx = 0;
+    for (int64_t i = 0; i < count; ++i) mx = std::max(mx, offsets[i]);
+    const uint64_t hi = low + mx;
+    if (low < 3 || hi < low || hi >= (1ull << 63)) return RS_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    for (int64_t done = 0; done < count; done += s->max_batch) {
+        const int64_t n = std::min<int64_t>(s->max_batch, count - done);
+        const int b = s->buf;
+        if (!s->h_pin[b]) {
+            HIPCHK(hipHostMalloc(&s->h_pin[b], (size_t)s->max_batch * 4, hipHostMallocDefault));
+            HIPCHK(hipMalloc(&s->d_offs[b], (size_t)s->max_batch * 4));
+        }
+        // the buffer's previous batch must be fully sieved before reuse
+        if (s->consumed_valid[b]) HIPCHK(hipEventSynchronize(s->consumed[b]));
+        memcpy(s->h_pin[b], offsets + done, (size_t)n * 4);
+        HIPCHK(hipMemcpyAsync(s->d_offs[b], s->h_pin[b], (size_t)n * 4, hipMemcpyHostToDevice,
+                              s->copy_stream));
+        HIPCHK(hipEventRecord(s->copied[b], s->copy_stream));
+        HIPCHK(hipStreamWaitEvent(s->stream, s->copied[b], 0));
+        rs_status st = sieve_launch(s, low, s->d_offs[b], n);
+        if (st != RS_OK) return st;
+        HIPCHK(hipEventRecord(s->consumed[b], s->stream));
+        s->consumed_valid[b] = true;
+        s->buf ^= 1;
+    }
+    return RS_OK;
+}
+
+static rs_status sieve_drain(rs_sieve* s) {
+    HIPCHK(hipSetDevice(s->device));
+    rs_status st = drain_hits(s->L, s->stream, &s->held, &s->lost);
+    if (st != RS_OK) return st;
+    for (auto& t : s->timings) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+        s->acc_ms += ms;
+        s->acc_primes += t.primes;
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    s->timings.clear();
+    return RS_OK;
+}
+
+extern "C" rs_status rs_sieve_sync(rs_sieve* s) {
+    if (!s) return RS_ERR_ARG;
+    return sieve_drain(s);
+}
+
+extern "C" rs_status rs_sieve_collect(rs_sieve* s, rs_factor* out, int64_t cap, int64_t* count,
+                                      int64_t* lost) {
+    if (!s || !count || cap < 0 || (cap > 0 && !out)) return RS_ERR_ARG;
+    rs_status st = sieve_drain(s);
+    if (st != RS_OK) return st;
+    std::sort(s->held.begin(), s->held.end(), hit_less);
+    *count = (int64_t)s->held.size();
+    if (lost) *lost = s->lost;
+    if ((int64_t)s->held.size() > cap) return RS_ERR_OVERFLOW;
+    for (size_t i = 0; i < s->held.size(); ++i) {
+        out[i].p = s->held[i].p;
+        out[i].k = s->held[i].k;
+        out[i].n = s->held[i].n;
+    }
+    s->held.clear();
+    s->lost = 0;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_sieve_timing(rs_sieve* s, double* kernel_ms, int64_t* primes) {
+    if (!s) return RS_ERR_ARG;
+    rs_status st = sieve_drain(s);
+    if (st != RS_OK) return st;
+    if (kernel_ms) *kernel_ms = s->acc_ms;
+    if (primes) *primes = s->acc_primes;
+    s->acc_ms = 0;
+    s->acc_primes = 0;
+    return RS_OK;
+}
+
+extern "C" void rs_sieve_destroy(rs_sieve* s) { free_sieve(s); }
+
+// ------------------------------------------------------------ batched jobs
+
+struct rs_batch {
+    int device = 0;
+    std::vector<DevJob> jobs;
+    JobDev* d_jobs = nullptr;
+    Launcher L;
+    hipStream_t stream = nullptr;
+    std::vector<ChunkDev> chunks;      // host copy, group0 filled at run
+    ChunkDev* d_chunks = nullptr;
+    size_t d_chunks_cap = 0;
+    long long ngroups = 0;
+    bool dirty = true;                 // chunk table changed since capture
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    bool timed = false;
+    double acc_ms = 0;
+    long long acc_primes = 0, pending_primes = 0;
+    std::vector<HitDev> held;
+    int64_t lost = 0;
+};
+
+static void free_batch(rs_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto& j : b->jobs) free_job(j);
+    (void)hipFree(b->d_jobs);
+    (void)hipFree(b->d_chunks);
+    free_launcher(b->L);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev_a) (void)hipEventDestroy(b->ev_a);
+    if (b->ev_b) (void)hipEventDestroy(b->ev_b);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" rs_status rs_batch_create(int device, int njobs, const rs_plan* const* plans,
+                                     const int* ms, int64_t hit_capacity, rs_batch** out) {
+    if (!out || njobs < 1 || !plans || hit_capacity < 1 || hit_capacity > 0x7fffffff)
+        return RS_ERR_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return RS_ERR_HIP;
+    rs_batch* b = new rs_batch();
+    b->device = device;
+    b->jobs.resize(njobs);
+    rs_status st = RS_OK;
+    int lds_words = 0, W = 0;
+    for (int i = 0; i < njobs && st == RS_OK; ++i) {
+        st = make_job(plans[i], ms ? ms[i] : 0, &b->jobs[i]);
+        if (st == RS_OK) {
+            lds_words = std::max(lds_words, b->jobs[i].J.lds_words);
+            W = std::max(W, b->jobs[i].J.W);
+        }
+    }
+    if (st == RS_OK) st = init_launcher(&b->L, device, lds_words, W, hit_capacity);
+    if (st == RS_OK) {
+        std::vector<JobDev> host(njobs);
+        for (int i = 0; i < njobs; ++i) host[i] = b->jobs[i].J;
+        if (hipMalloc(&b->d_jobs, sizeof(JobDev) * njobs) != hipSuccess ||
+            hipMemcpy(b->d_jobs, host.data(), sizeof(JobDev) * njobs, hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess)
+            st = RS_ERR_HIP;
+    }
+    if (st != RS_OK) {
+        free_batch(b);
+        return st;
+    }
+    *out = b;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_batch_job_info(const rs_batch* b, int job, int* m, int* T,
+                                       int* lds_bytes) {
+    if (!b || job < 0 || job >= (int)b->jobs.size()) return RS_ERR_ARG;
+    if (m) *m = b->jobs[job].J.m;
+    if (T) *T = b->jobs[job].J.T;
+    if (lds_bytes) *lds_bytes = b->jobs[job].J.lds_words * 4;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_batch_add_chunk(rs_batch* b, int job, uint64_t low,
+                                        const uint32_t* offsets_dev, int64_t count) {
+    if (!b || job < 0 || job >= (int)b->jobs.size() || count < 0 || (count > 0 && !offsets_dev))
+        return RS_ERR_ARG;
+    if (low < 3 || low >= (1ull << 63) - (1ull << 32)) return RS_ERR_ARG;
+    if (count == 0) return RS_OK;
+    ChunkDev c{};
+    c.low = low;
+    c.offs = offsets_dev;
+    c.count = count;
+    c.job = job;
+    b->chunks.push_back(c);
+    b->dirty = true;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_batch_clear(rs_batch* b) {
+    if (!b) return RS_ERR_ARG;
+    b->chunks.clear();
+    b->dirty = true;
+    return RS_OK;
+}
+
+// Uploads the chunk table and captures zero + kernel into a graph.
+static rs_status batch_capture(rs_batch* b) {
+    long long g = 0;
+    for (auto& c : b->chunks) {
+        c.group0 = g;
+        g += (c.count + 63) / 64;
+    }
+    b->ngroups = g;
+    if (b->chunks.size() > b->d_chunks_cap) {
+        (void)hipFree(b->d_chunks);
+        b->d_chunks = nullptr;
+        b->d_chunks_cap = std::max<size_t>(64, b->chunks.size());
+        if (hipMalloc(&b->d_chunks, sizeof(ChunkDev) * b->d_chunks_cap) != hipSuccess)
+            return RS_ERR_NOMEM;
+    }
+    HIPCHK(hipMemcpy(b->d_chunks, b->chunks.data(), sizeof(ChunkDev) * b->chunks.size(),
+                     hipMemcpyHostToDevice));
+    if (b->exec) {
+        HIPCHK(hipGraphExecDestroy(b->exec));
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        HIPCHK(hipGraphDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    ChunkDev c0{};
+    rs_status st = launch(b->L, b->stream, b->jobs[0].J, c0, b->d_jobs, b->d_chunks,
+                          (int)b->chunks.size(), b->ngroups);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (st != RS_OK) return st;
+    HIPCHK(e);
+    b->graph = graph;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->dirty = false;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_batch_run(rs_batch* b) {
+    if (!b) return RS_ERR_ARG;
+    if (b->chunks.empty()) return RS_ERR_STATE;
+    HIPCHK(hipSetDevice(b->device));
+    if (b->timed) return RS_ERR_STATE;  // collect the previous run first
+    if (b->dirty) {
+        rs_status st = batch_capture(b);
+        if (st != RS_OK) return st;
+    }
+    HIPCHK(hipEventRecord(b->ev_a, b->stream));
+    HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHK(hipEventRecord(b->ev_b, b->stream));
+    b->timed = true;
+    b->pending_primes = 0;
+    for (auto& c : b->chunks) b->pending_primes += c.count;
+    return RS_OK;
+}
+
+static rs_status batch_drain(rs_batch* b) {
+    HIPCHK(hipSetDevice(b->device));
+    rs_status st = drain_hits(b->L, b->stream, &b->held, &b->lost);
+    if (st != RS_OK) return st;
+    if (b->timed) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev_a, b->ev_b));
+        b->acc_ms += ms;
+        b->acc_primes += b->pending_primes;
+        b->timed = false;
+    }
+    return RS_OK;
+}
+
+extern "C" rs_status rs_batch_collect(rs_batch* b, rs_job_factor* out, int64_t cap,
+                                      int64_t* count, int64_t* lost) {
+    if (!b || !count || cap < 0 || (cap > 0 && !out)) return RS_ERR_ARG;
+    rs_status st = batch_drain(b);
+    if (st != RS_OK) return st;
+    std::sort(b->held.begin(), b->held.end(), hit_less);
+    *count = (int64_t)b->held.size();
+    if (lost) *lost = b->lost;
+    if ((int64_t)b->held.size() > cap) return RS_ERR_OVERFLOW;
+    for (size_t i = 0; i < b->held.size(); ++i) {
+        out[i].p = b->held[i].p;
+        out[i].k = b->held[i].k;
+        out[i].n = b->held[i].n;
+        out[i].job = b->held[i].job;
+        out[i].pad = 0;
+    }
+    b->held.clear();
+    b->lost = 0;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_batch_timing(rs_batch* b, double* kernel_ms, int64_t* primes) {
+    if (!b) return RS_ERR_ARG;
+    rs_status st = batch_drain(b);
+    if (st != RS_OK) return st;
+    if (kernel_ms) *kernel_ms = b->acc_ms;
+    if (primes) *primes = b->acc_primes;
+    b->acc_ms = 0;
+    b->acc_primes = 0;
+    return RS_OK;
+}
+
+extern "C" void rs_batch_destroy(rs_batch* b) { free_batch(b); }

@@ -3,8 +3,8 @@
 
 #include <stdint.h>
 
-// Record of one prime written by the set-up kernel and read by the BSGS
-// kernel (64-bit words).  All residues are Montgomery forms (R = 2^64).
+// Record of one prime written by the set-up phase and read by the BSGS
+// phase (64-bit words).  All residues are Montgomery forms (R = 2^64).
 //   [0] p   [1] mp = -p^-1 mod 2^64   [2] one = R mod p   [3] g^64
 //   [4 .. 12)   g^c for c = 0..7          (g = b^Q)
 //   [12 .. 20)  g^(8a) for a = 0..7
@@ -35,10 +35,23 @@ struct JobDev {
     const uint32_t* rowk;     // per row: k
 };
 
+// A chunk of primes p = low + offs[i], i < count, of one job; group0 is
+// its first 64-prime group in the launch's global numbering.
+struct ChunkDev {
+    uint64_t low;
+    const uint32_t* offs;
+    long long count;
+    long long group0;
+    int job;
+    int pad;
+};
+
 struct HitDev {
     uint64_t p;
     uint32_t k;
     uint32_t n;
+    uint32_t job;
+    uint32_t pad;
 };
 
 static inline int rec_words(int Q, int T, int nk) {

@@ -1,27 +1,33 @@
-// sieve.hip -- BSGS discrete-log sieve kernels for gfx950.
+// sieve.hip -- BSGS discrete-log sieve kernel for gfx950.
 //
 // Solves k*b^n == 1 (mod p) for every row (k, r), n = Q*x + r, of a job and
-// every prime p of a batch, the computation of the reference's addKernel1
-// (kernel_v0.24.6.cu:273-684), laid out for CDNA4:
+// every prime p of a chunk: the computation of the reference's addKernel1
+// (kernel_v0.24.6.cu:273-684), laid out for CDNA4 as one persistent kernel.
+// Each wavefront repeatedly takes a group of 64 consecutive primes of one
+// chunk from a device work counter and runs two phases:
 //
-//  rs_prep_kernel (one thread per prime): Montgomery constants, g = b^Q,
-//    the baby-step start powers, BH[r] and G^t factors of the giant steps and
-//    the Jacobi bits of every k (the reference computes these per thread at
-//    :288-356, :425-456, :491-509).  Writes one record per prime.
-//
-//  rs_bsgs_kernel (one wavefront per prime): the 64 lanes build the baby
-//    table {g^j : 0 <= j < m} in LDS (open addressing on the low residue
-//    bits; the reference keeps a chained table per thread in global memory,
+//  set-up, one lane per prime: Montgomery constants, g = b^Q, the baby-step
+//    start powers, the BH[r] and G^t factors of the giant steps and the
+//    Jacobi bits of every k (the reference computes these per thread at
+//    :288-356, :425-456, :491-509), written as a record to the wave's global
+//    scratch;
+//  BSGS, the whole wavefront per prime: the 64 lanes fill the baby table
+//    {g^j : 0 <= j < m} in LDS (8-slot buckets of 16-bit fingerprints; the
+//    reference keeps a chained table per thread in global memory,
 //    :358-411), form the Y table b^r*H*G^t, compact the rows that pass the
-//    Jacobi filter (:516-521), and spread the (row, t) giant steps over the
-//    lanes; each giant step is one 32x64 Montgomery product k*Y and one
-//    probe (:534-608).  Every fingerprint match is checked exactly against
-//    g^j before it is appended to the hit buffer (the reference keeps the
-//    last hit per prime only, :615-621).
+//    Jacobi filter (:516-521) and spread the (row, t) giant steps over the
+//    lanes; a giant step is one 32x64 Montgomery product k*Y and one bucket
+//    read (:534-608).  A fingerprint match is resolved by an exact rescan of
+//    the baby steps (the reference rescans on a 16-bit key match too, :578)
+//    and every solution is appended to the hit buffer (the reference keeps
+//    the last hit per prime only, :615-621).
 //
 // Giant step t of row (k, r) holds s = k*b^r*g^(x0 + m - 1 + t*m); s = g^j
 // means x = x0 + t*m + (m - 1 - j), so t in [0, T) covers x in
 // [x0, x0 + T*m - 1] which contains [floor(minN/Q), floor(maxN/Q)].
+//
+// Chunks and jobs come either by value (c0, j0: one job, one chunk) or from
+// device tables (batched launches of many jobs).
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -34,13 +40,34 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-extern "C" __global__ void __launch_bounds__(256)
-rs_prep_kernel(JobDev J, uint64_t low, const uint32_t* __restrict__ offs,
-               long long count, uint64_t* __restrict__ rec) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const u64 p = low + offs[i];
-    u64* R = rec + i * (long long)J.W;
+// Makes the wavefront's global stores visible to its other lanes.
+__device__ __forceinline__ void wave_sync_global() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ u64 readlane64(u64 x, int l) {
+    const u32 lo = __builtin_amdgcn_readlane(lo32(x), l);
+    const u32 hi = __builtin_amdgcn_readlane(hi32(x), l);
+    return ((u64)hi << 32) | lo;
+}
+
+// idx / T for idx < 2^16: umulhi with invT = ceil(2^32 / T) when T >= 2
+// (invT is 0 for T == 1, where the quotient is idx itself).
+__device__ __forceinline__ int div_T(u32 idx, u32 invT) {
+    return invT ? (int)__umulhi(idx, invT) : (int)idx;
+}
+
+// 16-bit table fingerprint of a residue: the bits above the bucket index;
+// 0xffff marks an empty slot, so that value is folded onto 0xfffe.
+__device__ __forceinline__ u32 fingerprint(u64 v, int logNB) {
+    const u32 raw = lo32(v >> logNB) & 0xffffu;
+    return raw - (raw == 0xffffu);
+}
+
+// Set-up of one prime (one lane): writes the record described in kernels.h.
+__device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restrict__ R) {
     const u64 mp = mont_mprime(p);
     const u64 one = (0ull - p) % p;                  // 2^64 mod p
     u64 r2 = one << 1;                               // Mont(2)
@@ -97,36 +124,15 @@ rs_prep_kernel(JobDev J, uint64_t low, const uint32_t* __restrict__ offs,
     R[REC_G64] = g64;
 }
 
-__device__ __forceinline__ u64 readlane64(u64 x, int l) {
-    const u32 lo = __builtin_amdgcn_readlane(lo32(x), l);
-    const u32 hi = __builtin_amdgcn_readlane(hi32(x), l);
-    return ((u64)hi << 32) | lo;
-}
-
-// idx / T for idx < 2^16: umulhi with invT = ceil(2^32 / T) when T >= 2
-// (invT is 0 for T == 1, where the quotient is idx itself).
-__device__ __forceinline__ int div_T(u32 idx, u32 invT) {
-    return invT ? (int)__umulhi(idx, invT) : (int)idx;
-}
-
-// 16-bit table fingerprint of a residue: the bits above the bucket index;
-// 0xffff marks an empty slot, so that value is folded onto 0xfffe.
-__device__ __forceinline__ u32 fingerprint(u64 v, int logNB) {
-    const u32 raw = lo32(v >> logNB) & 0xffffu;
-    return raw - (raw == 0xffffu);
-}
-
-extern "C" __global__ void __launch_bounds__(256)
-rs_bsgs_kernel(JobDev J, const uint64_t* __restrict__ rec, long long count,
-               HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count,
-               uint32_t hit_cap) {
-    extern __shared__ uint32_t lds[];
+// BSGS of one prime by the whole wavefront.  `pre` holds record words
+// 0..63 (lane l has word l), the rest are read from Rg; LDS at wbase.
+__device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
+                                           const u64* __restrict__ Rg, u32* __restrict__ wbase,
+                                           HitDev* __restrict__ hits,
+                                           u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wpb = blockDim.x >> 6;
     const u32 NB = 1u << J.logNB;
     const int Q = J.Q, T = J.T, m = J.m, W = J.W;
-    u32* const wbase = lds + wv * J.lds_words;
     uint4* const tab = (uint4*)wbase;
     uint16_t* const tab16 = (uint16_t*)wbase;
     u32* const cnt = wbase + 4 * NB;
@@ -135,133 +141,177 @@ rs_bsgs_kernel(JobDev J, const uint64_t* __restrict__ rec, long long count,
     u64* const R = (u64*)(list + J.nrows);
     const int L = m >> 6;
     const int dA = 64 / T, dT = 64 - dA * T;
-    const long long stride = (long long)gridDim.x * wpb;
 
-    long long i = (long long)blockIdx.x * wpb + wv;
-    // record words 0..63 of the next prime travel in one register per lane
-    u64 pre = (i < count && lane < W) ? rec[i * W + lane] : 0;
-    for (; i < count; i += stride) {
-        if (lane < W) R[lane] = pre;
-        for (int w = 64 + lane; w < W; w += 64) R[w] = rec[i * W + w];
-        const u64 p = readlane64(pre, REC_P), mp = readlane64(pre, REC_MP);
-        const u64 g64 = readlane64(pre, REC_G64);
-        const u32 mp0 = lo32(mp);
-        const long long ni = i + stride;
-        pre = (ni < count && lane < W) ? rec[ni * W + lane] : 0;
+    if (lane < W) R[lane] = pre;
+    for (int w = 64 + lane; w < W; w += 64) R[w] = Rg[w];
+    const u64 p = readlane64(pre, REC_P), mp = readlane64(pre, REC_MP);
+    const u64 g64 = readlane64(pre, REC_G64);
+    const u32 mp0 = lo32(mp);
 
-        // clear table and counts
-        for (u32 b = lane; b < NB; b += 64) tab[b] = make_uint4(~0u, ~0u, ~0u, ~0u);
-        for (u32 c = lane; c < NB / 2; c += 64) cnt[c] = 0;
-        wave_sync();
+    // clear table and counts
+    for (u32 b = lane; b < NB; b += 64) tab[b] = make_uint4(~0u, ~0u, ~0u, ~0u);
+    for (u32 c = lane; c < NB / 2; c += 64) cnt[c] = 0;
+    wave_sync();
 
-        // baby steps: lane l inserts g^j, j = l + 64 s, as a fingerprint in the
-        // first free slot of its bucket (or of the next non-full bucket)
-        const u64 vstart = montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp);
-        u64 v = vstart;
-        for (int s = 0; s < L; ++s) {
-            u32 b = lo32(v) & (NB - 1);
-            const u32 fp = fingerprint(v, J.logNB);
+    // baby steps: lane l inserts g^j, j = l + 64 s, as a fingerprint in the
+    // first free slot of its bucket (or of the next non-full bucket)
+    const u64 vstart = montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp);
+    u64 v = vstart;
+    for (int s = 0; s < L; ++s) {
+        u32 b = lo32(v) & (NB - 1);
+        const u32 fp = fingerprint(v, J.logNB);
+        while (true) {
+            const u32 sh = (b & 1) * 16;
+            const u32 c = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
+            if (c < 8) {
+                tab16[b * 8 + c] = (uint16_t)fp;
+                break;
+            }
+            b = (b + 1) & (NB - 1);
+        }
+        if (s + 1 < L) v = montmul(v, g64, p, mp);
+    }
+
+    // Y[r*T + t] = b^r H G^t 2^32 (Montgomery form)
+    for (int idx = lane; idx < Q * T; idx += 64) {
+        const int r = div_T((u32)idx, J.invT), t = idx - r * T;
+        Y[idx] = montmul(R[REC_BH + r], R[J.off_gt + t], p, mp);
+    }
+
+    // rows passing the Jacobi filter, compacted in order as (k, rT << 8 | r)
+    int A = 0;
+    for (int c = 0; c < J.nrows; c += 64) {
+        const int row = c + lane;
+        bool pass = false;
+        u32 k = 0, r = 0;
+        if (row < J.nrows) {
+            const u32 info = J.rowinfo[row];
+            const u32 kidx = info & 0xffffu;
+            r = info >> 16;
+            k = J.rowk[row];
+            const u64 w = R[J.off_j + 2 * (kidx >> 6) + (r & 1)];
+            pass = (w >> (kidx & 63)) & 1;
+        }
+        const u64 mask = __ballot(pass);
+        if (pass)
+            list[A + __popcll(mask & ((1ull << lane) - 1))] =
+                make_uint2(k, ((r * (u32)T) << 8) | r);
+        A += __popcll(mask);
+    }
+    wave_sync();
+
+    // giant steps: item (a, t) of the A*T grid, lanes stride by 64; a probe
+    // reads one bucket (8 fingerprints) and follows full buckets.  The loop
+    // is wave-uniform: the rescan below needs every lane.
+    const int total = A * T;
+    int a = div_T((u32)lane, J.invT), t = lane - a * T;
+    for (int it0 = 0; it0 < total; it0 += 64) {
+        const bool act = it0 + lane < total;
+        bool match = false;
+        u64 s = 0;
+        uint2 row = make_uint2(0, 0);
+        if (act) {
+            row = list[a];
+            s = halfmul(row.x, Y[(row.y >> 8) + t], p, mp0);
+            u32 b = lo32(s) & (NB - 1);
+            const u32 fp2 = fingerprint(s, J.logNB) * 0x10001u;
             while (true) {
-                const u32 sh = (b & 1) * 16;
-                const u32 c = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
-                if (c < 8) {
-                    tab16[b * 8 + c] = (uint16_t)fp;
-                    break;
-                }
+                const uint4 e = tab[b];
+                const u32 x0 = e.x ^ fp2, x1 = e.y ^ fp2, x2 = e.z ^ fp2, x3 = e.w ^ fp2;
+                // a 16-bit half of x is zero iff that slot holds the fingerprint
+                const u32 z = ((x0 - 0x10001u) & ~x0) | ((x1 - 0x10001u) & ~x1) |
+                              ((x2 - 0x10001u) & ~x2) | ((x3 - 0x10001u) & ~x3);
+                match |= (z & 0x80008000u) != 0;
+                if ((e.w >> 16) == 0xffffu) break;  // slot 7 free: chain ends here
                 b = (b + 1) & (NB - 1);
             }
-            if (s + 1 < L) v = montmul(v, g64, p, mp);
         }
-
-        // Y[r*T + t] = b^r H G^t 2^32 (Montgomery form)
-        for (int idx = lane; idx < Q * T; idx += 64) {
-            const int r = div_T((u32)idx, J.invT), t = idx - r * T;
-            Y[idx] = montmul(R[REC_BH + r], R[J.off_gt + t], p, mp);
-        }
-
-        // rows passing the Jacobi filter, compacted in order as (k, rT << 8 | r)
-        int A = 0;
-        for (int c = 0; c < J.nrows; c += 64) {
-            const int row = c + lane;
-            bool pass = false;
-            u32 k = 0, r = 0;
-            if (row < J.nrows) {
-                const u32 info = J.rowinfo[row];
-                const u32 kidx = info & 0xffffu;
-                r = info >> 16;
-                k = J.rowk[row];
-                const u64 w = R[J.off_j + 2 * (kidx >> 6) + (r & 1)];
-                pass = (w >> (kidx & 63)) & 1;
-            }
-            const u64 mask = __ballot(pass);
-            if (pass)
-                list[A + __popcll(mask & ((1ull << lane) - 1))] =
-                    make_uint2(k, ((r * (u32)T) << 8) | r);
-            A += __popcll(mask);
-        }
-        wave_sync();
-
-        // giant steps: item (a, t) of the A*T grid, lanes stride by 64; a probe
-        // reads one bucket (8 fingerprints) and follows full buckets.  The
-        // loop is wave-uniform: the rescan below needs every lane.
-        const int total = A * T;
-        int a = div_T((u32)lane, J.invT), t = lane - a * T;
-        for (int it0 = 0; it0 < total; it0 += 64) {
-            const bool act = it0 + lane < total;
-            bool match = false;
-            u64 s = 0;
-            uint2 row = make_uint2(0, 0);
-            if (act) {
-                row = list[a];
-                s = halfmul(row.x, Y[(row.y >> 8) + t], p, mp0);
-                u32 b = lo32(s) & (NB - 1);
-                const u32 fp2 = fingerprint(s, J.logNB) * 0x10001u;
-                while (true) {
-                    const uint4 e = tab[b];
-                    const u32 x0 = e.x ^ fp2, x1 = e.y ^ fp2, x2 = e.z ^ fp2, x3 = e.w ^ fp2;
-                    // a 16-bit half of x is zero iff that slot holds the fingerprint
-                    const u32 z = ((x0 - 0x10001u) & ~x0) | ((x1 - 0x10001u) & ~x1) |
-                                  ((x2 - 0x10001u) & ~x2) | ((x3 - 0x10001u) & ~x3);
-                    match |= (z & 0x80008000u) != 0;
-                    if ((e.w >> 16) == 0xffffu) break;  // slot 7 free: chain ends here
-                    b = (b + 1) & (NB - 1);
-                }
-            }
-            // fingerprint match: all lanes rescan the baby steps (lane l checks
-            // j = l + 64 u) for exact equalities with the matching lane's s
-            u64 mb = __ballot(match);
-            while (mb) {
-                const int src = __builtin_ctzll(mb);
-                mb &= mb - 1;
-                const u64 target = readlane64(s, src);
-                const u32 kk = __builtin_amdgcn_readlane(row.x, src);
-                const u32 ry = __builtin_amdgcn_readlane(row.y, src);
-                const int tt = __builtin_amdgcn_readlane(t, src);
-                u64 w = vstart;
-                for (int u = 0; u < L; ++u) {
-                    if (w == target) {
-                        const int j = lane + 64 * u;
-                        const long long xx = (long long)J.x0 + (long long)tt * m + (m - 1 - j);
-                        const long long n = (long long)Q * xx + (int)(ry & 0xffu);
-                        if (n >= J.minN && n <= J.maxN) {
-                            const u32 at = atomicAdd(hit_count, 1u);
-                            if (at < hit_cap) {
-                                HitDev h;
-                                h.p = p;
-                                h.k = kk;
-                                h.n = (u32)n;
-                                hits[at] = h;
-                            }
+        // fingerprint match: all lanes rescan the baby steps (lane l checks
+        // j = l + 64 u) for exact equalities with the matching lane's s
+        u64 mb = __ballot(match);
+        while (mb) {
+            const int src = __builtin_ctzll(mb);
+            mb &= mb - 1;
+            const u64 target = readlane64(s, src);
+            const u32 kk = __builtin_amdgcn_readlane(row.x, src);
+            const u32 ry = __builtin_amdgcn_readlane(row.y, src);
+            const int tt = __builtin_amdgcn_readlane(t, src);
+            u64 w = vstart;
+            for (int u = 0; u < L; ++u) {
+                if (w == target) {
+                    const int j = lane + 64 * u;
+                    const long long xx = (long long)J.x0 + (long long)tt * m + (m - 1 - j);
+                    const long long n = (long long)Q * xx + (int)(ry & 0xffu);
+                    if (n >= J.minN && n <= J.maxN) {
+                        const u32 at = atomicAdd(hit_count, 1u);
+                        if (at < hit_cap) {
+                            HitDev h;
+                            h.p = p;
+                            h.k = kk;
+                            h.n = (u32)n;
+                            h.job = (u32)job;
+                            h.pad = 0;
+                            hits[at] = h;
                         }
                     }
-                    w = montmul(w, g64, p, mp);
                 }
+                w = montmul(w, g64, p, mp);
             }
-            t += dT;
-            a += dA;
-            if (t >= T) { t -= T; a += 1; }
         }
-        wave_sync();  // the next prime reuses the table
+        t += dT;
+        a += dA;
+        if (t >= T) { t -= T; a += 1; }
+    }
+    wave_sync();  // the next prime reuses the table
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
+                const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
+                int lds_words, int scratch_words, uint64_t* __restrict__ scratch,
+                unsigned long long* __restrict__ work, HitDev* __restrict__ hits,
+                uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
+    extern __shared__ uint32_t lds[];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    u32* const wbase = lds + wv * lds_words;
+    u64* const S = scratch + ((long long)blockIdx.x * wpb + wv) * scratch_words;
+
+    while (true) {
+        unsigned long long grp = 0;
+        if (lane == 0) grp = atomicAdd(work, 1ull);
+        grp = ((unsigned long long)__builtin_amdgcn_readfirstlane((u32)(grp >> 32)) << 32) |
+              __builtin_amdgcn_readfirstlane((u32)grp);
+        if ((long long)grp >= ngroups) break;
+        // the chunk holding this group (chunks sorted by group0)
+        int ci = 0;
+        if (chunks) {
+            int hi = nchunks - 1;
+            while (ci < hi) {
+                const int mid = (ci + hi + 1) >> 1;
+                if (chunks[mid].group0 <= (long long)grp) ci = mid;
+                else hi = mid - 1;
+            }
+        }
+        const ChunkDev C = chunks ? chunks[ci] : c0;
+        const JobDev J = jobs ? jobs[C.job] : j0;
+        const long long q0 = ((long long)grp - C.group0) * 64;
+        const int n = (int)min(64ll, C.count - q0);
+
+        // set-up, one lane per prime
+        if (lane < n) prep_record(J, C.low + C.offs[q0 + lane], S + lane * J.W);
+        wave_sync_global();
+
+        // BSGS, the wavefront per prime; record words 0..63 of the next prime
+        // are prefetched into one register per lane
+        u64 pre = lane < J.W ? S[lane] : 0;
+        for (int i = 0; i < n; ++i) {
+            const u64 cur = pre;
+            if (i + 1 < n) pre = lane < J.W ? S[(i + 1) * J.W + lane] : 0;
+            bsgs_prime(J, C.job, cur, S + i * J.W, wbase, hits, hit_count, hit_cap);
+        }
+        wave_sync_global();  // records are rewritten by the next group
     }
 }
 

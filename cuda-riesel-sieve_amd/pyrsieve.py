@@ -36,7 +36,8 @@ class Factor(ctypes.Structure):
     _fields_ = [("p", u64), ("k", u32), ("n", u32)]
 
 
-FACTOR_DTYPE = np.dtype([("p", np.uint64), ("k", np.uint32), ("n", np.uint32)])
+class JobFactor(ctypes.Structure):
+    _fields_ = [("p", u64), ("k", u32), ("n", u32), ("job", u32), ("pad", u32)]
 
 _SIGS = {
     "rs_strerror": (ctypes.c_char_p, [i32]),
@@ -66,8 +67,16 @@ _SIGS = {
     "rs_sieve_submit_device": (i32, [vp, u64, vp, i64]),
     "rs_sieve_collect": (i32, [vp, P(Factor), i64, P(i64), P(i64)]),
     "rs_sieve_sync": (i32, [vp]),
-    "rs_sieve_timing": (i32, [vp, P(ctypes.c_double), P(ctypes.c_double), P(i64)]),
+    "rs_sieve_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
     "rs_sieve_destroy": (None, [vp]),
+    "rs_batch_create": (i32, [i32, i32, P(vp), P(i32), i64, P(vp)]),
+    "rs_batch_job_info": (i32, [vp, i32, P(i32), P(i32), P(i32)]),
+    "rs_batch_add_chunk": (i32, [vp, i32, u64, vp, i64]),
+    "rs_batch_clear": (i32, [vp]),
+    "rs_batch_run": (i32, [vp]),
+    "rs_batch_collect": (i32, [vp, P(JobFactor), i64, P(i64), P(i64)]),
+    "rs_batch_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
+    "rs_batch_destroy": (None, [vp]),
     "rs_dev_montmul": (i32, [i32, P(u64), P(u64), P(u64), P(u64), i64]),
     "rs_dev_halfmul": (i32, [i32, P(u32), P(u64), P(u64), P(u64), i64]),
     "rs_dev_jacobi": (i32, [i32, P(u32), P(u64), P(i32), i64]),
@@ -335,14 +344,76 @@ class Sieve:
         _chk("rs_sieve_sync", lib().rs_sieve_sync(self.h))
 
     def timing(self):
-        a, b, n = ctypes.c_double(), ctypes.c_double(), i64()
-        _chk("rs_sieve_timing", lib().rs_sieve_timing(self.h, ctypes.byref(a), ctypes.byref(b),
-                                                      ctypes.byref(n)))
-        return dict(bsgs_ms=a.value, prep_ms=b.value, primes=n.value)
+        """Kernel ms and primes since the last call (HIP events)."""
+        a, n = ctypes.c_double(), i64()
+        _chk("rs_sieve_timing", lib().rs_sieve_timing(self.h, ctypes.byref(a), ctypes.byref(n)))
+        return dict(kernel_ms=a.value, primes=n.value)
 
     def close(self):
         if self.h:
             lib().rs_sieve_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        if _lib is not None and self.h:
+            self.close()
+
+
+class Batch:
+    """Many jobs sieved in one persistent (graph-captured) launch."""
+
+    h = vp()
+
+    def __init__(self, plans, ms=None, hit_capacity=1 << 20, device=0):
+        hs = (vp * len(plans))(*[pl.h.value for pl in plans])
+        msa = np.ascontiguousarray(ms if ms is not None else [0] * len(plans), np.int32)
+        h = vp()
+        _chk("rs_batch_create", lib().rs_batch_create(device, len(plans), hs, _p(msa, i32),
+                                                      hit_capacity, ctypes.byref(h)))
+        self.h = h
+        self.plans = plans
+        self.info = []
+        for j in range(len(plans)):
+            mm, T, lds = i32(), i32(), i32()
+            _chk("rs_batch_job_info", lib().rs_batch_job_info(self.h, j, ctypes.byref(mm),
+                                                              ctypes.byref(T), ctypes.byref(lds)))
+            self.info.append(dict(m=mm.value, T=T.value, lds_bytes=lds.value))
+
+    def add_chunk(self, job, low, dbuf, start=0, count=None):
+        count = dbuf.count - start if count is None else count
+        _chk("rs_batch_add_chunk", lib().rs_batch_add_chunk(self.h, job, low,
+                                                            vp(dbuf.ptr.value + 4 * start),
+                                                            count))
+
+    def clear(self):
+        _chk("rs_batch_clear", lib().rs_batch_clear(self.h))
+
+    def run(self):
+        _chk("rs_batch_run", lib().rs_batch_run(self.h))
+
+    def collect(self):
+        """Hits of the last run as a sorted list of (job, p, k, n)."""
+        cnt, lost = i64(), i64()
+        cap = 4096
+        while True:
+            buf = (JobFactor * cap)()
+            st = lib().rs_batch_collect(self.h, buf, cap, ctypes.byref(cnt), ctypes.byref(lost))
+            if st == RS_ERR_OVERFLOW:
+                cap = cnt.value
+                continue
+            _chk("rs_batch_collect", st)
+            if lost.value:
+                raise RuntimeError(f"device hit buffer overflowed: {lost.value} hits lost")
+            return [(int(f.job), int(f.p), int(f.k), int(f.n)) for f in buf[: cnt.value]]
+
+    def timing(self):
+        a, n = ctypes.c_double(), i64()
+        _chk("rs_batch_timing", lib().rs_batch_timing(self.h, ctypes.byref(a), ctypes.byref(n)))
+        return dict(kernel_ms=a.value, primes=n.value)
+
+    def close(self):
+        if self.h:
+            lib().rs_batch_destroy(self.h)
             self.h = vp()
 
     def __del__(self):

@@ -136,9 +136,10 @@ int rs_format_factor(const rs_factor* f, uint32_t b, char* buf, int cap);
 /* ------------------------------------------------------------ device sieve */
 /* One sieve job on one GPU: base, Q, table size m (multiple of 64, or 0 for
  * the cost model's pick), rows and n window from a plan.  Replaces the
- * set-up at :1367-1512.  max_batch bounds the primes of one launch (the
- * reference's arraySize, :851); hit_capacity bounds the device append
- * buffer (the reference keeps one slot per prime, :615-621). */
+ * set-up at :1367-1512.  max_batch bounds the primes copied from host
+ * memory per launch (the reference's arraySize, :851); hit_capacity bounds
+ * the device append buffer (the reference keeps one slot per prime,
+ * :615-621). */
 typedef struct rs_sieve rs_sieve;
 
 rs_status rs_sieve_create(int device, const rs_plan* pl, int m,
@@ -152,7 +153,8 @@ rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
  * Replaces :1543-1578. */
 rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* offsets,
                           int64_t count);
-/* Same, with offsets already in device memory (HBM-resident input). */
+/* Same, with offsets already in device memory (HBM-resident input): one
+ * persistent launch, asynchronous. */
 rs_status rs_sieve_submit_device(rs_sieve* s, uint64_t low,
                                  const uint32_t* offsets_dev, int64_t count);
 /* Waits for everything submitted and returns all hits found since the last
@@ -164,11 +166,47 @@ rs_status rs_sieve_collect(rs_sieve* s, rs_factor* out, int64_t cap,
                            int64_t* count, int64_t* lost);
 /* Waits for all submitted work. */
 rs_status rs_sieve_sync(rs_sieve* s);
-/* Device time of the sieve kernels since the last call, in ms, and the
- * number of primes processed (from HIP events on the compute stream). */
-rs_status rs_sieve_timing(rs_sieve* s, double* bsgs_ms, double* prep_ms,
-                          int64_t* primes);
+/* Device time of the sieve kernel since the last call, in ms (HIP events on
+ * the compute stream), and the number of primes it processed. */
+rs_status rs_sieve_timing(rs_sieve* s, double* kernel_ms, int64_t* primes);
 void rs_sieve_destroy(rs_sieve* s);
+
+/* ------------------------------------------------------------ batched jobs */
+/* Many sieve jobs (different bases, k counts, n windows and table sizes)
+ * in one persistent launch on one GPU.  No reference counterpart: the
+ * reference runs one job per process.  ms[i] = 0 takes the cost model's m
+ * for plans[i]; the launch shares one LDS layout sized for the largest job. */
+typedef struct rs_batch rs_batch;
+
+/* One hit of a batched launch: p | k*b^n-1 for job `job`. */
+typedef struct rs_job_factor {
+    uint64_t p;
+    uint32_t k;
+    uint32_t n;
+    uint32_t job;
+    uint32_t pad;
+} rs_job_factor;
+
+rs_status rs_batch_create(int device, int njobs, const rs_plan* const* plans,
+                          const int* ms, int64_t hit_capacity, rs_batch** out);
+rs_status rs_batch_job_info(const rs_batch* b, int job, int* m, int* T,
+                            int* lds_bytes);
+/* Queues primes p = low + offsets_dev[i] (device memory) for job `job`. */
+rs_status rs_batch_add_chunk(rs_batch* b, int job, uint64_t low,
+                             const uint32_t* offsets_dev, int64_t count);
+/* Drops every queued chunk. */
+rs_status rs_batch_clear(rs_batch* b);
+/* Sieves every queued chunk of every job in one persistent launch, captured
+ * in a hipGraph when the chunk list changed and replayed otherwise.
+ * Asynchronous; call rs_batch_collect before the next run. */
+rs_status rs_batch_run(rs_batch* b);
+/* Waits for the run and returns its hits sorted by (job, p, k, n); same
+ * overflow contract as rs_sieve_collect. */
+rs_status rs_batch_collect(rs_batch* b, rs_job_factor* out, int64_t cap,
+                           int64_t* count, int64_t* lost);
+/* Kernel time (HIP events) and primes of the runs since the last call. */
+rs_status rs_batch_timing(rs_batch* b, double* kernel_ms, int64_t* primes);
+void rs_batch_destroy(rs_batch* b);
 
 /* Device-side helpers for tests (exact arithmetic on the GPU): out[i] =
  * montmul(a[i], b[i]) mod p[i] (R = 2^64) and out[i] = k[i]*y[i]*2^-32
