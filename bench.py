@@ -42,7 +42,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--m", type=int, default=0, help="baby steps (0 = cost model)")
+    ap.add_argument("--m", type=int, default=0,
+                    help="baby steps (0 = autotuned, or the cost model with --no-tune)")
+    ap.add_argument("--tile", type=int, default=0, help="giant-step tile (0 = autotuned / 2)")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--chunk-log2", type=int, default=30)
     ap.add_argument("--max-batch", type=int, default=1 << 22,
@@ -84,8 +87,16 @@ def main():
     gen_s = time.time() - t
     nprimes = sum(counts)
     log(f"primes: {nprimes} in [{p0}, {p0 + K * chunk}) generated+uploaded in {gen_s:.1f}s")
-    sv = rs.Sieve(plan, m=args.m, max_batch=args.max_batch, hit_capacity=1 << 22)
-    log(f"sieve: m={sv.m} T={sv.T} slots={sv.slots} lds/wave={sv.lds_bytes}B")
+    tune = None
+    m, tile = args.m, args.tile
+    if not args.no_tune and m == 0:
+        ts = time.time()
+        m, tile, cands = rs.autotune(plan, p0, bufs[0], count=min(counts[0], 1 << 20))
+        tune = {"sample_primes": min(counts[0], 1 << 20), "best_m": m, "best_tile": tile,
+                "candidates": [[cm, ct, round(ns, 4)] for cm, ct, ns in cands]}
+        log(f"autotune: m={m} tile={tile} ({len(cands)} candidates, {time.time() - ts:.1f}s)")
+    sv = rs.Sieve(plan, m=m, tile=tile, max_batch=args.max_batch, hit_capacity=1 << 22)
+    log(f"sieve: m={sv.m} tile={sv.tile} T={sv.T} slots={sv.slots} lds/wave={sv.lds_bytes}B")
     for w in range(W):
         sv.submit_device(p0 + (w % K) * chunk, bufs[w % K])
         sv.collect()
@@ -114,11 +125,15 @@ def main():
     bsgs_s_per_launch = tm["kernel_ms"] / 1e3 / launches
     achieved = muls_per_prime * per_launch_primes / bsgs_s_per_launch
     peak = rs.dev_mad_rate()
+    # streaming copy of a known size: HBM reference for the traffic counters
+    copy_bytes = 1 << 30
+    hbm = rs.dev_copy_rate(copy_bytes)
     roof = {"bound": "valu", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "Gmul/s",
             "frac": achieved / peak, "traffic": None,
             "kernel": "rs_sieve_kernel", "muls_per_prime": muls_per_prime,
             "primes_per_launch": per_launch_primes,
-            "mean_launch_ms": bsgs_s_per_launch * 1e3}
+            "mean_launch_ms": bsgs_s_per_launch * 1e3,
+            "hbm_copy_GBps": hbm / 1e9, "copy_probe_bytes": copy_bytes}
 
     cpu = None
     if not args.no_cpu:
@@ -157,7 +172,7 @@ def main():
         "config": {"workload": "R745-shaped plain BSGS: base 745, 22 odd k in [3,1e7] "
                                f"(seed {args.seed}), n in [180000,250000] presieved by odd "
                                f"primes < 1e5, p in [1e12, 1e12+{K}*2^{args.chunk_log2})",
-                   "Q": plan.Q, "rows": plan.nrows, "m": sv.m, "T": sv.T,
+                   "Q": plan.Q, "rows": plan.nrows, "m": sv.m, "T": sv.T, "tile": sv.tile,
                    "primes": nprimes, "chunk": chunk, "global_batch": counts[0],
                    "parallelism": "single GPU, wavefront per prime"},
         "primes_per_sec": primes_rate,
@@ -166,6 +181,7 @@ def main():
         "host_primegen_s": gen_s,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "autotune": tune,
     }
     print(json.dumps(out), flush=True)
 

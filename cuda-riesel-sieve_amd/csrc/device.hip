@@ -56,10 +56,17 @@ void free_job(DevJob& j) {
     j.d_kvals = j.d_rowinfo = j.d_rowk = nullptr;
 }
 
-// Fills the job constants from a plan and table size m (0: cost model).
-rs_status make_job(const rs_plan* pl, int m, DevJob* dj) {
+uint32_t magic_div(int d) {
+    return d == 1 ? 0u : (uint32_t)((0x100000000ull + d - 1) / d);
+}
+
+// Fills the job constants from a plan, table size m (0: cost model) and
+// giant-step tile tw (0: 2).
+rs_status make_job(const rs_plan* pl, int m, int tw, DevJob* dj) {
     if (!pl) return RS_ERR_ARG;
     if (m <= 0) m = rs_plan_auto_m(pl);
+    if (tw <= 0) tw = 2;
+    if (tw > 4) return RS_ERR_ARG;
     if (m < 64 || m % 64 != 0 || m > (1 << 16)) return RS_ERR_ARG;
     if (pl->Q < 2 || pl->Q % 2 != 0 || pl->minN < 1 || pl->rows_k.empty()) return RS_ERR_ARG;
     const int nrows = (int)pl->rows_k.size();
@@ -89,7 +96,10 @@ rs_status make_job(const rs_plan* pl, int m, DevJob* dj) {
     while ((1 << logNB) * 3 < m) ++logNB;
     J.logNB = std::max(logNB, 2);
     if (J.Q * J.T >= 65536 || J.T > 65535) return RS_ERR_ARG;
-    J.invT = J.T == 1 ? 0u : (uint32_t)((0x100000000ull + J.T - 1) / J.T);
+    J.invT = magic_div(J.T);
+    J.tw = std::min(tw, J.T);
+    J.ntiles = (J.T + J.tw - 1) / J.tw;
+    J.invNT = magic_div(J.ntiles);
     J.W = rec_words(J.Q, J.T, J.nk);
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + J.T;
@@ -263,8 +273,8 @@ static void free_sieve(rs_sieve* s) {
     delete s;
 }
 
-extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int64_t max_batch,
-                                     int64_t hit_capacity, rs_sieve** out) {
+extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int tile,
+                                     int64_t max_batch, int64_t hit_capacity, rs_sieve** out) {
     if (!pl || !out || max_batch < 1 || hit_capacity < 1 || hit_capacity > 0x7fffffff)
         return RS_ERR_ARG;
     *out = nullptr;
@@ -272,7 +282,7 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int64
     rs_sieve* s = new rs_sieve();
     s->device = device;
     s->max_batch = max_batch;
-    rs_status st = make_job(pl, m, &s->job);
+    rs_status st = make_job(pl, m, tile, &s->job);
     if (st == RS_OK) st = init_launcher(&s->L, device, s->job.J.lds_words, s->job.J.W, hit_capacity);
     if (st == RS_OK &&
         (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -291,8 +301,9 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int64
 }
 
 extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
-                                   int* lds_bytes_per_wave) {
+                                   int* lds_bytes_per_wave, int* tile) {
     if (!s) return RS_ERR_ARG;
+    if (tile) *tile = s->job.J.tw;
     if (m) *m = s->job.J.m;
     if (T) *T = s->job.J.T;
     if (table_slots) *table_slots = 8 << s->job.J.logNB;
@@ -415,6 +426,63 @@ extern "C" rs_status rs_sieve_timing(rs_sieve* s, double* kernel_ms, int64_t* pr
 
 extern "C" void rs_sieve_destroy(rs_sieve* s) { free_sieve(s); }
 
+// ------------------------------------------------------------ autotuner
+
+extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
+                                 const uint32_t* offsets_dev, int64_t count, int* best_m,
+                                 int* best_tile, int* cand_m, int* cand_tile,
+                                 double* ns_per_prime, int cap, int* ncand) {
+    if (!pl || !offsets_dev || count < 1 || !best_m || !best_tile) return RS_ERR_ARG;
+    int max_lds = 0;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    const int m0 = rs_plan_auto_m(pl);
+    std::vector<int> ms;
+    for (int num = 4; num <= 16; ++num) {  // m0 * num / 8, rounded to 64
+        const int m = std::max(64, (m0 * num / 8 + 32) / 64 * 64);
+        if (std::find(ms.begin(), ms.end(), m) == ms.end()) ms.push_back(m);
+    }
+    double best = 1e300;
+    int n = 0;
+    *best_m = 0;
+    *best_tile = 0;
+    for (int m : ms) {
+        for (int tile : {1, 2, 4}) {
+            rs_sieve* s = nullptr;
+            rs_status st = rs_sieve_create(device, pl, m, tile, 1, 1 << 16, &s);
+            if (st == RS_ERR_ARG) break;  // m does not fit: larger m will not either
+            if (st != RS_OK) return st;
+            if (s->job.J.lds_words * 4 > max_lds) {
+                rs_sieve_destroy(s);
+                break;
+            }
+            double ms_time = 0;
+            int64_t primes = 0;
+            // one untimed pass, then the timed one
+            st = rs_sieve_submit_device(s, low, offsets_dev, count);
+            if (st == RS_OK) st = rs_sieve_timing(s, nullptr, nullptr);
+            if (st == RS_OK) st = rs_sieve_submit_device(s, low, offsets_dev, count);
+            if (st == RS_OK) st = rs_sieve_timing(s, &ms_time, &primes);
+            rs_sieve_destroy(s);
+            if (st != RS_OK) return st;
+            const double ns = ms_time * 1e6 / (double)count;
+            if (n < cap) {
+                if (cand_m) cand_m[n] = m;
+                if (cand_tile) cand_tile[n] = tile;
+                if (ns_per_prime) ns_per_prime[n] = ns;
+            }
+            ++n;
+            if (ns < best) {
+                best = ns;
+                *best_m = m;
+                *best_tile = tile;
+            }
+        }
+    }
+    if (ncand) *ncand = n;
+    return *best_m ? RS_OK : RS_ERR_ARG;
+}
+
 // ------------------------------------------------------------ batched jobs
 
 struct rs_batch {
@@ -455,7 +523,8 @@ static void free_batch(rs_batch* b) {
 }
 
 extern "C" rs_status rs_batch_create(int device, int njobs, const rs_plan* const* plans,
-                                     const int* ms, int64_t hit_capacity, rs_batch** out) {
+                                     const int* ms, const int* tiles, int64_t hit_capacity,
+                                     rs_batch** out) {
     if (!out || njobs < 1 || !plans || hit_capacity < 1 || hit_capacity > 0x7fffffff)
         return RS_ERR_ARG;
     *out = nullptr;
@@ -466,7 +535,7 @@ extern "C" rs_status rs_batch_create(int device, int njobs, const rs_plan* const
     rs_status st = RS_OK;
     int lds_words = 0, W = 0;
     for (int i = 0; i < njobs && st == RS_OK; ++i) {
-        st = make_job(plans[i], ms ? ms[i] : 0, &b->jobs[i]);
+        st = make_job(plans[i], ms ? ms[i] : 0, tiles ? tiles[i] : 0, &b->jobs[i]);
         if (st == RS_OK) {
             lds_words = std::max(lds_words, b->jobs[i].J.lds_words);
             W = std::max(W, b->jobs[i].J.W);
@@ -492,8 +561,9 @@ extern "C" rs_status rs_batch_create(int device, int njobs, const rs_plan* const
 }
 
 extern "C" rs_status rs_batch_job_info(const rs_batch* b, int job, int* m, int* T,
-                                       int* lds_bytes) {
+                                       int* lds_bytes, int* tile) {
     if (!b || job < 0 || job >= (int)b->jobs.size()) return RS_ERR_ARG;
+    if (tile) *tile = b->jobs[job].J.tw;
     if (m) *m = b->jobs[job].J.m;
     if (T) *T = b->jobs[job].J.T;
     if (lds_bytes) *lds_bytes = b->jobs[job].J.lds_words * 4;

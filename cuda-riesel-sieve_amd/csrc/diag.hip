@@ -13,6 +13,7 @@ extern "C" __global__ void rs_test_halfmul(const uint32_t*, const uint64_t*, con
                                            uint64_t*, long long);
 extern "C" __global__ void rs_test_jacobi(const uint32_t*, const uint64_t*, int32_t*, long long);
 extern "C" __global__ void rs_mad_probe(uint64_t*, int);
+extern "C" __global__ void rs_copy_probe(const uint4*, uint4*, long long);
 
 #define HIPCHK(x)                                 \
     do {                                          \
@@ -151,5 +152,37 @@ extern "C" rs_status rs_dev_mad_rate(int device, int blocks, int iters, double* 
     (void)hipEventDestroy(b);
     (void)hipFree(out);
     *ops_per_s = (double)blocks * 256.0 * 8.0 * iters / (ms * 1e-3);
+    return RS_OK;
+}
+
+extern "C" rs_status rs_dev_copy_rate(int device, int64_t bytes, double* bytes_per_s) {
+    if (!bytes_per_s || bytes < 16 || bytes % 16 != 0) return RS_ERR_ARG;
+    HIPCHK(hipSetDevice(device));
+    uint4 *src = nullptr, *dst = nullptr;
+    if (hipMalloc(&src, (size_t)bytes) != hipSuccess) return RS_ERR_NOMEM;
+    if (hipMalloc(&dst, (size_t)bytes) != hipSuccess) {
+        (void)hipFree(src);
+        return RS_ERR_NOMEM;
+    }
+    HIPCHK(hipMemset(src, 1, (size_t)bytes));
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    const long long n = bytes / 16;
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const int grid = cus * 16;
+    hipLaunchKernelGGL(rs_copy_probe, dim3(grid), dim3(256), 0, 0, src, dst, n);  // warm-up
+    HIPCHK(hipEventRecord(a, 0));
+    hipLaunchKernelGGL(rs_copy_probe, dim3(grid), dim3(256), 0, 0, src, dst, n);
+    HIPCHK(hipEventRecord(b, 0));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    *bytes_per_s = 2.0 * (double)bytes / (ms * 1e-3);  // read + write
     return RS_OK;
 }

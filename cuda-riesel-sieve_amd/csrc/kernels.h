@@ -26,6 +26,9 @@ struct JobDev {
     int nk;       // distinct k
     int logNB;    // baby table: NB = 2^logNB buckets of 8 16-bit fingerprints
     uint32_t invT;  // ceil(2^32 / T) for T >= 2, else 0: idx / T for idx < 2^16
+    int tw;         // giant-step tile: t-steps per lane item, 1..4
+    int ntiles;     // ceil(T / tw) tiles per row
+    uint32_t invNT;  // ceil(2^32 / ntiles) for ntiles >= 2, else 0
     int W;        // record words
     int off_gt;   // REC_BH + Q
     int off_j;    // off_gt + T

@@ -140,7 +140,6 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     uint2* const list = (uint2*)(Y + Q * T);
     u64* const R = (u64*)(list + J.nrows);
     const int L = m >> 6;
-    const int dA = 64 / T, dT = 64 - dA * T;
 
     if (lane < W) R[lane] = pre;
     for (int w = 64 + lane; w < W; w += 64) R[w] = Rg[w];
@@ -200,67 +199,78 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     }
     wave_sync();
 
-    // giant steps: item (a, t) of the A*T grid, lanes stride by 64; a probe
+    // giant steps: item (a, c) of the A x ntiles grid covers t-steps
+    // t = c*tw .. c*tw + tw - 1 of row a; lanes stride by 64 items.  A probe
     // reads one bucket (8 fingerprints) and follows full buckets.  The loop
     // is wave-uniform: the rescan below needs every lane.
-    const int total = A * T;
-    int a = div_T((u32)lane, J.invT), t = lane - a * T;
+    const int TW = J.tw, NT = J.ntiles;
+    const int total = A * NT;
+    const int dAi = 64 / NT, dCi = 64 - dAi * NT;
+    int a = div_T((u32)lane, J.invNT), c = lane - a * NT;
     for (int it0 = 0; it0 < total; it0 += 64) {
         const bool act = it0 + lane < total;
-        bool match = false;
-        u64 s = 0;
-        uint2 row = make_uint2(0, 0);
-        if (act) {
-            row = list[a];
-            s = halfmul(row.x, Y[(row.y >> 8) + t], p, mp0);
-            u32 b = lo32(s) & (NB - 1);
-            const u32 fp2 = fingerprint(s, J.logNB) * 0x10001u;
-            while (true) {
-                const uint4 e = tab[b];
-                const u32 x0 = e.x ^ fp2, x1 = e.y ^ fp2, x2 = e.z ^ fp2, x3 = e.w ^ fp2;
-                // a 16-bit half of x is zero iff that slot holds the fingerprint
-                const u32 z = ((x0 - 0x10001u) & ~x0) | ((x1 - 0x10001u) & ~x1) |
-                              ((x2 - 0x10001u) & ~x2) | ((x3 - 0x10001u) & ~x3);
-                match |= (z & 0x80008000u) != 0;
-                if ((e.w >> 16) == 0xffffu) break;  // slot 7 free: chain ends here
-                b = (b + 1) & (NB - 1);
+        const uint2 row = act ? list[a] : make_uint2(0, 0);
+        const u32 rT = row.y >> 8;
+        u64 sv[4] = {0, 0, 0, 0};
+        u32 mbits = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = c * TW + u;
+            if (u < TW && act && t < T) {
+                const u64 s = halfmul(row.x, Y[rT + t], p, mp0);
+                sv[u] = s;
+                u32 b = lo32(s) & (NB - 1);
+                const u32 fp2 = fingerprint(s, J.logNB) * 0x10001u;
+                while (true) {
+                    const uint4 e = tab[b];
+                    const u32 x0 = e.x ^ fp2, x1 = e.y ^ fp2, x2 = e.z ^ fp2, x3 = e.w ^ fp2;
+                    // a 16-bit half of x is zero iff that slot holds the fingerprint
+                    const u32 z = ((x0 - 0x10001u) & ~x0) | ((x1 - 0x10001u) & ~x1) |
+                                  ((x2 - 0x10001u) & ~x2) | ((x3 - 0x10001u) & ~x3);
+                    if (z & 0x80008000u) mbits |= 1u << u;
+                    if ((e.w >> 16) == 0xffffu) break;  // slot 7 free: chain ends here
+                    b = (b + 1) & (NB - 1);
+                }
             }
         }
         // fingerprint match: all lanes rescan the baby steps (lane l checks
-        // j = l + 64 u) for exact equalities with the matching lane's s
-        u64 mb = __ballot(match);
-        while (mb) {
-            const int src = __builtin_ctzll(mb);
-            mb &= mb - 1;
-            const u64 target = readlane64(s, src);
-            const u32 kk = __builtin_amdgcn_readlane(row.x, src);
-            const u32 ry = __builtin_amdgcn_readlane(row.y, src);
-            const int tt = __builtin_amdgcn_readlane(t, src);
-            u64 w = vstart;
-            for (int u = 0; u < L; ++u) {
-                if (w == target) {
-                    const int j = lane + 64 * u;
-                    const long long xx = (long long)J.x0 + (long long)tt * m + (m - 1 - j);
-                    const long long n = (long long)Q * xx + (int)(ry & 0xffu);
-                    if (n >= J.minN && n <= J.maxN) {
-                        const u32 at = atomicAdd(hit_count, 1u);
-                        if (at < hit_cap) {
-                            HitDev h;
-                            h.p = p;
-                            h.k = kk;
-                            h.n = (u32)n;
-                            h.job = (u32)job;
-                            h.pad = 0;
-                            hits[at] = h;
+        // j = l + 64 v) for exact equalities with the matching lane's s
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            u64 mb = __ballot((mbits >> u) & 1);
+            while (mb) {
+                const int src = __builtin_ctzll(mb);
+                mb &= mb - 1;
+                const u64 target = readlane64(sv[u], src);
+                const u32 kk = __builtin_amdgcn_readlane(row.x, src);
+                const u32 ry = __builtin_amdgcn_readlane(row.y, src);
+                const int tt = __builtin_amdgcn_readlane(c, src) * TW + u;
+                u64 w = vstart;
+                for (int v2 = 0; v2 < L; ++v2) {
+                    if (w == target) {
+                        const int j = lane + 64 * v2;
+                        const long long xx = (long long)J.x0 + (long long)tt * m + (m - 1 - j);
+                        const long long n = (long long)Q * xx + (int)(ry & 0xffu);
+                        if (n >= J.minN && n <= J.maxN) {
+                            const u32 at = atomicAdd(hit_count, 1u);
+                            if (at < hit_cap) {
+                                HitDev h;
+                                h.p = p;
+                                h.k = kk;
+                                h.n = (u32)n;
+                                h.job = (u32)job;
+                                h.pad = 0;
+                                hits[at] = h;
+                            }
                         }
                     }
+                    w = montmul(w, g64, p, mp);
                 }
-                w = montmul(w, g64, p, mp);
             }
         }
-        t += dT;
-        a += dA;
-        if (t >= T) { t -= T; a += 1; }
+        c += dCi;
+        a += dAi;
+        if (c >= NT) { c -= NT; a += 1; }
     }
     wave_sync();  // the next prime reuses the table
 }
@@ -356,6 +366,16 @@ extern "C" __global__ void __launch_bounds__(256) rs_mad_probe(uint64_t* out, in
 #pragma unroll
     for (int k = 0; k < 8; ++k) s ^= acc[k];
     out[(long long)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// Streaming copy of n uint4 (16 B) elements, grid-stride: the HBM traffic
+// calibration reference (reads and writes exactly 16 n bytes).
+extern "C" __global__ void __launch_bounds__(256) rs_copy_probe(const uint4* __restrict__ src,
+                                                                uint4* __restrict__ dst,
+                                                                long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 extern "C" __global__ void rs_zero_u32(uint32_t* x, int n) {

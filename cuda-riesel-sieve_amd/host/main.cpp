@@ -137,10 +137,10 @@ int main(int argc, char* argv[]) {
     }
 
     rs_sieve* sv = nullptr;
-    if ((st = rs_sieve_create(device, plan, m, arraySize, 1 << 20, &sv)) != RS_OK)
+    if ((st = rs_sieve_create(device, plan, m, 0, arraySize, 1 << 20, &sv)) != RS_OK)
         return fail("device set-up", st);
-    int mUsed, T, slots, ldsw;
-    rs_sieve_info(sv, &mUsed, &T, &slots, &ldsw);
+    int mUsed, T, slots, ldsw, tile;
+    rs_sieve_info(sv, &mUsed, &T, &slots, &ldsw, &tile);
     printf("Each prime uses %d baby steps, %d giant steps per row, a %d-slot LDS table "
            "(%d bytes of LDS per wavefront)\n\n", mUsed, T, slots, ldsw);
     rs_primegen* gen = nullptr;

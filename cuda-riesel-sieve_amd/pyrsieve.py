@@ -61,16 +61,18 @@ _SIGS = {
     "rs_primegen_free": (None, [vp]),
     "rs_verify_factor": (i32, [u64, u32, u32, u32]),
     "rs_format_factor": (i32, [P(Factor), u32, ctypes.c_char_p, i32]),
-    "rs_sieve_create": (i32, [i32, vp, i32, i64, i64, P(vp)]),
-    "rs_sieve_info": (i32, [vp, P(i32), P(i32), P(i32), P(i32)]),
+    "rs_sieve_create": (i32, [i32, vp, i32, i32, i64, i64, P(vp)]),
+    "rs_sieve_info": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32)]),
     "rs_sieve_submit": (i32, [vp, u64, P(u32), i64]),
     "rs_sieve_submit_device": (i32, [vp, u64, vp, i64]),
     "rs_sieve_collect": (i32, [vp, P(Factor), i64, P(i64), P(i64)]),
     "rs_sieve_sync": (i32, [vp]),
     "rs_sieve_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
     "rs_sieve_destroy": (None, [vp]),
-    "rs_batch_create": (i32, [i32, i32, P(vp), P(i32), i64, P(vp)]),
-    "rs_batch_job_info": (i32, [vp, i32, P(i32), P(i32), P(i32)]),
+    "rs_autotune": (i32, [i32, vp, u64, vp, i64, P(i32), P(i32), P(i32), P(i32),
+                          P(ctypes.c_double), i32, P(i32)]),
+    "rs_batch_create": (i32, [i32, i32, P(vp), P(i32), P(i32), i64, P(vp)]),
+    "rs_batch_job_info": (i32, [vp, i32, P(i32), P(i32), P(i32), P(i32)]),
     "rs_batch_add_chunk": (i32, [vp, i32, u64, vp, i64]),
     "rs_batch_clear": (i32, [vp]),
     "rs_batch_run": (i32, [vp]),
@@ -84,6 +86,7 @@ _SIGS = {
     "rs_dev_upload": (i32, [i32, vp, vp, i64]),
     "rs_dev_free": (i32, [i32, vp]),
     "rs_dev_mad_rate": (i32, [i32, i32, i32, P(ctypes.c_double)]),
+    "rs_dev_copy_rate": (i32, [i32, i64, P(ctypes.c_double)]),
     "rs_device_count": (i32, []),
 }
 
@@ -305,16 +308,18 @@ class Sieve:
 
     h = vp()
 
-    def __init__(self, plan, m=0, max_batch=1 << 20, hit_capacity=1 << 20, device=0):
+    def __init__(self, plan, m=0, max_batch=1 << 20, hit_capacity=1 << 20, device=0, tile=0):
         h = vp()
-        _chk("rs_sieve_create", lib().rs_sieve_create(device, plan.h, m, max_batch, hit_capacity,
-                                                      ctypes.byref(h)))
+        _chk("rs_sieve_create", lib().rs_sieve_create(device, plan.h, m, tile, max_batch,
+                                                      hit_capacity, ctypes.byref(h)))
         self.h = h
         self.plan = plan
-        mm, T, slots, lds = i32(), i32(), i32(), i32()
+        mm, T, slots, lds, tw = i32(), i32(), i32(), i32(), i32()
         _chk("rs_sieve_info", lib().rs_sieve_info(self.h, ctypes.byref(mm), ctypes.byref(T),
-                                                  ctypes.byref(slots), ctypes.byref(lds)))
+                                                  ctypes.byref(slots), ctypes.byref(lds),
+                                                  ctypes.byref(tw)))
         self.m, self.T, self.slots, self.lds_bytes = mm.value, T.value, slots.value, lds.value
+        self.tile = tw.value
 
     def submit(self, low, offsets):
         offs = np.ascontiguousarray(offsets, np.uint32)
@@ -364,20 +369,23 @@ class Batch:
 
     h = vp()
 
-    def __init__(self, plans, ms=None, hit_capacity=1 << 20, device=0):
+    def __init__(self, plans, ms=None, hit_capacity=1 << 20, device=0, tiles=None):
         hs = (vp * len(plans))(*[pl.h.value for pl in plans])
         msa = np.ascontiguousarray(ms if ms is not None else [0] * len(plans), np.int32)
+        tla = np.ascontiguousarray(tiles if tiles is not None else [0] * len(plans), np.int32)
         h = vp()
         _chk("rs_batch_create", lib().rs_batch_create(device, len(plans), hs, _p(msa, i32),
-                                                      hit_capacity, ctypes.byref(h)))
+                                                      _p(tla, i32), hit_capacity,
+                                                      ctypes.byref(h)))
         self.h = h
         self.plans = plans
         self.info = []
         for j in range(len(plans)):
-            mm, T, lds = i32(), i32(), i32()
+            mm, T, lds, tw = i32(), i32(), i32(), i32()
             _chk("rs_batch_job_info", lib().rs_batch_job_info(self.h, j, ctypes.byref(mm),
-                                                              ctypes.byref(T), ctypes.byref(lds)))
-            self.info.append(dict(m=mm.value, T=T.value, lds_bytes=lds.value))
+                                                              ctypes.byref(T), ctypes.byref(lds),
+                                                              ctypes.byref(tw)))
+            self.info.append(dict(m=mm.value, T=T.value, lds_bytes=lds.value, tile=tw.value))
 
     def add_chunk(self, job, low, dbuf, start=0, count=None):
         count = dbuf.count - start if count is None else count
@@ -421,6 +429,21 @@ class Batch:
             self.close()
 
 
+def autotune(plan, low, dbuf, count=None, device=0):
+    """(best_m, best_tile, [(m, tile, ns_per_prime), ...]) measured on dbuf."""
+    count = dbuf.count if count is None else count
+    bm, bt, n = i32(), i32(), i32()
+    cap = 256
+    cm = np.zeros(cap, np.int32)
+    ct = np.zeros(cap, np.int32)
+    ns = np.zeros(cap, np.float64)
+    _chk("rs_autotune", lib().rs_autotune(device, plan.h, low, dbuf.ptr, count, ctypes.byref(bm),
+                                          ctypes.byref(bt), _p(cm, i32), _p(ct, i32),
+                                          _p(ns, ctypes.c_double), cap, ctypes.byref(n)))
+    k = min(n.value, cap)
+    return bm.value, bt.value, list(zip(cm[:k].tolist(), ct[:k].tolist(), ns[:k].tolist()))
+
+
 def dev_montmul(a, b, p, device=0):
     a, b, p = (np.ascontiguousarray(x, np.uint64) for x in (a, b, p))
     out = np.zeros_like(a)
@@ -450,4 +473,10 @@ def dev_jacobi(a, p, device=0):
 def dev_mad_rate(blocks=8192, iters=4096, device=0):
     r = ctypes.c_double()
     _chk("rs_dev_mad_rate", lib().rs_dev_mad_rate(device, blocks, iters, ctypes.byref(r)))
+    return r.value
+
+
+def dev_copy_rate(nbytes=1 << 30, device=0):
+    r = ctypes.c_double()
+    _chk("rs_dev_copy_rate", lib().rs_dev_copy_rate(device, nbytes, ctypes.byref(r)))
     return r.value

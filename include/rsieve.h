@@ -135,18 +135,19 @@ int rs_format_factor(const rs_factor* f, uint32_t b, char* buf, int cap);
 
 /* ------------------------------------------------------------ device sieve */
 /* One sieve job on one GPU: base, Q, table size m (multiple of 64, or 0 for
- * the cost model's pick), rows and n window from a plan.  Replaces the
- * set-up at :1367-1512.  max_batch bounds the primes copied from host
- * memory per launch (the reference's arraySize, :851); hit_capacity bounds
- * the device append buffer (the reference keeps one slot per prime,
- * :615-621). */
+ * the cost model's pick), rows and n window from a plan.  tile (1..4, 0 for
+ * 2) is the number of giant steps of one row a lane takes per item.
+ * Replaces the set-up at :1367-1512.  max_batch bounds the primes copied
+ * from host memory per launch (the reference's arraySize, :851);
+ * hit_capacity bounds the device append buffer (the reference keeps one
+ * slot per prime, :615-621). */
 typedef struct rs_sieve rs_sieve;
 
-rs_status rs_sieve_create(int device, const rs_plan* pl, int m,
+rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int tile,
                           int64_t max_batch, int64_t hit_capacity,
                           rs_sieve** out);
 rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
-                        int* lds_bytes_per_wave);
+                        int* lds_bytes_per_wave, int* tile);
 /* Primes p = low + offsets[i] from HOST memory (p odd, 3 <= p < 2^63, p ∤ b).
  * Copied through pinned double buffers on a copy stream, then sieved on the
  * compute stream; returns once the copy of this call's data is issued.
@@ -171,11 +172,24 @@ rs_status rs_sieve_sync(rs_sieve* s);
 rs_status rs_sieve_timing(rs_sieve* s, double* kernel_ms, int64_t* primes);
 void rs_sieve_destroy(rs_sieve* s);
 
+/* Autotuner (no reference counterpart: the reference takes m from -m).
+ * Times candidate (m, tile) pairs for a plan on `count` primes
+ * p = low + offsets_dev[i] (device memory) and returns the fastest.
+ * Candidates: multiples of 64 from 1/2 to 2 times the cost model's m that
+ * fit one wavefront's LDS limit, each with tiles 1, 2 and 4.  cand_m,
+ * cand_tile, ns_per_prime (optional, capacity cap) receive the measured
+ * candidates, *ncand their number. */
+rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
+                      const uint32_t* offsets_dev, int64_t count, int* best_m,
+                      int* best_tile, int* cand_m, int* cand_tile,
+                      double* ns_per_prime, int cap, int* ncand);
+
 /* ------------------------------------------------------------ batched jobs */
 /* Many sieve jobs (different bases, k counts, n windows and table sizes)
  * in one persistent launch on one GPU.  No reference counterpart: the
  * reference runs one job per process.  ms[i] = 0 takes the cost model's m
- * for plans[i]; the launch shares one LDS layout sized for the largest job. */
+ * for plans[i] and tiles[i] = 0 tile 2 (tiles may be NULL); the launch
+ * shares one LDS layout sized for the largest job. */
 typedef struct rs_batch rs_batch;
 
 /* One hit of a batched launch: p | k*b^n-1 for job `job`. */
@@ -188,9 +202,10 @@ typedef struct rs_job_factor {
 } rs_job_factor;
 
 rs_status rs_batch_create(int device, int njobs, const rs_plan* const* plans,
-                          const int* ms, int64_t hit_capacity, rs_batch** out);
+                          const int* ms, const int* tiles, int64_t hit_capacity,
+                          rs_batch** out);
 rs_status rs_batch_job_info(const rs_batch* b, int job, int* m, int* T,
-                            int* lds_bytes);
+                            int* lds_bytes, int* tile);
 /* Queues primes p = low + offsets_dev[i] (device memory) for job `job`. */
 rs_status rs_batch_add_chunk(rs_batch* b, int job, uint64_t low,
                              const uint32_t* offsets_dev, int64_t count);
@@ -229,6 +244,10 @@ rs_status rs_dev_free(int device, void* ptr);
  * device, in lane-operations per second, from a dependent-chain kernel of
  * `iters` mads per lane over `blocks` x 256 lanes. */
 rs_status rs_dev_mad_rate(int device, int blocks, int iters, double* ops_per_s);
+
+/* HBM bandwidth of a streaming copy of `bytes` (read + write counted), in
+ * bytes per second; also the known-size reference for traffic counters. */
+rs_status rs_dev_copy_rate(int device, int64_t bytes, double* bytes_per_s);
 
 /* Number of visible HIP devices (0 on a machine without a GPU). */
 int rs_device_count(void);
