@@ -91,8 +91,8 @@ def main():
     m, tile = args.m, args.tile
     if not args.no_tune and m == 0:
         ts = time.time()
-        m, tile, cands = rs.autotune(plan, p0, bufs[0], count=min(counts[0], 1 << 20))
-        tune = {"sample_primes": min(counts[0], 1 << 20), "best_m": m, "best_tile": tile,
+        m, tile, cands = rs.autotune(plan, p0, bufs[0], count=min(counts[0], 1 << 22))
+        tune = {"sample_primes": min(counts[0], 1 << 22), "best_m": m, "best_tile": tile,
                 "candidates": [[cm, ct, round(ns, 4)] for cm, ct, ns in cands]}
         log(f"autotune: m={m} tile={tile} ({len(cands)} candidates, {time.time() - ts:.1f}s)")
     sv = rs.Sieve(plan, m=m, tile=tile, max_batch=args.max_batch, hit_capacity=1 << 22)

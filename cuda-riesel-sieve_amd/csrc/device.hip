@@ -61,7 +61,7 @@ uint32_t magic_div(int d) {
 }
 
 // Fills the job constants from a plan, table size m (0: cost model) and
-// giant-step tile tw (0: 2).
+// giant-step tile tw (1, 2 or 4 items per lane; 0: 2).
 rs_status make_job(const rs_plan* pl, int m, int tw, DevJob* dj) {
     if (!pl) return RS_ERR_ARG;
     if (m <= 0) m = rs_plan_auto_m(pl);
@@ -97,9 +97,10 @@ rs_status make_job(const rs_plan* pl, int m, int tw, DevJob* dj) {
     J.logNB = std::max(logNB, 2);
     if (J.Q * J.T >= 65536 || J.T > 65535) return RS_ERR_ARG;
     J.invT = magic_div(J.T);
-    J.tw = std::min(tw, J.T);
-    J.ntiles = (J.T + J.tw - 1) / J.tw;
-    J.invNT = magic_div(J.ntiles);
+    if (tw == 3) tw = 2;
+    J.tw = tw;
+    // flattened giant-step indices (and their division by T) stay below 2^16
+    if ((long long)nrows * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;
     J.W = rec_words(J.Q, J.T, J.nk);
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + J.T;

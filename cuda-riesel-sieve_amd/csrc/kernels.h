@@ -26,9 +26,7 @@ struct JobDev {
     int nk;       // distinct k
     int logNB;    // baby table: NB = 2^logNB buckets of 8 16-bit fingerprints
     uint32_t invT;  // ceil(2^32 / T) for T >= 2, else 0: idx / T for idx < 2^16
-    int tw;         // giant-step tile: t-steps per lane item, 1..4
-    int ntiles;     // ceil(T / tw) tiles per row
-    uint32_t invNT;  // ceil(2^32 / ntiles) for ntiles >= 2, else 0
+    int tw;         // giant-step tile: consecutive (row, t) items per lane, 1, 2 or 4
     int W;        // record words
     int off_gt;   // REC_BH + Q
     int off_j;    // off_gt + T
@@ -62,7 +60,7 @@ static inline int rec_words(int Q, int T, int nk) {
 }
 
 // LDS layout of one wavefront, in 32-bit words from its base:
-//   [0, 4 NB)                 table: NB buckets x 8 uint16 fingerprints
+//   [0, 4 NB)                 table: NB buckets x 8 uint16 fingerprints (0 = empty)
 //   [4 NB, 4 NB + NB/2)       bucket fill counts, two uint16 per word
 //   then 2 Q T                Y products (uint64)
 //   then 2 nrows              active rows (k, r T << 8 | r)

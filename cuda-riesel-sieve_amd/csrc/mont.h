@@ -35,6 +35,13 @@ __device__ __forceinline__ u64 mont_mprime(u64 p) {
     return 0ull - y;
 }
 
+// r - p if r >= p, else r, for r < 2p and p < 2^63: r - p wraps to at
+// least 2^64 - p > 2^63 exactly when r < p, so the sign bit decides.
+__device__ __forceinline__ u64 csub(u64 r, u64 p) {
+    const u64 d = r - p;
+    return (long long)d < 0 ? r : d;
+}
+
 // Montgomery product a*b*2^-64 mod p.  Needs b < p and a < 2^64 (a may be
 // an unreduced integer such as k or b): then a*b + m*p < 2^128 and the
 // result before the final subtraction is < 2p.
@@ -58,8 +65,7 @@ __device__ __forceinline__ u64 montmul(u64 a, u64 b, u64 p, u64 mp) {
     const u64 s1 = mad32(m0, p1, hi32(s0));
     const u64 s2 = mad32(m1, p0, lo32(s1)) + l1;    // low word is 0
     const u64 s3 = mad32(m1, p1, hi32(s1)) + hi32(s2);
-    u64 r = t3 + s3;
-    return r >= p ? r - p : r;
+    return csub(t3 + s3, p);
 }
 
 // k*y*2^-32 mod p for 32-bit k and y < p: one 32-bit Montgomery step.
@@ -70,8 +76,7 @@ __device__ __forceinline__ u64 halfmul(u32 k, u64 y, u64 p, u32 mp0) {
     const u32 m = lo32(a0) * mp0;
     const u64 s0 = mad32(m, lo32(p), lo32(a0));     // low word is 0
     const u64 s1 = mad32(m, hi32(p), hi32(s0));
-    u64 r = a1 + s1;
-    return r >= p ? r - p : r;
+    return csub(a1 + s1, p);
 }
 
 // x^e in Montgomery form (x and one = R mod p in Montgomery form).

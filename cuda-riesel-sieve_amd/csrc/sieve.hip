@@ -59,11 +59,128 @@ __device__ __forceinline__ int div_T(u32 idx, u32 invT) {
     return invT ? (int)__umulhi(idx, invT) : (int)idx;
 }
 
-// 16-bit table fingerprint of a residue: the bits above the bucket index;
-// 0xffff marks an empty slot, so that value is folded onto 0xfffe.
+// 16-bit table fingerprint of a residue: 15 bits above the bucket index
+// with the top bit set, so it is never 0, the empty-slot value.
 __device__ __forceinline__ u32 fingerprint(u64 v, int logNB) {
-    const u32 raw = lo32(v >> logNB) & 0xffffu;
-    return raw - (raw == 0xffffu);
+    return (lo32(v >> logNB) & 0x7fffu) | 0x8000u;
+}
+
+// Nonzero iff some 16-bit half of x is zero (exact as a boolean: the
+// lowest zero half takes no borrow from below).
+__device__ __forceinline__ u32 has_zero_half(u32 x) {
+    return (x - 0x10001u) & ~x & 0x80008000u;
+}
+
+// Emits every baby step j with g^j == target (all 64 lanes take part:
+// lane l checks j = l + 64 v) as a hit of giant step tt of row (k, r).
+__device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32 k, u32 r,
+                                       int tt, u64 vstart, u64 g64, u64 p, u64 mp,
+                                       HitDev* __restrict__ hits, u32* __restrict__ hit_count,
+                                       u32 hit_cap) {
+    const int lane = threadIdx.x & 63;
+    u64 w = vstart;
+    for (int v = 0; v < (J.m >> 6); ++v) {
+        if (w == target) {
+            const int j = lane + 64 * v;
+            const long long xx = (long long)J.x0 + (long long)tt * J.m + (J.m - 1 - j);
+            const long long n = (long long)J.Q * xx + (int)r;
+            if (n >= J.minN && n <= J.maxN) {
+                const u32 at = atomicAdd(hit_count, 1u);
+                if (at < hit_cap) {
+                    HitDev h;
+                    h.p = p;
+                    h.k = k;
+                    h.n = (u32)n;
+                    h.job = (u32)job;
+                    h.pad = 0;
+                    hits[at] = h;
+                }
+            }
+        }
+        w = montmul(w, g64, p, mp);
+    }
+}
+
+// Giant steps of one prime: the A*T grid of (active row a, step t), t
+// fastest, is cut into runs of TW consecutive items per lane and 64*TW per
+// round.  Each item is one 32x64 Montgomery product s = k*Y[r][t] and one
+// bucket read; all TW of a lane are issued before any is tested.  A full
+// bucket (slot 7 taken) continues into the next bucket, and a fingerprint
+// match is resolved by rescan(); both are rare and handled after the fast
+// path.  The loop is wave-uniform because rescan() needs every lane.
+template <int TW>
+__device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A,
+                                            const uint4* __restrict__ tab,
+                                            const u64* __restrict__ Y,
+                                            const uint2* __restrict__ list, u64 vstart, u64 g64,
+                                            u64 p, u64 mp, HitDev* __restrict__ hits,
+                                            u32* __restrict__ hit_count, u32 hit_cap) {
+    const int lane = threadIdx.x & 63;
+    const u32 NB = 1u << J.logNB;
+    const int T = J.T;
+    const int total = A * T;
+    const u32 mp0 = lo32(mp);
+    for (int base = 0; base < total; base += 64 * TW) {
+        const int i0 = base + TW * lane;
+        int a = div_T((u32)i0, J.invT), t = i0 - a * T;
+        u64 s[TW];
+        uint4 e[TW];
+        u32 fp2[TW], bk[TW], ku[TW], ru[TW];
+        int tu[TW];
+        bool ok[TW];
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            ok[u] = i0 + u < total;
+            const int ac = min(a, A - 1);
+            const uint2 row = list[ac];
+            ku[u] = row.x;
+            ru[u] = row.y & 0xffu;
+            tu[u] = t;
+            s[u] = halfmul(row.x, Y[(row.y >> 8) + min(t, T - 1)], p, mp0);
+            bk[u] = lo32(s[u]) & (NB - 1);
+            fp2[u] = fingerprint(s[u], J.logNB) * 0x10001u;
+            if (++t == T) { t = 0; ++a; }
+        }
+#pragma unroll
+        for (int u = 0; u < TW; ++u) e[u] = tab[bk[u]];
+        u32 mbits = 0, fbits = 0;
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            const u32 z = has_zero_half(e[u].x ^ fp2[u]) | has_zero_half(e[u].y ^ fp2[u]) |
+                          has_zero_half(e[u].z ^ fp2[u]) | has_zero_half(e[u].w ^ fp2[u]);
+            mbits |= (u32)(ok[u] && z != 0) << u;
+            fbits |= (u32)(ok[u] && e[u].w >= 0x10000u) << u;  // slot 7 taken
+        }
+        // full buckets: follow the chain to the first bucket with slot 7 free
+        if (__ballot(fbits != 0)) {
+#pragma unroll
+            for (int u = 0; u < TW; ++u) {
+                if ((fbits >> u) & 1) {
+                    u32 b = bk[u];
+                    while (true) {
+                        b = (b + 1) & (NB - 1);
+                        const uint4 f = tab[b];
+                        const u32 z = has_zero_half(f.x ^ fp2[u]) | has_zero_half(f.y ^ fp2[u]) |
+                                      has_zero_half(f.z ^ fp2[u]) | has_zero_half(f.w ^ fp2[u]);
+                        if (z) mbits |= 1u << u;
+                        if (f.w < 0x10000u) break;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            u64 mb = __ballot((mbits >> u) & 1);
+            while (mb) {
+                const int src = __builtin_ctzll(mb);
+                mb &= mb - 1;
+                rescan(J, job, readlane64(s[u], src), __builtin_amdgcn_readlane(ku[u], src),
+                       __builtin_amdgcn_readlane(ru[u], src),
+                       __builtin_amdgcn_readlane(tu[u], src), vstart, g64, p, mp, hits,
+                       hit_count, hit_cap);
+            }
+        }
+    }
 }
 
 // Set-up of one prime (one lane): writes the record described in kernels.h.
@@ -145,10 +262,9 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     for (int w = 64 + lane; w < W; w += 64) R[w] = Rg[w];
     const u64 p = readlane64(pre, REC_P), mp = readlane64(pre, REC_MP);
     const u64 g64 = readlane64(pre, REC_G64);
-    const u32 mp0 = lo32(mp);
 
     // clear table and counts
-    for (u32 b = lane; b < NB; b += 64) tab[b] = make_uint4(~0u, ~0u, ~0u, ~0u);
+    for (u32 b = lane; b < NB; b += 64) tab[b] = make_uint4(0, 0, 0, 0);
     for (u32 c = lane; c < NB / 2; c += 64) cnt[c] = 0;
     wave_sync();
 
@@ -199,79 +315,12 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     }
     wave_sync();
 
-    // giant steps: item (a, c) of the A x ntiles grid covers t-steps
-    // t = c*tw .. c*tw + tw - 1 of row a; lanes stride by 64 items.  A probe
-    // reads one bucket (8 fingerprints) and follows full buckets.  The loop
-    // is wave-uniform: the rescan below needs every lane.
-    const int TW = J.tw, NT = J.ntiles;
-    const int total = A * NT;
-    const int dAi = 64 / NT, dCi = 64 - dAi * NT;
-    int a = div_T((u32)lane, J.invNT), c = lane - a * NT;
-    for (int it0 = 0; it0 < total; it0 += 64) {
-        const bool act = it0 + lane < total;
-        const uint2 row = act ? list[a] : make_uint2(0, 0);
-        const u32 rT = row.y >> 8;
-        u64 sv[4] = {0, 0, 0, 0};
-        u32 mbits = 0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int t = c * TW + u;
-            if (u < TW && act && t < T) {
-                const u64 s = halfmul(row.x, Y[rT + t], p, mp0);
-                sv[u] = s;
-                u32 b = lo32(s) & (NB - 1);
-                const u32 fp2 = fingerprint(s, J.logNB) * 0x10001u;
-                while (true) {
-                    const uint4 e = tab[b];
-                    const u32 x0 = e.x ^ fp2, x1 = e.y ^ fp2, x2 = e.z ^ fp2, x3 = e.w ^ fp2;
-                    // a 16-bit half of x is zero iff that slot holds the fingerprint
-                    const u32 z = ((x0 - 0x10001u) & ~x0) | ((x1 - 0x10001u) & ~x1) |
-                                  ((x2 - 0x10001u) & ~x2) | ((x3 - 0x10001u) & ~x3);
-                    if (z & 0x80008000u) mbits |= 1u << u;
-                    if ((e.w >> 16) == 0xffffu) break;  // slot 7 free: chain ends here
-                    b = (b + 1) & (NB - 1);
-                }
-            }
-        }
-        // fingerprint match: all lanes rescan the baby steps (lane l checks
-        // j = l + 64 v) for exact equalities with the matching lane's s
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            u64 mb = __ballot((mbits >> u) & 1);
-            while (mb) {
-                const int src = __builtin_ctzll(mb);
-                mb &= mb - 1;
-                const u64 target = readlane64(sv[u], src);
-                const u32 kk = __builtin_amdgcn_readlane(row.x, src);
-                const u32 ry = __builtin_amdgcn_readlane(row.y, src);
-                const int tt = __builtin_amdgcn_readlane(c, src) * TW + u;
-                u64 w = vstart;
-                for (int v2 = 0; v2 < L; ++v2) {
-                    if (w == target) {
-                        const int j = lane + 64 * v2;
-                        const long long xx = (long long)J.x0 + (long long)tt * m + (m - 1 - j);
-                        const long long n = (long long)Q * xx + (int)(ry & 0xffu);
-                        if (n >= J.minN && n <= J.maxN) {
-                            const u32 at = atomicAdd(hit_count, 1u);
-                            if (at < hit_cap) {
-                                HitDev h;
-                                h.p = p;
-                                h.k = kk;
-                                h.n = (u32)n;
-                                h.job = (u32)job;
-                                h.pad = 0;
-                                hits[at] = h;
-                            }
-                        }
-                    }
-                    w = montmul(w, g64, p, mp);
-                }
-            }
-        }
-        c += dCi;
-        a += dAi;
-        if (c >= NT) { c -= NT; a += 1; }
-    }
+    if (J.tw >= 4)
+        giant_steps<4>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
+    else if (J.tw == 2)
+        giant_steps<2>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
+    else
+        giant_steps<1>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
     wave_sync();  // the next prime reuses the table
 }
 
