@@ -128,8 +128,17 @@ def main():
     # streaming copy of a known size: HBM reference for the traffic counters
     copy_bytes = 1 << 30
     hbm = rs.dev_copy_rate(copy_bytes)
+    # HBM bytes per prime from the committed counter summary (FETCH_SIZE and
+    # WRITE_SIZE scaled by the copy-probe calibration), times primes per launch
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "rs_sieve_kernel_r745_pmc.txt")
+    if os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_prime:"):
+                traffic = float(line.split(":")[1]) * per_launch_primes
     roof = {"bound": "valu", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "Gmul/s",
-            "frac": achieved / peak, "traffic": None,
+            "frac": achieved / peak, "traffic": traffic,
+            "traffic_source": "profiles/rs_sieve_kernel_r745_pmc.txt (m=576, tile=2)",
             "kernel": "rs_sieve_kernel", "muls_per_prime": muls_per_prime,
             "primes_per_launch": per_launch_primes,
             "mean_launch_ms": bsgs_s_per_launch * 1e3,
