@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../host/host.h"
+#include "gpuprimes.h"
 #include "kernels.h"
 
 extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
@@ -31,6 +32,7 @@ extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev*
                                            uint64_t* scratch, unsigned long long* work,
                                            HitDev* hits, uint32_t* hit_count, uint32_t hit_cap);
 extern "C" __global__ void rs_zero_u32(uint32_t*, int);
+extern "C" __global__ void rs_accum_count(const uint32_t*, unsigned long long*);
 
 #define HIPCHK(x)                                 \
     do {                                          \
@@ -250,6 +252,11 @@ struct rs_sieve {
     long long acc_primes = 0;
     std::vector<HitDev> held;
     int64_t lost = 0;
+    // device-generated chunks (rs_sieve_submit_range)
+    rs_gpu_primegen* gpg = nullptr;
+    uint32_t* d_range = nullptr;
+    int64_t range_cap = 0;
+    unsigned long long* d_range_primes = nullptr;  // primes of those chunks so far
 };
 
 static void free_sieve(rs_sieve* s) {
@@ -259,6 +266,9 @@ static void free_sieve(rs_sieve* s) {
     if (s->copy_stream) (void)hipStreamSynchronize(s->copy_stream);
     free_job(s->job);
     free_launcher(s->L);
+    if (s->gpg) rs_gpu_primegen_free(s->gpg);
+    (void)hipFree(s->d_range);
+    (void)hipFree(s->d_range_primes);
     for (int b = 0; b < 2; ++b) {
         if (s->h_pin[b]) (void)hipHostFree(s->h_pin[b]);
         (void)hipFree(s->d_offs[b]);
@@ -374,10 +384,72 @@ extern "C" rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* 
     return RS_OK;
 }
 
+extern "C" rs_status rs_sieve_submit_range(rs_sieve* s, uint64_t lo, uint64_t hi) {
+    if (!s || hi < lo || lo < 3 || hi > (1ull << 63)) return RS_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    for (uint64_t a = lo; a < hi;) {
+        const uint64_t b = std::min<uint64_t>(hi, a + (1ull << 32));
+        if (!s->gpg || rs_gpu_primegen_hi(s->gpg) < b) {
+            // sieving primes for up to 4x this bound, so later chunks reuse them
+            if (s->gpg) {
+                HIPCHK(hipStreamSynchronize(s->stream));
+                rs_gpu_primegen_free(s->gpg);
+                s->gpg = nullptr;
+            }
+            const uint64_t cap_hi = b > (1ull << 61) ? (1ull << 63) : 4 * b;
+            rs_status st = rs_gpu_primegen_create(s->device, cap_hi, &s->gpg);
+            if (st != RS_OK) return st;
+        }
+        const int64_t need = gpu_primes_capacity(b - a);
+        if (need > s->range_cap) {
+            HIPCHK(hipStreamSynchronize(s->stream));
+            (void)hipFree(s->d_range);
+            s->d_range = nullptr;
+            if (hipMalloc(&s->d_range, (size_t)need * 4) != hipSuccess) return RS_ERR_NOMEM;
+            s->range_cap = need;
+        }
+        if (!s->d_range_primes) {
+            HIPCHK(hipMalloc(&s->d_range_primes, 8));
+            HIPCHK(hipMemset(s->d_range_primes, 0, 8));
+        }
+        // generation and sieve are ordered on the sieve stream: the next
+        // chunk's generation starts after this chunk's sieve has read d_range
+        const uint32_t* cnt_dev = nullptr;
+        rs_status st = gpu_primes_enqueue(s->gpg, a, b, s->d_range, s->range_cap, s->stream,
+                                          &cnt_dev);
+        if (st != RS_OK) return st;
+        Timing t{};
+        HIPCHK(hipEventCreate(&t.a));
+        HIPCHK(hipEventCreate(&t.b));
+        t.primes = 0;  // counted on the device, see d_range_primes
+        HIPCHK(hipEventRecord(t.a, s->stream));
+        ChunkDev c0{};
+        c0.low = a;
+        c0.offs = s->d_range;
+        c0.count = s->range_cap;
+        c0.count_dev = cnt_dev;
+        st = launch(s->L, s->stream, s->job.J, c0, nullptr, nullptr, 1, (s->range_cap + 63) / 64);
+        if (st != RS_OK) return st;
+        HIPCHK(hipEventRecord(t.b, s->stream));
+        hipLaunchKernelGGL(rs_accum_count, dim3(1), dim3(64), 0, s->stream, cnt_dev,
+                           s->d_range_primes);
+        HIPCHK(hipGetLastError());
+        s->timings.push_back(t);
+        a = b;
+    }
+    return RS_OK;
+}
+
 static rs_status sieve_drain(rs_sieve* s) {
     HIPCHK(hipSetDevice(s->device));
     rs_status st = drain_hits(s->L, s->stream, &s->held, &s->lost);
     if (st != RS_OK) return st;
+    if (s->d_range_primes) {
+        unsigned long long n = 0;
+        HIPCHK(hipMemcpy(&n, s->d_range_primes, 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemset(s->d_range_primes, 0, 8));
+        s->acc_primes += (long long)n;
+    }
     for (auto& t : s->timings) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));

@@ -127,6 +127,13 @@ extern "C" rs_status rs_dev_upload(int device, void* dst, const void* src, int64
     return RS_OK;
 }
 
+extern "C" rs_status rs_dev_download(int device, void* dst, const void* src, int64_t bytes) {
+    if (!dst || !src || bytes < 0) return RS_ERR_ARG;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return RS_OK;
+}
+
 extern "C" rs_status rs_dev_free(int device, void* ptr) {
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipFree(ptr));

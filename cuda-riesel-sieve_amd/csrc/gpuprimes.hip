@@ -1,0 +1,311 @@
+// gpuprimes.hip -- prime offsets generated on the GPU: the step before the
+// sieve, replacing the CPU generator of the reference (generateGPUPrimes,
+// kernel_v0.24.6.cu:1838-1987) when the chunk should never leave HBM.
+//
+// Every prime p in [lo, hi) (hi - lo <= 2^32, 3 <= lo, hi <= 2^63) is
+// written as the uint32 offset p - lo, ascending.  The odd numbers of the
+// range are split into segments of SEG_BITS; one workgroup sieves a segment
+// in an LDS bitmap with the odd primes q <= S (S = min(sqrt(hi), 2^22)),
+// primes below SMALL_Q marked by the whole block and larger ones by one
+// thread each; survivors at or above S^2 are proved by deterministic
+// Miller-Rabin (bases 2, 325, 9375, 28178, 450775, 9780504, 1795265022,
+// exact for 64-bit n).  A scan of the segment counts then places each
+// segment's offsets.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../host/host.h"
+#include "gpuprimes.h"
+#include "mont.h"
+
+#define SEG_BITS (1u << 18)   // odd numbers per segment: 32 KiB of LDS bitmap
+#define SEG_WORDS (SEG_BITS / 32)
+#define SMALL_Q 4096u         // primes below this are marked block-wide
+
+__device__ __forceinline__ bool mr_prime(u64 n) {
+    // n odd, n > 2^20 here (only survivors above S^2 are tested)
+    const u64 mp = mont_mprime(n);
+    const u64 one = (0ull - n) % n;
+    const u64 mone = n - one;                      // Montgomery form of n - 1
+    u64 r2 = one << 1;
+    if (r2 >= n) r2 -= n;
+    for (int s = 0; s < 6; ++s) r2 = montmul(r2, r2, n, mp);
+    u64 d = n - 1;
+    int s = 0;
+    while ((d & 1) == 0) {
+        d >>= 1;
+        ++s;
+    }
+    const u64 bases[7] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+    for (int k = 0; k < 7; ++k) {
+        const u64 a = bases[k] % n;
+        if (a == 0) continue;
+        u64 x = one, b = montmul(a, r2, n, mp);
+        for (u64 e = d; e; e >>= 1) {
+            if (e & 1) x = montmul(x, b, n, mp);
+            b = montmul(b, b, n, mp);
+        }
+        if (x == one || x == mone) continue;
+        bool comp = true;
+        for (int i = 1; i < s && comp; ++i) {
+            x = montmul(x, x, n, mp);
+            if (x == mone) comp = false;
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+
+// Sieves segment blockIdx.x of the odd numbers base + 2i, i < nodd; writes
+// the bitmap (bit set = composite or excluded) and the prime count.
+extern "C" __global__ void __launch_bounds__(256)
+rs_prime_sieve_kernel(uint64_t base, long long nodd, const uint32_t* __restrict__ sp, int np,
+                      int nsmall, uint64_t s2, uint32_t* __restrict__ bitmap,
+                      uint32_t* __restrict__ counts) {
+    __shared__ uint32_t bm[SEG_WORDS];
+    __shared__ uint32_t red[256];
+    const int tid = threadIdx.x;
+    const long long i_lo = (long long)blockIdx.x * SEG_BITS;
+    const u32 nbits = (u32)min((long long)SEG_BITS, nodd - i_lo);
+    const u64 v_lo = base + 2ull * (u64)i_lo;
+    const u64 v_hi = v_lo + 2ull * (nbits - 1);
+    for (u32 w = tid; w < SEG_WORDS; w += 256) bm[w] = 0;
+    __syncthreads();
+    // first index i with q | v_lo + 2i and v_lo + 2i >= q^2 (q odd)
+    auto first = [&](u32 q) -> u64 {
+        const u64 qq = (u64)q * q;
+        if (qq >= v_lo) return (qq - v_lo) >> 1;
+        const u32 r = (u32)(v_lo % q);
+        const u32 d = r == 0 ? 0u : q - r;
+        return (d & 1) ? (u64)((d + q) >> 1) : (u64)(d >> 1);
+    };
+    // small primes: the whole block marks each one
+    for (int k = 0; k < nsmall; ++k) {
+        const u32 q = sp[k];
+        if ((u64)q * q > v_hi) break;
+        const u64 i0 = first(q);
+        for (u64 i = i0 + (u64)tid * q; i < nbits; i += 256ull * q)
+            atomicOr(&bm[i >> 5], 1u << (i & 31));
+    }
+    // large primes: one thread each (at most SEG_BITS / SMALL_Q marks)
+    for (int k = nsmall + tid; k < np; k += 256) {
+        const u32 q = sp[k];
+        if ((u64)q * q > v_hi) break;
+        for (u64 i = first(q); i < nbits; i += q) atomicOr(&bm[i >> 5], 1u << (i & 31));
+    }
+    __syncthreads();
+    // exclusions: 1, the tail past nbits, survivors >= S^2 that fail MR
+    u32 cnt = 0;
+    for (u32 w = tid; w < SEG_WORDS; w += 256) {
+        u32 x = bm[w];
+        const u32 b0 = w * 32;
+        if (b0 >= nbits) x = ~0u;
+        else if (nbits - b0 < 32) x |= ~0u << (nbits - b0);
+        if (b0 == 0 && v_lo == 1) x |= 1u;
+        u32 free_bits = ~x;
+        if (free_bits && v_lo + 2ull * (b0 + 31) >= s2) {
+            while (free_bits) {
+                const int b = __builtin_ctz(free_bits);
+                free_bits &= free_bits - 1;
+                const u64 v = v_lo + 2ull * (b0 + b);
+                if (v >= s2 && !mr_prime(v)) x |= 1u << b;
+            }
+        }
+        bitmap[(i_lo >> 5) + w] = x;
+        cnt += __popc(~x);
+    }
+    red[tid] = cnt;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) counts[blockIdx.x] = red[0];
+}
+
+// Exclusive scan of the segment counts in place (one block); total in
+// counts[nseg].
+extern "C" __global__ void __launch_bounds__(1024)
+rs_prime_scan_kernel(uint32_t* __restrict__ counts, int nseg) {
+    __shared__ uint32_t part[1024];
+    const int tid = threadIdx.x;
+    const int per = (nseg + 1023) / 1024;
+    const int a = tid * per, b = min(nseg, a + per);
+    u32 s = 0;
+    for (int i = a; i < b; ++i) s += counts[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const u32 v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    u32 run = part[tid] - s;  // exclusive prefix of this thread's slice
+    for (int i = a; i < b; ++i) {
+        const u32 c = counts[i];
+        counts[i] = run;
+        run += c;
+    }
+    if (tid == 1023) counts[nseg] = part[1023];
+}
+
+// Writes the offsets of segment blockIdx.x starting at counts[seg].
+extern "C" __global__ void __launch_bounds__(256)
+rs_prime_emit_kernel(uint64_t base, uint64_t lo, const uint32_t* __restrict__ bitmap,
+                     const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
+                     long long cap) {
+    __shared__ uint32_t part[256];
+    const int tid = threadIdx.x;
+    const long long w0 = (long long)blockIdx.x * SEG_WORDS;
+    // each thread owns SEG_WORDS / 256 consecutive words
+    const int per = SEG_WORDS / 256;
+    u32 c = 0;
+    for (int k = 0; k < per; ++k) c += __popc(~bitmap[w0 + tid * per + k]);
+    part[tid] = c;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const u32 v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    long long at = (long long)counts[blockIdx.x] + part[tid] - c;
+    const u64 d0 = base - lo;
+    for (int k = 0; k < per; ++k) {
+        const long long w = w0 + tid * per + k;
+        u32 f = ~bitmap[w];
+        while (f) {
+            const int b = __builtin_ctz(f);
+            f &= f - 1;
+            if (at < cap) out[at] = (uint32_t)(d0 + 2ull * (u64)(w * 32 + b));
+            ++at;
+        }
+    }
+}
+
+#define HIPCHK(x)                                 \
+    do {                                          \
+        if ((x) != hipSuccess) return RS_ERR_HIP; \
+    } while (0)
+
+extern "C" __global__ void rs_zero_u32(uint32_t*, int);
+
+struct rs_gpu_primegen {
+    int device = 0;  // buffers below are reused by every call
+    uint64_t hi_max = 0;
+    uint64_t s2 = 0;           // survivors >= s2 need Miller-Rabin
+    uint32_t* d_sp = nullptr;  // odd sieving primes
+    int np = 0, nsmall = 0;
+    uint32_t* d_bitmap = nullptr;
+    size_t bitmap_words = 0;
+    uint32_t* d_counts = nullptr;
+    int counts_cap = 0;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" rs_status rs_gpu_primegen_create(int device, uint64_t hi_max, rs_gpu_primegen** out) {
+    if (!out || hi_max < 3 || hi_max > (1ull << 63)) return RS_ERR_ARG;
+    *out = nullptr;
+    HIPCHK(hipSetDevice(device));
+    rs_gpu_primegen* g = new rs_gpu_primegen();
+    g->device = device;
+    g->hi_max = hi_max;
+    uint64_t S = (uint64_t)std::sqrt((double)hi_max) + 2;
+    S = std::min<uint64_t>(S, 1ull << 22);
+    std::vector<uint32_t> ps = rsh::primes_upto((uint32_t)S);
+    std::vector<uint32_t> odd(ps.begin() + 1, ps.end());
+    g->np = (int)odd.size();
+    g->nsmall = (int)(std::lower_bound(odd.begin(), odd.end(), SMALL_Q) - odd.begin());
+    const uint64_t last = odd.empty() ? 3 : odd.back();
+    g->s2 = last * last;
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&g->d_sp, std::max<size_t>(4, odd.size() * 4)) != hipSuccess ||
+        (odd.size() && hipMemcpy(g->d_sp, odd.data(), odd.size() * 4, hipMemcpyHostToDevice) !=
+                           hipSuccess)) {
+        rs_gpu_primegen_free(g);
+        return RS_ERR_HIP;
+    }
+    *out = g;
+    return RS_OK;
+}
+
+rs_status gpu_primes_enqueue(rs_gpu_primegen* g, uint64_t lo, uint64_t hi, uint32_t* out,
+                             int64_t cap, hipStream_t stream, const uint32_t** count_dev) {
+    if (!g || hi < lo || hi - lo > (1ull << 32) || lo < 3 || hi > g->hi_max ||
+        (cap > 0 && !out) || !count_dev)
+        return RS_ERR_ARG;
+    const uint64_t base = lo | 1;
+    const long long nodd = base >= hi ? 0 : (long long)((hi - base + 1) / 2);
+    const int nseg = (int)std::max<long long>(1, (nodd + SEG_BITS - 1) / SEG_BITS);
+    const size_t words = (size_t)nseg * SEG_WORDS;
+    // buffers are reused: earlier work on any stream must not still read them
+    if (words > g->bitmap_words || nseg + 1 > g->counts_cap) HIPCHK(hipDeviceSynchronize());
+    if (words > g->bitmap_words) {
+        (void)hipFree(g->d_bitmap);
+        g->d_bitmap = nullptr;
+        if (hipMalloc(&g->d_bitmap, words * 4) != hipSuccess) return RS_ERR_NOMEM;
+        g->bitmap_words = words;
+    }
+    if (nseg + 1 > g->counts_cap) {
+        (void)hipFree(g->d_counts);
+        g->d_counts = nullptr;
+        if (hipMalloc(&g->d_counts, (size_t)(nseg + 1) * 4) != hipSuccess) return RS_ERR_NOMEM;
+        g->counts_cap = nseg + 1;
+    }
+    if (nodd == 0) {
+        hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, g->d_counts + nseg, 1);
+        HIPCHK(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(rs_prime_sieve_kernel, dim3(nseg), dim3(256), 0, stream, base, nodd,
+                           g->d_sp, g->np, g->nsmall, g->s2, g->d_bitmap, g->d_counts);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(rs_prime_scan_kernel, dim3(1), dim3(1024), 0, stream, g->d_counts,
+                           nseg);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(rs_prime_emit_kernel, dim3(nseg), dim3(256), 0, stream, base, lo,
+                           g->d_bitmap, g->d_counts, out, (long long)cap);
+        HIPCHK(hipGetLastError());
+    }
+    *count_dev = g->d_counts + nseg;
+    return RS_OK;
+}
+
+uint64_t rs_gpu_primegen_hi(const rs_gpu_primegen* g) { return g->hi_max; }
+
+int64_t gpu_primes_capacity(uint64_t width) {
+    // pi(x + y) - pi(x) <= 2 y / ln y (Montgomery-Vaughan), plus slack
+    if (width < 64) return 64;
+    return (int64_t)(2.0 * (double)width / std::log((double)width)) + 64;
+}
+
+// Sieves [lo, hi) into offsets_dev (device memory, capacity cap) and returns
+// the count (RS_ERR_OVERFLOW if it exceeds cap; nothing past cap is written).
+extern "C" rs_status rs_gpu_primes_range(rs_gpu_primegen* g, uint64_t lo, uint64_t hi,
+                                         uint32_t* offsets_dev, int64_t cap, int64_t* count) {
+    if (!g || !count) return RS_ERR_ARG;
+    *count = 0;
+    HIPCHK(hipSetDevice(g->device));
+    const uint32_t* cnt_dev = nullptr;
+    rs_status st = gpu_primes_enqueue(g, lo, hi, offsets_dev, cap, g->stream, &cnt_dev);
+    if (st != RS_OK) return st;
+    uint32_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, cnt_dev, 4, hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    *count = total;
+    return (int64_t)total > cap ? RS_ERR_OVERFLOW : RS_OK;
+}
+
+extern "C" void rs_gpu_primegen_free(rs_gpu_primegen* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    (void)hipFree(g->d_sp);
+    (void)hipFree(g->d_bitmap);
+    (void)hipFree(g->d_counts);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}

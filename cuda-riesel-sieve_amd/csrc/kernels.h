@@ -37,12 +37,15 @@ struct JobDev {
 };
 
 // A chunk of primes p = low + offs[i], i < count, of one job; group0 is
-// its first 64-prime group in the launch's global numbering.
+// its first 64-prime group in the launch's global numbering.  When
+// count_dev is set (a single chunk whose primes were generated on the
+// device), the count is read from there and `count` is only a bound.
 struct ChunkDev {
     uint64_t low;
     const uint32_t* offs;
     long long count;
     long long group0;
+    const uint32_t* count_dev;
     int job;
     int pad;
 };

@@ -356,7 +356,9 @@ rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
         const ChunkDev C = chunks ? chunks[ci] : c0;
         const JobDev J = jobs ? jobs[C.job] : j0;
         const long long q0 = ((long long)grp - C.group0) * 64;
-        const int n = (int)min(64ll, C.count - q0);
+        const long long cnt = C.count_dev ? (long long)*C.count_dev : C.count;
+        const int n = (int)min(64ll, cnt - q0);
+        if (n <= 0) break;  // device-counted chunk: groups past the end are empty
 
         // set-up, one lane per prime
         if (lane < n) prep_record(J, C.low + C.offs[q0 + lane], S + lane * J.W);
@@ -425,6 +427,11 @@ extern "C" __global__ void __launch_bounds__(256) rs_copy_probe(const uint4* __r
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+// acc += *c (prime counts of device-generated chunks, for timing reports).
+extern "C" __global__ void rs_accum_count(const uint32_t* c, unsigned long long* acc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *acc += *c;
 }
 
 extern "C" __global__ void rs_zero_u32(uint32_t* x, int n) {

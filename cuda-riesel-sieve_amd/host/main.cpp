@@ -6,15 +6,19 @@
 // -P P1, -Q Q (even), -O FILE (factor file, default factors.txt), -b/-t
 // SCALE (primes per launch = 128*2^(b-1) * 128*2^(t-1), :817-818, :849-851),
 // -s SCALE (at least 4096*s sieving primes, :1274), -v (no verification:
-// datless output), -h.  Unlike :705, -h and -v take no argument.  With no
+// datless output), -g (generate the primes on the GPU; the default is the
+// CPU generator of the reference's design), -h.  Unlike :705, -h and -v
+// take no argument.  With no
 // -p/-P the default test range of :837-839 is used.  The sieved range is
 // clipped to [P0, P1] (the reference runs past P1 with its last batch).
 // Output lines keep the reference's formats: factors "p | k*b^n-1"
 // (:1833) and a progress line every 20 launches (:1759).
 #include <getopt.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cinttypes>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -37,6 +41,7 @@ static void usage() {
         "-t SCALE : Scale the primes per launch by 2^(SCALE-1) (threads).\n"
         "-s SCALE : Use at least 4096*SCALE small primes for the prime generator. Default 256.\n"
         "-v       : Don't verify factors. This implies running datless.\n"
+        "-g       : Generate the primes on the GPU instead of the CPU generator.\n"
         "-d DEV   : GPU device index (default 0).\n"
         "-h       : Prints this help\n");
 }
@@ -48,12 +53,12 @@ static int fail(const char* what, rs_status st) {
 
 int main(int argc, char* argv[]) {
     int blockScale = 1, threadScale = 1, Qin = 0, hashFlag = 0, verify = 1;
-    int smallPrimeScale = 256, device = 0;
+    int smallPrimeScale = 256, device = 0, gpuPrimes = 0;
     const char* abcdFile = "sr_745.abcd";
     const char* factorFile = "factors.txt";
     unsigned long long low = 0, high = 0;
     int opt;
-    while ((opt = getopt(argc, argv, "b:hi:m:p:P:Q:O:s:t:vd:")) != -1) {
+    while ((opt = getopt(argc, argv, "b:hi:m:p:P:Q:O:s:t:vd:g")) != -1) {
         switch (opt) {
             case 'b': blockScale = (int)strtol(optarg, nullptr, 0); break;
             case 'h': usage(); return 0;
@@ -73,6 +78,7 @@ int main(int argc, char* argv[]) {
             case 't': threadScale = (int)strtol(optarg, nullptr, 0); break;
             case 'v': verify = 0; break;
             case 'd': device = (int)strtol(optarg, nullptr, 0); break;
+            case 'g': gpuPrimes = 1; break;
             default: usage(); return 1;
         }
     }
@@ -153,6 +159,14 @@ int main(int argc, char* argv[]) {
     // one batch: the first arraySize primes >= lo (lo odd), clipped to <= high
     auto make_batch = [&](uint64_t lo, std::vector<uint32_t>& buf, long long* n,
                           uint64_t* nextLow) -> rs_status {
+        if (gpuPrimes) {  // a p-range holding about arraySize primes, sieved on the GPU
+            const double lnp = std::log((double)std::max<uint64_t>(lo, 16));
+            uint64_t width = (uint64_t)(arraySize * lnp);
+            width = std::min<uint64_t>(std::max<uint64_t>(width, 2), 1ull << 32);
+            *nextLow = std::min<uint64_t>(lo + width, high + 1);
+            *n = -1;
+            return RS_OK;
+        }
         rs_status s2 = rs_primes_batch(gen, lo, buf.data(), arraySize, nextLow);
         if (s2 != RS_OK) return s2;
         long long c = arraySize;
@@ -169,9 +183,16 @@ int main(int argc, char* argv[]) {
     while (true) {
         auto tb = std::chrono::steady_clock::now();
         ++kernelCount;
-        if (curN > 0 && (st = rs_sieve_submit(sv, curLow, cur.data(), curN)) != RS_OK)
-            return fail("submit", st);
-        const unsigned long long progress = curN > 0 ? cur[curN - 1] - cur[0] : 0;
+        unsigned long long progress = 0;
+        if (curN < 0) {  // GPU-generated primes of [curLow, nextLow)
+            if ((st = rs_sieve_submit_range(sv, curLow, nextLow)) != RS_OK)
+                return fail("submit", st);
+            progress = nextLow - curLow;
+        } else {
+            if (curN > 0 && (st = rs_sieve_submit(sv, curLow, cur.data(), curN)) != RS_OK)
+                return fail("submit", st);
+            progress = curN > 0 ? cur[curN - 1] - cur[0] : 0;
+        }
         const bool more = nextLow <= high && nextLow > curLow;
         const uint64_t nLow = nextLow;
         if (more && (st = make_batch(nLow, nxt, &nxtN, &nextLow)) != RS_OK)

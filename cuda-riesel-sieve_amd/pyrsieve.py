@@ -59,12 +59,16 @@ _SIGS = {
     "rs_primes_range": (i32, [vp, u64, u64, P(u32), i64, P(i64)]),
     "rs_primes_batch": (i32, [vp, u64, P(u32), i64, P(u64)]),
     "rs_primegen_free": (None, [vp]),
+    "rs_gpu_primegen_create": (i32, [i32, u64, P(vp)]),
+    "rs_gpu_primes_range": (i32, [vp, u64, u64, vp, i64, P(i64)]),
+    "rs_gpu_primegen_free": (None, [vp]),
     "rs_verify_factor": (i32, [u64, u32, u32, u32]),
     "rs_format_factor": (i32, [P(Factor), u32, ctypes.c_char_p, i32]),
     "rs_sieve_create": (i32, [i32, vp, i32, i32, i64, i64, P(vp)]),
     "rs_sieve_info": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32)]),
     "rs_sieve_submit": (i32, [vp, u64, P(u32), i64]),
     "rs_sieve_submit_device": (i32, [vp, u64, vp, i64]),
+    "rs_sieve_submit_range": (i32, [vp, u64, u64]),
     "rs_sieve_collect": (i32, [vp, P(Factor), i64, P(i64), P(i64)]),
     "rs_sieve_sync": (i32, [vp]),
     "rs_sieve_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
@@ -84,6 +88,7 @@ _SIGS = {
     "rs_dev_jacobi": (i32, [i32, P(u32), P(u64), P(i32), i64]),
     "rs_dev_alloc": (i32, [i32, i64, P(vp)]),
     "rs_dev_upload": (i32, [i32, vp, vp, i64]),
+    "rs_dev_download": (i32, [i32, vp, vp, i64]),
     "rs_dev_free": (i32, [i32, vp]),
     "rs_dev_mad_rate": (i32, [i32, i32, i32, P(ctypes.c_double)]),
     "rs_dev_copy_rate": (i32, [i32, i64, P(ctypes.c_double)]),
@@ -266,6 +271,37 @@ class PrimeGen:
             self.h = vp()
 
 
+class GpuPrimeGen:
+    """Prime offsets generated on the device (into a DeviceBuffer)."""
+
+    h = vp()
+
+    def __init__(self, hi_max, device=0):
+        h = vp()
+        _chk("rs_gpu_primegen_create", lib().rs_gpu_primegen_create(device, hi_max,
+                                                                    ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def range(self, lo, hi):
+        """Primes of [lo, hi) as a host uint32 array (through a device buffer)."""
+        cap = int(2 * (hi - lo) / max(1.0, np.log(max(hi - lo, 3)))) + 64
+        buf = DeviceBuffer(self.device, np.zeros(cap, np.uint32))
+        n = i64()
+        _chk("rs_gpu_primes_range", lib().rs_gpu_primes_range(self.h, lo, hi, buf.ptr, cap,
+                                                              ctypes.byref(n)))
+        out = np.zeros(n.value, np.uint32)
+        _chk("rs_dev_download", lib().rs_dev_download(self.device, out.ctypes.data_as(vp),
+                                                      buf.ptr, out.nbytes))
+        buf.free()
+        return out
+
+    def __del__(self):
+        if _lib is not None and self.h:
+            _lib.rs_gpu_primegen_free(self.h)
+            self.h = vp()
+
+
 def verify_factor(p, k, b, n):
     return bool(lib().rs_verify_factor(p, k, b, n))
 
@@ -329,6 +365,10 @@ class Sieve:
         count = dbuf.count - start if count is None else count
         ptr = vp(dbuf.ptr.value + 4 * start)
         _chk("rs_sieve_submit_device", lib().rs_sieve_submit_device(self.h, low, ptr, count))
+
+    def submit_range(self, lo, hi):
+        """Primes of [lo, hi) generated on the device, then sieved."""
+        _chk("rs_sieve_submit_range", lib().rs_sieve_submit_range(self.h, lo, hi))
 
     def collect(self):
         """All hits since the last collect as a sorted list of (p, k, n)."""

@@ -126,6 +126,23 @@ rs_status rs_primes_batch(rs_primegen* g, uint64_t low, uint32_t* offsets,
                           int64_t count, uint64_t* next_low);
 void rs_primegen_free(rs_primegen* g);
 
+/* Prime generation on the GPU (the step before the sieve): segmented
+ * sieve in LDS with the odd primes up to min(sqrt(hi_max), 2^22), and
+ * deterministic Miller-Rabin above that bound squared.  Offsets p - lo go to
+ * DEVICE memory.  No reference counterpart (the reference generates on the
+ * CPU, :1838-1987). */
+typedef struct rs_gpu_primegen rs_gpu_primegen;
+
+rs_status rs_gpu_primegen_create(int device, uint64_t hi_max,
+                                 rs_gpu_primegen** out);
+/* Every prime in [lo, hi) (hi - lo <= 2^32, 3 <= lo, hi <= hi_max) into
+ * offsets_dev; RS_ERR_OVERFLOW with *count = the number found if it exceeds
+ * cap (nothing past cap is written).  Synchronous. */
+rs_status rs_gpu_primes_range(rs_gpu_primegen* g, uint64_t lo, uint64_t hi,
+                              uint32_t* offsets_dev, int64_t cap,
+                              int64_t* count);
+void rs_gpu_primegen_free(rs_gpu_primegen* g);
+
 /* ------------------------------------------------------------ verification */
 /* 1 if p | k*b^n - 1 (128-bit square-and-multiply, :1656-1671). */
 int rs_verify_factor(uint64_t p, uint32_t k, uint32_t b, uint32_t n);
@@ -158,6 +175,11 @@ rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* offsets,
  * persistent launch, asynchronous. */
 rs_status rs_sieve_submit_device(rs_sieve* s, uint64_t low,
                                  const uint32_t* offsets_dev, int64_t count);
+/* Generates every prime of [lo, hi) on the device and sieves them: no host
+ * prime generation and no PCIe copy (3 <= lo, hi <= 2^63; ranges wider than
+ * 2^32 are split).  Asynchronous.  Replaces generateGPUPrimes (:1838-1987)
+ * + :1543-1578 in one call. */
+rs_status rs_sieve_submit_range(rs_sieve* s, uint64_t lo, uint64_t hi);
 /* Waits for everything submitted and returns all hits found since the last
  * collect, sorted by (p, k, n).  If cap is too small: RS_ERR_OVERFLOW with
  * *count = hits held (kept for the next call).  *lost (optional) > 0 means
@@ -238,6 +260,7 @@ rs_status rs_dev_jacobi(int device, const uint32_t* a, const uint64_t* p,
  * reference counterpart beyond the cudaMalloc/cudaMemcpy of :1387-1512). */
 rs_status rs_dev_alloc(int device, int64_t bytes, void** ptr);
 rs_status rs_dev_upload(int device, void* dst, const void* src, int64_t bytes);
+rs_status rs_dev_download(int device, void* dst, const void* src, int64_t bytes);
 rs_status rs_dev_free(int device, void* ptr);
 
 /* Measured rate of the 32x32+64 -> 64 multiply-add (v_mad_u64_u32) on the

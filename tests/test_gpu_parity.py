@@ -314,3 +314,45 @@ def test_batched_jobs_equal_alone_and_oracle(rs, orc, gpu):
         assert mine == run_oracle(orc, pl, m, lows[j], bufs[j]), j
     assert len(first) > 0
     bat.close()
+
+
+def test_gpu_primes_equal_cpu_generator(rs, gpu):
+    """GPU segmented sieve (+ Miller-Rabin above the sieving bound) vs the
+    host generator, including tiny ranges, 1e12 and ranges near 2^62/2^63."""
+    cases = [(3, 1 << 20, 1 << 22), (10**9, 10**9 + (1 << 24), 10**12),
+             (10**12, 10**12 + (1 << 26), 10**12 + (1 << 27)),
+             ((1 << 62) - (1 << 20), 1 << 62, 1 << 62),
+             ((1 << 63) - (1 << 20), 1 << 63, 1 << 63)]
+    for lo, hi, hmax in cases:
+        g = rs.GpuPrimeGen(hmax)
+        got = g.range(lo, hi)
+        want = rs.PrimeGen(hi, threads=16).range(lo, hi)
+        assert np.array_equal(got, want), (lo, hi)
+
+
+def test_submit_range_equals_host_primes(rs, orc, gpu):
+    plan = rs.Plan(rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2))
+    low, width = 10**12 + 1, 1 << 24
+    offs = rs.PrimeGen(low + width).range(low, low + width)
+    a = run_device(rs, plan, low, offs, m=576)
+    sv = rs.Sieve(plan, m=576)
+    sv.submit_range(low, low + width // 2)
+    sv.submit_range(low + width // 2, low + width)
+    b = sv.collect()
+    t = sv.timing()
+    sv.close()
+    assert a == b
+    assert t["primes"] == len(offs)
+    assert a == run_oracle(orc, plan, 576, low, offs)
+
+
+def test_cli_gpu_generated_primes(tmp_path, gpu):
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+    outs = []
+    for extra in ([], ["-g"]):
+        out = tmp_path / f"f{len(extra)}.txt"
+        r = subprocess.run([exe, "-i", R745, "-O", str(out), "-v", "-m", "8"] + extra,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(sorted(out.read_text().splitlines()))
+    assert outs[0] == outs[1] and f"{KNOWN_P} | 18300*745^242670-1" in outs[1]
