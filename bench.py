@@ -144,6 +144,22 @@ def main():
             "mean_launch_ms": bsgs_s_per_launch * 1e3,
             "hbm_copy_GBps": hbm / 1e9, "copy_probe_bytes": copy_bytes}
 
+    # end to end on the device: primes generated on the GPU, then sieved
+    # (same p-range, no host generation, no PCIe), as an extra figure
+    sv.collect()
+    sv.timing()
+    t0 = time.perf_counter()
+    for i in range(K):
+        sv.submit_range(p0 + i * chunk, p0 + (i + 1) * chunk)
+    hits_e2e = sv.collect()
+    e2e_s = time.perf_counter() - t0
+    e2e_primes = sv.timing()["primes"]
+    if hits_e2e != hits or e2e_primes != nprimes:
+        raise RuntimeError("device-generated primes disagree with the host-generated run")
+    end_to_end = {"value": K * chunk / e2e_s, "unit": "p/sec", "primes_per_sec": e2e_primes / e2e_s,
+                  "what": "GPU prime generation + sieve of the same p-range, timed together"}
+    log(f"end to end (GPU primes): {K * chunk / e2e_s:.3e} p/s")
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -187,6 +203,7 @@ def main():
         "primes_per_sec": primes_rate,
         "factors": len(hits),
         "kernel_ms": tm["kernel_ms"],
+        "end_to_end_gpu_primes": end_to_end,
         "host_primegen_s": gen_s,
         "roofline": roof,
         "cpu_baseline": cpu,

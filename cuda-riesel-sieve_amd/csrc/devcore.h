@@ -1,0 +1,68 @@
+// devcore.h -- internal interface shared by the device sieve front ends.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../host/host.h"
+#include "kernels.h"
+
+extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
+                                           const ChunkDev* chunks, int nchunks,
+                                           long long ngroups, int lds_words, int scratch_words,
+                                           uint64_t* scratch, unsigned long long* work,
+                                           HitDev* hits, uint32_t* hit_count, uint32_t hit_cap);
+extern "C" __global__ void rs_zero_u32(uint32_t*, int);
+
+namespace rsdev {
+
+const int kMaxWavesPerBlock = 4;
+
+// Job constants and device row tables built from a plan.
+struct DevJob {
+    JobDev J{};
+    uint32_t* d_kvals = nullptr;
+    uint32_t* d_rowinfo = nullptr;
+    uint32_t* d_rowk = nullptr;
+    uint32_t* d_bitmap = nullptr;
+};
+
+// Resources of the persistent kernel on one device: grid shape, per-wave
+// record scratch, work counter and hit buffer.
+struct Launcher {
+    int device = 0;
+    int grid = 0;
+    int wpb = 1;
+    int lds_words = 0;
+    size_t lds_bytes = 0;
+    int scratch_words = 0;  // per wavefront: 64 records
+    uint64_t* scratch = nullptr;
+    unsigned long long* work = nullptr;
+    HitDev* hits = nullptr;
+    uint32_t* hit_count = nullptr;
+    uint32_t hit_cap = 0;
+};
+
+struct Timing {
+    hipEvent_t a, b;
+    long long primes;
+};
+
+uint32_t magic_div(int d);
+void free_job(DevJob& j);
+// Job constants from a plan, table size m (0: cost model), giant-step tile
+// (1, 2 or 4; 0: 2); with dat, the plan's n-list bitmaps go to the device
+// and hits outside the lists are dropped by the kernel.
+rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj);
+// Device bytes make_job allocates for a plan (row tables, bitmaps if dat).
+size_t job_device_bytes(const rs_plan* pl, bool dat);
+void free_launcher(Launcher& L);
+rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity);
+rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const ChunkDev& c0,
+                 const JobDev* jobs, const ChunkDev* chunks, int nchunks, long long ngroups);
+rs_status drain_hits(const Launcher& L, hipStream_t stream, std::vector<HitDev>* out,
+                     int64_t* lost);
+bool hit_less(const HitDev& a, const HitDev& b);
+
+}  // namespace rsdev

@@ -23,15 +23,10 @@
 #include <vector>
 
 #include "../host/host.h"
+#include "devcore.h"
 #include "gpuprimes.h"
 #include "kernels.h"
 
-extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
-                                           const ChunkDev* chunks, int nchunks,
-                                           long long ngroups, int lds_words, int scratch_words,
-                                           uint64_t* scratch, unsigned long long* work,
-                                           HitDev* hits, uint32_t* hit_count, uint32_t hit_cap);
-extern "C" __global__ void rs_zero_u32(uint32_t*, int);
 extern "C" __global__ void rs_accum_count(const uint32_t*, unsigned long long*);
 
 #define HIPCHK(x)                                 \
@@ -39,193 +34,8 @@ extern "C" __global__ void rs_accum_count(const uint32_t*, unsigned long long*);
         if ((x) != hipSuccess) return RS_ERR_HIP; \
     } while (0)
 
-namespace {
+using namespace rsdev;
 
-const int kMaxWavesPerBlock = 4;
-
-// Job constants and device row tables built from a plan.
-struct DevJob {
-    JobDev J{};
-    uint32_t* d_kvals = nullptr;
-    uint32_t* d_rowinfo = nullptr;
-    uint32_t* d_rowk = nullptr;
-};
-
-void free_job(DevJob& j) {
-    (void)hipFree(j.d_kvals);
-    (void)hipFree(j.d_rowinfo);
-    (void)hipFree(j.d_rowk);
-    j.d_kvals = j.d_rowinfo = j.d_rowk = nullptr;
-}
-
-uint32_t magic_div(int d) {
-    return d == 1 ? 0u : (uint32_t)((0x100000000ull + d - 1) / d);
-}
-
-// Fills the job constants from a plan, table size m (0: cost model) and
-// giant-step tile tw (1, 2 or 4 items per lane; 0: 2).
-rs_status make_job(const rs_plan* pl, int m, int tw, DevJob* dj) {
-    if (!pl) return RS_ERR_ARG;
-    if (m <= 0) m = rs_plan_auto_m(pl);
-    if (tw <= 0) tw = 2;
-    if (tw > 4) return RS_ERR_ARG;
-    if (m < 64 || m % 64 != 0 || m > (1 << 16)) return RS_ERR_ARG;
-    if (pl->Q < 2 || pl->Q % 2 != 0 || pl->minN < 1 || pl->rows_k.empty()) return RS_ERR_ARG;
-    const int nrows = (int)pl->rows_k.size();
-    std::vector<uint32_t> kvals, rowinfo(nrows), rowk(nrows);
-    for (int32_t k : pl->rows_k)
-        if (kvals.empty() || kvals.back() != (uint32_t)k) kvals.push_back((uint32_t)k);
-    if (kvals.size() > 65535) return RS_ERR_ARG;
-    for (int i = 0; i < nrows; ++i) {
-        const uint32_t kidx =
-            (uint32_t)(std::lower_bound(kvals.begin(), kvals.end(), (uint32_t)pl->rows_k[i]) -
-                       kvals.begin());
-        rowinfo[i] = kidx | ((uint32_t)pl->rows_r[i] << 16);
-        rowk[i] = (uint32_t)pl->rows_k[i];
-    }
-    JobDev& J = dj->J;
-    J.base = pl->base;
-    J.Q = pl->Q;
-    J.m = m;
-    J.x0 = pl->x0;
-    J.T = (pl->x1 - pl->x0 + 1 + m - 1) / m;
-    J.minN = pl->minN;
-    J.maxN = pl->maxN;
-    J.nrows = nrows;
-    J.nk = (int)kvals.size();
-    // buckets: average load between 1.5 and 3 of the 8 slots
-    int logNB = 0;
-    while ((1 << logNB) * 3 < m) ++logNB;
-    J.logNB = std::max(logNB, 2);
-    if (J.Q * J.T >= 65536 || J.T > 65535) return RS_ERR_ARG;
-    J.invT = magic_div(J.T);
-    if (tw == 3) tw = 2;
-    J.tw = tw;
-    // flattened giant-step indices (and their division by T) stay below 2^16
-    if ((long long)nrows * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;
-    J.W = rec_words(J.Q, J.T, J.nk);
-    J.off_gt = REC_BH + J.Q;
-    J.off_j = J.off_gt + J.T;
-    J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.T, nrows, J.W);
-    HIPCHK(hipMalloc(&dj->d_kvals, kvals.size() * 4));
-    HIPCHK(hipMalloc(&dj->d_rowinfo, rowinfo.size() * 4));
-    HIPCHK(hipMalloc(&dj->d_rowk, rowk.size() * 4));
-    HIPCHK(hipMemcpy(dj->d_kvals, kvals.data(), kvals.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dj->d_rowinfo, rowinfo.data(), rowinfo.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dj->d_rowk, rowk.data(), rowk.size() * 4, hipMemcpyHostToDevice));
-    J.kvals = dj->d_kvals;
-    J.rowinfo = dj->d_rowinfo;
-    J.rowk = dj->d_rowk;
-    return RS_OK;
-}
-
-// Resources of the persistent kernel on one device: grid shape, per-wave
-// record scratch, work counter and hit buffer.
-struct Launcher {
-    int device = 0;
-    int grid = 0;
-    int wpb = 1;
-    int lds_words = 0;
-    size_t lds_bytes = 0;
-    int scratch_words = 0;  // per wavefront: 64 records
-    uint64_t* scratch = nullptr;
-    unsigned long long* work = nullptr;
-    HitDev* hits = nullptr;
-    uint32_t* hit_count = nullptr;
-    uint32_t hit_cap = 0;
-};
-
-void free_launcher(Launcher& L) {
-    (void)hipFree(L.scratch);
-    (void)hipFree(L.work);
-    (void)hipFree(L.hits);
-    (void)hipFree(L.hit_count);
-    L.scratch = nullptr;
-    L.work = nullptr;
-    L.hits = nullptr;
-    L.hit_count = nullptr;
-}
-
-// Sizes the grid for lds_words per wave and record size W (both maxima over
-// the jobs of the launch) and allocates scratch, counter and hits.
-rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity) {
-    L->device = device;
-    int max_lds = 0, cus = 0;
-    HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    const size_t per_wave = (size_t)lds_words * 4;
-    if (per_wave > (size_t)max_lds) return RS_ERR_ARG;
-    // waves of a block are independent (one table each); up to 4 per block
-    L->wpb = (int)std::min<size_t>(kMaxWavesPerBlock, (size_t)max_lds / per_wave);
-    L->lds_words = lds_words;
-    L->lds_bytes = per_wave * L->wpb;
-    if (L->lds_bytes > 65536)
-        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(rs_sieve_kernel),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)L->lds_bytes));
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(rs_sieve_kernel), 64 * L->wpb, L->lds_bytes));
-    L->grid = std::max(1, cus * std::max(1, per_cu));
-    L->scratch_words = 64 * W;
-    const size_t waves = (size_t)L->grid * L->wpb;
-    if (hipMalloc(&L->scratch, waves * L->scratch_words * 8) != hipSuccess) return RS_ERR_NOMEM;
-    HIPCHK(hipMalloc(&L->work, 8));
-    L->hit_cap = (uint32_t)hit_capacity;
-    if (hipMalloc(&L->hits, (size_t)hit_capacity * sizeof(HitDev)) != hipSuccess)
-        return RS_ERR_NOMEM;
-    HIPCHK(hipMalloc(&L->hit_count, 4));
-    HIPCHK(hipMemset(L->hit_count, 0, 4));
-    return RS_OK;
-}
-
-// Zeroes the work counter and launches the persistent kernel on `stream`.
-// Kernel-only (no memset nodes), so it can be captured in a graph.
-rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const ChunkDev& c0,
-                 const JobDev* jobs, const ChunkDev* chunks, int nchunks, long long ngroups) {
-    hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)L.work, 2);
-    HIPCHK(hipGetLastError());
-    const int grid = (int)std::min<long long>(L.grid, (ngroups + L.wpb - 1) / L.wpb);
-    hipLaunchKernelGGL(rs_sieve_kernel, dim3(std::max(1, grid)), dim3(64 * L.wpb), L.lds_bytes,
-                       stream, j0, c0, jobs, chunks, nchunks, ngroups, L.lds_words,
-                       L.scratch_words, L.scratch, L.work, L.hits, L.hit_count, L.hit_cap);
-    HIPCHK(hipGetLastError());
-    return RS_OK;
-}
-
-// Copies the hits out (appending to `out`) and zeroes the device counter.
-rs_status drain_hits(const Launcher& L, hipStream_t stream, std::vector<HitDev>* out,
-                     int64_t* lost) {
-    HIPCHK(hipStreamSynchronize(stream));
-    uint32_t n = 0;
-    HIPCHK(hipMemcpy(&n, L.hit_count, 4, hipMemcpyDeviceToHost));
-    const uint32_t got = std::min(n, L.hit_cap);
-    if (n > L.hit_cap) *lost += (int64_t)(n - L.hit_cap);
-    if (got) {
-        const size_t old = out->size();
-        out->resize(old + got);
-        HIPCHK(hipMemcpy(out->data() + old, L.hits, (size_t)got * sizeof(HitDev),
-                         hipMemcpyDeviceToHost));
-    }
-    hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, L.hit_count, 1);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(stream));
-    return RS_OK;
-}
-
-bool hit_less(const HitDev& a, const HitDev& b) {
-    if (a.job != b.job) return a.job < b.job;
-    if (a.p != b.p) return a.p < b.p;
-    if (a.k != b.k) return a.k < b.k;
-    return a.n < b.n;
-}
-
-struct Timing {
-    hipEvent_t a, b;
-    long long primes;
-};
-
-}  // namespace
 
 extern "C" int rs_device_count(void) {
     int n = 0;
@@ -293,7 +103,7 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int t
     rs_sieve* s = new rs_sieve();
     s->device = device;
     s->max_batch = max_batch;
-    rs_status st = make_job(pl, m, tile, &s->job);
+    rs_status st = make_job(pl, m, tile, false, &s->job);
     if (st == RS_OK) st = init_launcher(&s->L, device, s->job.J.lds_words, s->job.J.W, hit_capacity);
     if (st == RS_OK &&
         (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -608,7 +418,7 @@ extern "C" rs_status rs_batch_create(int device, int njobs, const rs_plan* const
     rs_status st = RS_OK;
     int lds_words = 0, W = 0;
     for (int i = 0; i < njobs && st == RS_OK; ++i) {
-        st = make_job(plans[i], ms ? ms[i] : 0, tiles ? tiles[i] : 0, &b->jobs[i]);
+        st = make_job(plans[i], ms ? ms[i] : 0, tiles ? tiles[i] : 0, false, &b->jobs[i]);
         if (st == RS_OK) {
             lds_words = std::max(lds_words, b->jobs[i].J.lds_words);
             W = std::max(W, b->jobs[i].J.W);

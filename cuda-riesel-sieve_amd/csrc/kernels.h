@@ -34,6 +34,8 @@ struct JobDev {
     const uint32_t* kvals;    // distinct k, ascending [nk]
     const uint32_t* rowinfo;  // per row: kidx | r << 16
     const uint32_t* rowk;     // per row: k
+    const uint32_t* bitmap;   // optional n-list bitmaps (rows x words, MSB first)
+    int words;                // 32-bit words per row bitmap
 };
 
 // A chunk of primes p = low + offs[i], i < count, of one job; group0 is
@@ -66,7 +68,7 @@ static inline int rec_words(int Q, int T, int nk) {
 //   [0, 4 NB)                 table: NB buckets x 8 uint16 fingerprints (0 = empty)
 //   [4 NB, 4 NB + NB/2)       bucket fill counts, two uint16 per word
 //   then 2 Q T                Y products (uint64)
-//   then 2 nrows              active rows (k, r T << 8 | r)
+//   then 2 nrows              active rows (k, row << 8 | r)
 //   then 2 W                  the prime's record (uint64)
 static inline int lds_words_per_wave(int logNB, int Q, int T, int nrows, int W) {
     const int NB = 1 << logNB;

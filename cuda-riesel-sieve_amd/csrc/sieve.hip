@@ -72,19 +72,26 @@ __device__ __forceinline__ u32 has_zero_half(u32 x) {
 }
 
 // Emits every baby step j with g^j == target (all 64 lanes take part:
-// lane l checks j = l + 64 v) as a hit of giant step tt of row (k, r).
-__device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32 k, u32 r,
+// lane l checks j = l + 64 v) as a hit of giant step tt of row `row`
+// (k, r).  With n-list bitmaps only n in the row's list is emitted.
+__device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32 k, u32 ry,
                                        int tt, u64 vstart, u64 g64, u64 p, u64 mp,
                                        HitDev* __restrict__ hits, u32* __restrict__ hit_count,
                                        u32 hit_cap) {
     const int lane = threadIdx.x & 63;
+    const u32 r = ry & 0xffu, row = ry >> 8;
     u64 w = vstart;
     for (int v = 0; v < (J.m >> 6); ++v) {
         if (w == target) {
             const int j = lane + 64 * v;
             const long long xx = (long long)J.x0 + (long long)tt * J.m + (J.m - 1 - j);
             const long long n = (long long)J.Q * xx + (int)r;
-            if (n >= J.minN && n <= J.maxN) {
+            bool keep = n >= J.minN && n <= J.maxN;
+            if (keep && J.bitmap) {
+                const long long xp = xx - J.x0;  // x' of the bitmap, < 32 words
+                keep = (J.bitmap[(long long)row * J.words + (xp >> 5)] >> (31 - (xp & 31))) & 1;
+            }
+            if (keep) {
                 const u32 at = atomicAdd(hit_count, 1u);
                 if (at < hit_cap) {
                     HitDev h;
@@ -134,9 +141,9 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A,
             const int ac = min(a, A - 1);
             const uint2 row = list[ac];
             ku[u] = row.x;
-            ru[u] = row.y & 0xffu;
+            ru[u] = row.y;
             tu[u] = t;
-            s[u] = halfmul(row.x, Y[(row.y >> 8) + min(t, T - 1)], p, mp0);
+            s[u] = halfmul(row.x, Y[(row.y & 0xffu) * T + min(t, T - 1)], p, mp0);
             bk[u] = lo32(s[u]) & (NB - 1);
             fp2[u] = fingerprint(s[u], J.logNB) * 0x10001u;
             if (++t == T) { t = 0; ++a; }
@@ -293,7 +300,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
         Y[idx] = montmul(R[REC_BH + r], R[J.off_gt + t], p, mp);
     }
 
-    // rows passing the Jacobi filter, compacted in order as (k, rT << 8 | r)
+    // rows passing the Jacobi filter, compacted in order as (k, row << 8 | r)
     int A = 0;
     for (int c = 0; c < J.nrows; c += 64) {
         const int row = c + lane;
@@ -310,7 +317,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
         const u64 mask = __ballot(pass);
         if (pass)
             list[A + __popcll(mask & ((1ull << lane) - 1))] =
-                make_uint2(k, ((r * (u32)T) << 8) | r);
+                make_uint2(k, ((u32)row << 8) | r);
         A += __popcll(mask);
     }
     wave_sync();

@@ -83,6 +83,14 @@ _SIGS = {
     "rs_batch_collect": (i32, [vp, P(JobFactor), i64, P(i64), P(i64)]),
     "rs_batch_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
     "rs_batch_destroy": (None, [vp]),
+    "rs_server_create": (i32, [i32, i64, i32, u64, i32, i64, P(vp)]),
+    "rs_server_submit": (i32, [vp, vp, i32, i32, i32, u64, u64, P(i64)]),
+    "rs_server_step": (i32, [vp, P(i32), P(i32), P(i64)]),
+    "rs_server_poll": (i32, [vp, i64, P(i32), P(u64), P(i64)]),
+    "rs_server_results": (i32, [vp, i64, P(Factor), i64, P(i64)]),
+    "rs_server_release": (i32, [vp, i64]),
+    "rs_server_stats": (i32, [vp, P(i64), P(i64), P(ctypes.c_double), P(i64)]),
+    "rs_server_destroy": (None, [vp]),
     "rs_dev_montmul": (i32, [i32, P(u64), P(u64), P(u64), P(u64), i64]),
     "rs_dev_halfmul": (i32, [i32, P(u32), P(u64), P(u64), P(u64), i64]),
     "rs_dev_jacobi": (i32, [i32, P(u32), P(u64), P(i32), i64]),
@@ -269,6 +277,70 @@ class PrimeGen:
         if _lib is not None and self.h:
             _lib.rs_primegen_free(self.h)
             self.h = vp()
+
+
+class Server:
+    """Continuous-batching sieve server (jobs admitted by HBM budget)."""
+
+    QUEUED, RUNNING, DONE, REJECTED = 0, 1, 2, -1
+    h = vp()
+
+    def __init__(self, hbm_budget, max_active=8, chunk_width=1 << 30, lds_cap=16384,
+                 hit_capacity=1 << 20, device=0):
+        h = vp()
+        _chk("rs_server_create", lib().rs_server_create(device, hbm_budget, max_active,
+                                                        chunk_width, lds_cap, hit_capacity,
+                                                        ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, plan, p_lo, p_hi, m=0, tile=0, dat=False):
+        jid = i64()
+        _chk("rs_server_submit", lib().rs_server_submit(self.h, plan.h, m, tile, int(dat), p_lo,
+                                                        p_hi, ctypes.byref(jid)))
+        return jid.value
+
+    def step(self):
+        """(active, queued, primes sieved in this step)."""
+        a, q, n = i32(), i32(), i64()
+        _chk("rs_server_step", lib().rs_server_step(self.h, ctypes.byref(a), ctypes.byref(q),
+                                                    ctypes.byref(n)))
+        return a.value, q.value, n.value
+
+    def poll(self, jid):
+        st, done, nh = i32(), u64(), i64()
+        _chk("rs_server_poll", lib().rs_server_poll(self.h, jid, ctypes.byref(st),
+                                                    ctypes.byref(done), ctypes.byref(nh)))
+        return st.value, done.value, nh.value
+
+    def results(self, jid):
+        cnt = i64()
+        cap = 1024
+        while True:
+            buf = (Factor * cap)()
+            st = lib().rs_server_results(self.h, jid, buf, cap, ctypes.byref(cnt))
+            if st == RS_ERR_OVERFLOW:
+                cap = cnt.value
+                continue
+            _chk("rs_server_results", st)
+            return [(int(f.p), int(f.k), int(f.n)) for f in buf[: cnt.value]]
+
+    def release(self, jid):
+        _chk("rs_server_release", lib().rs_server_release(self.h, jid))
+
+    def stats(self):
+        u, b, ms, n = i64(), i64(), ctypes.c_double(), i64()
+        _chk("rs_server_stats", lib().rs_server_stats(self.h, ctypes.byref(u), ctypes.byref(b),
+                                                      ctypes.byref(ms), ctypes.byref(n)))
+        return dict(used=u.value, budget=b.value, kernel_ms=ms.value, primes=n.value)
+
+    def close(self):
+        if self.h:
+            lib().rs_server_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        if _lib is not None and self.h:
+            self.close()
 
 
 class GpuPrimeGen:

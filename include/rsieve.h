@@ -245,6 +245,43 @@ rs_status rs_batch_collect(rs_batch* b, rs_job_factor* out, int64_t cap,
 rs_status rs_batch_timing(rs_batch* b, double* kernel_ms, int64_t* primes);
 void rs_batch_destroy(rs_batch* b);
 
+/* ------------------------------------------------------------ serving */
+/* Continuous-batching server on one GPU (no reference counterpart).  Jobs
+ * arrive at any time and queue FIFO; each rs_server_step admits queued jobs
+ * while a slot is free and their device footprint (row tables, n-list
+ * bitmaps with dat, one prime buffer of chunk_width) fits hbm_budget,
+ * generates on the GPU the primes of every distinct p-window of the active
+ * jobs (jobs at equal progress share one), sieves all active jobs in one
+ * persistent launch, moves hits to their jobs and retires finished jobs.
+ * Job state stays resident on the device between steps.  lds_cap_bytes
+ * bounds one wavefront's LDS (a job's m is lowered to fit).  A job larger
+ * than the whole budget is rejected (state -1). */
+typedef struct rs_server rs_server;
+
+rs_status rs_server_create(int device, int64_t hbm_budget, int max_active,
+                           uint64_t chunk_width, int lds_cap_bytes,
+                           int64_t hit_capacity, rs_server** out);
+/* Queues a job: the plan is copied; m/tile as for rs_sieve_create; dat != 0
+ * keeps only n in the job's lists; primes p in [p_lo, p_hi). */
+rs_status rs_server_submit(rs_server* s, const rs_plan* pl, int m, int tile,
+                           int dat, uint64_t p_lo, uint64_t p_hi,
+                           int64_t* job_id);
+/* One scheduling step (synchronous).  Reports active and queued jobs after
+ * the step and the primes sieved in it. */
+rs_status rs_server_step(rs_server* s, int* active, int* queued,
+                         int64_t* primes_step);
+/* state: 0 queued, 1 running, 2 done, -1 rejected; p_done: p-range sieved. */
+rs_status rs_server_poll(rs_server* s, int64_t job_id, int* state,
+                         uint64_t* p_done, int64_t* nhits);
+/* The job's hits so far (sorted once the job is done). */
+rs_status rs_server_results(rs_server* s, int64_t job_id, rs_factor* out,
+                            int64_t cap, int64_t* count);
+/* Forgets a finished or rejected job. */
+rs_status rs_server_release(rs_server* s, int64_t job_id);
+rs_status rs_server_stats(rs_server* s, int64_t* used_bytes, int64_t* budget,
+                          double* kernel_ms, int64_t* primes);
+void rs_server_destroy(rs_server* s);
+
 /* Device-side helpers for tests (exact arithmetic on the GPU): out[i] =
  * montmul(a[i], b[i]) mod p[i] (R = 2^64) and out[i] = k[i]*y[i]*2^-32
  * mod p[i]; count elements, host buffers. */

@@ -356,3 +356,57 @@ def test_cli_gpu_generated_primes(tmp_path, gpu):
         assert r.returncode == 0, r.stderr
         outs.append(sorted(out.read_text().splitlines()))
     assert outs[0] == outs[1] and f"{KNOWN_P} | 18300*745^242670-1" in outs[1]
+
+
+def test_server_results_alone_and_in_the_mix(rs, orc, gpu):
+    """Jobs arriving between steps are admitted as slots and budget free up;
+    each job's hits equal the job sieved alone (and the oracle), datless and
+    with device-resident n-lists (dat)."""
+    specs = [(745, 6, 10**6, 20000, 30000, 1000), (2, 9, 10**8, 100000, 140000, 2000),
+             (5, 3, 10**5, 5000, 7000, 500), (1030, 12, 10**7, 40000, 90000, 3000),
+             (3, 1, 10**4, 1000, 100000, 200), (31, 4, 10**6, 30000, 60000, 1000)]
+    plans = [rs.Plan(rs.Abcd.synth(b, nk, 3, kmax, nlo, nhi, ps, seed=20 + i,
+                                   odd_k=(b % 2 == 0)))
+             for i, (b, nk, kmax, nlo, nhi, ps) in enumerate(specs)]
+    width = 1 << 22
+    ranges = [(10**9 + 1 + i * width, 10**9 + 1 + i * width + (2 + i % 3) * width)
+              for i in range(len(plans))]
+    ranges[1] = ranges[0]  # two jobs on the same windows share their primes
+    per_job = 4 * (2 * width / np.log(width) + 64)  # each job's prime buffer
+    srv = rs.Server(hbm_budget=int(2.5 * per_job), max_active=3, chunk_width=width,
+                    lds_cap=16384)
+    ids, pending = [], list(range(len(plans)))
+    steps = 0
+    while pending or any(srv.poll(j)[0] in (0, 1) for j in ids):
+        if pending and steps % 2 == 0:  # arrivals while others run
+            i = pending.pop(0)
+            ids.append(srv.submit(plans[i], *ranges[i], dat=(i == 3)))
+        srv.step()
+        steps += 1
+        assert steps < 200
+    for i, j in enumerate(ids):
+        st, done, nh = srv.poll(j)
+        assert st == srv.DONE and done == ranges[i][1] - ranges[i][0]
+        got = srv.results(j)
+        sv = rs.Sieve(plans[i], m=256 if i % 2 else 0)
+        sv.submit_range(*ranges[i])
+        alone = sv.collect()
+        sv.close()
+        if i == 3:
+            alone = [h for h in alone if plans[i].contains(h[1], h[2])]
+        assert got == alone, i
+    offs = rs.PrimeGen(ranges[2][1]).range(ranges[2][0], ranges[2][1])
+    assert srv.results(ids[2]) == run_oracle(orc, plans[2], 256, ranges[2][0], offs)
+    st = srv.stats()
+    assert st["used"] == 0 and st["primes"] > 0
+    srv.close()
+
+
+def test_server_rejects_job_over_budget(rs, gpu):
+    plan = rs.Plan(rs.Abcd.synth(745, 4, 3, 10**6, 10000, 20000, 1000, seed=1))
+    srv = rs.Server(hbm_budget=1 << 16, max_active=2, chunk_width=1 << 24)
+    j = srv.submit(plan, 10**9, 10**9 + (1 << 24))
+    assert srv.step() == (0, 0, 0)
+    assert srv.poll(j)[0] == srv.REJECTED
+    srv.release(j)
+    srv.close()
