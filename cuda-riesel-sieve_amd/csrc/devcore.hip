@@ -81,8 +81,9 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.invT = magic_div(J.T);
     if (tw == 3) tw = 2;
     J.tw = tw;
-    // flattened giant-step indices (and their division by T) stay below 2^16
-    if ((long long)nrows * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;
+    // flattened giant-step indices of a row slice (and their division by T)
+    // stay below 2^16
+    if ((long long)std::min<int>(nrows, ROW_SLICE) * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;
     J.W = rec_words(J.Q, J.T, J.nk);
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + J.T;

@@ -15,6 +15,10 @@
 enum { REC_P = 0, REC_MP = 1, REC_ONE = 2, REC_G64 = 3, REC_GP = 4, REC_G8P = 12,
        REC_BH = 20 };
 
+// Rows handled per compaction pass of the BSGS phase: bounds the LDS list
+// of active rows whatever the job's row count.
+enum { ROW_SLICE = 512 };
+
 struct JobDev {
     uint32_t base;
     int Q;
@@ -68,10 +72,11 @@ static inline int rec_words(int Q, int T, int nk) {
 //   [0, 4 NB)                 table: NB buckets x 8 uint16 fingerprints (0 = empty)
 //   [4 NB, 4 NB + NB/2)       bucket fill counts, two uint16 per word
 //   then 2 Q T                Y products (uint64)
-//   then 2 nrows              active rows (k, row << 8 | r)
+//   then 2 min(nrows, ROW_SLICE)  active rows of one slice (k, row << 8 | r)
 //   then 2 W                  the prime's record (uint64)
 static inline int lds_words_per_wave(int logNB, int Q, int T, int nrows, int W) {
     const int NB = 1 << logNB;
-    int w = 4 * NB + NB / 2 + 2 * Q * T + 2 * nrows + 2 * W;
+    const int list_rows = nrows < ROW_SLICE ? nrows : ROW_SLICE;  // one row slice
+    int w = 4 * NB + NB / 2 + 2 * Q * T + 2 * list_rows + 2 * W;
     return (w + 3) & ~3;
 }

@@ -262,7 +262,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     u32* const cnt = wbase + 4 * NB;
     u64* const Y = (u64*)(cnt + NB / 2);
     uint2* const list = (uint2*)(Y + Q * T);
-    u64* const R = (u64*)(list + J.nrows);
+    u64* const R = (u64*)(list + min(J.nrows, ROW_SLICE));
     const int L = m >> 6;
 
     if (lane < W) R[lane] = pre;
@@ -300,34 +300,37 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
         Y[idx] = montmul(R[REC_BH + r], R[J.off_gt + t], p, mp);
     }
 
-    // rows passing the Jacobi filter, compacted in order as (k, row << 8 | r)
-    int A = 0;
-    for (int c = 0; c < J.nrows; c += 64) {
-        const int row = c + lane;
-        bool pass = false;
-        u32 k = 0, r = 0;
-        if (row < J.nrows) {
-            const u32 info = J.rowinfo[row];
-            const u32 kidx = info & 0xffffu;
-            r = info >> 16;
-            k = J.rowk[row];
-            const u64 w = R[J.off_j + 2 * (kidx >> 6) + (r & 1)];
-            pass = (w >> (kidx & 63)) & 1;
+    // rows in slices of at most ROW_SLICE: those passing the Jacobi filter
+    // are compacted in order as (k, row << 8 | r), then their giant steps run
+    for (int r0 = 0; r0 < J.nrows; r0 += ROW_SLICE) {
+        const int r1 = min(J.nrows, r0 + ROW_SLICE);
+        int A = 0;
+        for (int c = r0; c < r1; c += 64) {
+            const int row = c + lane;
+            bool pass = false;
+            u32 k = 0, r = 0;
+            if (row < r1) {
+                const u32 info = J.rowinfo[row];
+                const u32 kidx = info & 0xffffu;
+                r = info >> 16;
+                k = J.rowk[row];
+                const u64 w = R[J.off_j + 2 * (kidx >> 6) + (r & 1)];
+                pass = (w >> (kidx & 63)) & 1;
+            }
+            const u64 mask = __ballot(pass);
+            if (pass)
+                list[A + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(k, ((u32)row << 8) | r);
+            A += __popcll(mask);
         }
-        const u64 mask = __ballot(pass);
-        if (pass)
-            list[A + __popcll(mask & ((1ull << lane) - 1))] =
-                make_uint2(k, ((u32)row << 8) | r);
-        A += __popcll(mask);
+        wave_sync();
+        if (J.tw >= 4)
+            giant_steps<4>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
+        else if (J.tw == 2)
+            giant_steps<2>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
+        else
+            giant_steps<1>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
+        wave_sync();  // the next slice rewrites the list
     }
-    wave_sync();
-
-    if (J.tw >= 4)
-        giant_steps<4>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
-    else if (J.tw == 2)
-        giant_steps<2>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
-    else
-        giant_steps<1>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
     wave_sync();  // the next prime reuses the table
 }
 

@@ -11,6 +11,7 @@
 // raises maxN only on delta lines).
 #include <algorithm>
 #include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -175,12 +176,26 @@ uint64_t inv_mod(uint64_t a, uint64_t q) {  // q prime, a != 0 mod q
     return rsh::powmod(a % q, q - 2, q);
 }
 
+// Multiplicative order of b mod prime q (b != 0 mod q), from q-1's factors.
+uint32_t order_mod(uint64_t b, uint32_t q) {
+    uint32_t ord = q - 1, r = q - 1;
+    for (uint32_t f = 2; (uint64_t)f * f <= r; ++f) {
+        if (r % f) continue;
+        while (r % f == 0) r /= f;
+        while (ord % f == 0 && rsh::powmod(b, ord / f, q) == 1) ord /= f;
+    }
+    if (r > 1 && rsh::powmod(b, ord / r, q) == 1) ord /= r;
+    return ord;
+}
+
 }  // namespace
 
 // Presieve: n survives iff no odd prime q < bound (q ∤ b) divides k*b^n-1.
-// For each q the discrete-log table of b mod q gives, per k, the residue
-// class n == dlog(k^-1) (mod ord_q(b)) that q removes.  Work is split over
-// q across threads, which mark one shared removal byte per (k, n).
+// For each q the subgroup generated by b mod q is walked once (b^i, i up to
+// ord_q(b)), looking each element up in a small hash of the targets
+// k^-1 mod q of the batch; a hit at i removes n == i (mod ord_q(b)) for that
+// k.  Work is split over q across threads, which mark one shared removal
+// byte per (k, n).
 extern "C" rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_t kmax,
                                    int odd_k, int nmin, int nmax,
                                    uint32_t presieve_bound, uint64_t seed,
@@ -220,26 +235,73 @@ extern "C" rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_
         std::vector<std::thread> th;
         for (int t = 0; t < nthreads; ++t) {
             th.emplace_back([&, t]() {
-                std::vector<int32_t> dlog;
+                // Per q: x_k = dlog_b(k^-1) mod ord by baby-step/giant-step
+                // over all k at once.  Baby entries t_k*b^j (j < m) go into
+                // an open-addressing table with chains for equal values;
+                // giant steps b^(i*m) then give x_k = i*m - j.  m is about
+                // sqrt(ord/B), so the cost is ~2 sqrt(ord*B) per q instead
+                // of ord (m = 1 is a plain walk of the subgroup).
+                std::vector<uint32_t> hval, hidx, ej;
+                std::vector<int32_t> next, ek;
+                std::vector<uint32_t> xk(B);
                 uint8_t* rm = removed.data();
                 for (size_t qi = t; qi < qs.size(); qi += nthreads) {
                     const uint32_t q = qs[qi];
                     const uint64_t bq = base % q;
-                    dlog.assign(q, -1);
-                    uint64_t v = 1;
-                    uint32_t ord = 0;
-                    do {
-                        dlog[v] = (int32_t)ord;
-                        v = v * bq % q;
-                        ++ord;
-                    } while (v != 1);
+                    const uint32_t ord = order_mod(bq, q);
+                    uint32_t m = (uint32_t)std::sqrt((double)ord / (double)B);
+                    if (m < 1) m = 1;
+                    if (m > ord) m = ord;
+                    const size_t E = B * (size_t)m;
+                    int hb = 1;
+                    while (((size_t)1 << hb) < 2 * E) ++hb;
+                    const uint32_t hmask = (1u << hb) - 1;
+                    hval.assign(hmask + 1, 0);
+                    hidx.assign(hmask + 1, 0xffffffffu);
+                    next.resize(E);
+                    ek.resize(E);
+                    ej.resize(E);
+                    std::fill(xk.begin(), xk.end(), 0xffffffffu);
+                    size_t ne = 0, want = 0;
                     for (size_t i = 0; i < B; ++i) {
                         const uint64_t k = batch[i] % q;
                         if (k == 0) continue;  // k*b^n-1 == -1 mod q
-                        const int32_t d = dlog[inv_mod(k, q)];
-                        if (d < 0) continue;
-                        // first n >= nmin with n == d (mod ord)
-                        long long n = d;
+                        ++want;
+                        uint64_t v = inv_mod(k, q);
+                        for (uint32_t j = 0; j < m; ++j, v = v * bq % q) {
+                            uint32_t h = ((uint32_t)v * 2654435761u) >> (32 - hb);
+                            while (hidx[h] != 0xffffffffu && hval[h] != (uint32_t)v)
+                                h = (h + 1) & hmask;
+                            next[ne] = hidx[h] == 0xffffffffu ? -1 : (int32_t)hidx[h];
+                            hval[h] = (uint32_t)v;
+                            hidx[h] = (uint32_t)ne;
+                            ek[ne] = (int32_t)i;
+                            ej[ne] = j;
+                            ++ne;
+                        }
+                    }
+                    if (!want) continue;
+                    const uint64_t gm = rsh::powmod(bq, m, q);
+                    uint64_t g = gm;
+                    size_t got = 0;
+                    for (uint64_t i = 1; i * m < (uint64_t)ord + m && got < want; ++i, g = g * gm % q) {
+                        uint32_t h = ((uint32_t)g * 2654435761u) >> (32 - hb);
+                        while (hidx[h] != 0xffffffffu) {
+                            if (hval[h] == (uint32_t)g) {
+                                for (int32_t e = (int32_t)hidx[h]; e >= 0; e = next[e])
+                                    if (xk[ek[e]] == 0xffffffffu) {
+                                        xk[ek[e]] = (uint32_t)((i * m - ej[e]) % ord);
+                                        ++got;
+                                    }
+                                break;
+                            }
+                            h = (h + 1) & hmask;
+                        }
+                    }
+                    for (size_t i = 0; i < B; ++i) {
+                        if (xk[i] == 0xffffffffu) continue;  // k^-1 not a power of b
+                        // first n >= nmin with n == x (mod ord)
+                        long long n = xk[i];
                         if (n < nmin) n += ((nmin - n + ord - 1) / ord) * (long long)ord;
                         uint8_t* row = rm + i * (size_t)width;
                         for (; n <= nmax; n += ord)

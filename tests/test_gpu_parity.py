@@ -226,6 +226,37 @@ def test_large_p_near_2_62_and_2_63(rs, orc, gpu):
         assert got == want
 
 
+def test_many_rows_sliced_all_tiles(rs, orc, gpu):
+    """200 k of base 1030 give ~1000+ active (k, r) rows, more than one LDS
+    row slice (512): every slice and every giant-step tile width must agree
+    with the oracle."""
+    plan = rs.Plan(rs.Abcd.synth(1030, 200, 3, 10**7, 100000, 120000, 10**5, seed=21))
+    assert plan.nrows > 1024
+    low = 10**9 + 1
+    offs = rs.PrimeGen(low + 2**19).range(low, low + 2**19)
+    m = plan.auto_m()
+    want = run_oracle(orc, plan, m, low, offs)
+    assert len(want) > 0
+    for tile in (1, 2, 4):
+        sv = rs.Sieve(plan, m=m, max_batch=1 << 15, tile=tile)
+        sv.submit(low, offs)
+        got = sv.collect()
+        sv.close()
+        assert got == want, tile
+
+
+def test_many_k_base2_near_2_62(rs, orc, gpu):
+    """Many-k / large-p edge: base 2, 500 odd k in [3, 1e9], n in [1e6, 2e6]
+    presieved by primes < 1e6, p in a window just below 2^62."""
+    plan = rs.Plan(rs.Abcd.synth(2, 500, 3, 10**9, 1000000, 2000000, 10**6, seed=3))
+    low = (1 << 62) - (1 << 18)
+    offs = rs.PrimeGen(1 << 62, threads=16).range(low, 1 << 62)
+    m = plan.auto_m()
+    got = run_device(rs, plan, low, offs, m=m, max_batch=1 << 14)
+    want = run_oracle(orc, plan, m, low, offs)
+    assert got == want
+
+
 def test_many_hits_small_p_datless(rs, orc, gpu):
     """p near 1e6: many hits per prime, exercises the append buffer."""
     plan = rs.Plan(rs.Abcd.synth(5, 10, 3, 10**5, 2000, 60000, 200, seed=4, odd_k=False))

@@ -141,6 +141,34 @@ def test_synth_presieve_exact(rs):
             assert (n in kept) == survives, (k, n)
 
 
+def test_synth_presieve_exact_many_k_base2(rs):
+    # more k and larger q than above: the per-q discrete logs use baby steps
+    # t_k*b^j with m > 1 and orders up to q-1
+    a = rs.Abcd.synth(2, 40, 3, 10**9, 1000, 2500, 20000, seed=5)
+    ks, kstart, ns = a.lists()
+    sieve = np.ones(20000, dtype=bool)
+    sieve[:3] = False
+    for i in range(2, 142):
+        if sieve[i]:
+            sieve[i * i::i] = False
+    qs = [int(q) for q in np.nonzero(sieve)[0] if q > 2]
+    width = 1501
+    dead = np.zeros((len(ks), width), dtype=bool)
+    kv = ks.astype(object)
+    for q in qs:
+        pw = np.empty(width, dtype=np.int64)
+        v = pow(2, 1000, q)
+        for j in range(width):
+            pw[j] = v
+            v = v * 2 % q
+        for i, k in enumerate(ks.tolist()):
+            dead[i] |= (k % q) * pw % q == 1
+    for i in range(len(ks)):
+        kept = np.zeros(width, dtype=bool)
+        kept[ns[kstart[i]:kstart[i + 1]] - 1000] = True
+        assert np.array_equal(kept, ~dead[i]), int(kv[i])
+
+
 def test_synth_is_seeded(rs):
     a = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2)
     b = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2)
