@@ -19,6 +19,18 @@ multiplies per launch / mean launch time (HIP events on the sieve's own
 stream), peak = the same instruction's rate measured on this device.
 cpu_baseline: the CPU oracle (a C++ restatement of the reference
 algorithm, OpenMP over primes) on a bounded sample of chunk 0.
+
+batched: 64 heterogeneous jobs (workloads.job_specs: base U[3,1030], k-count
+log-U[1,200], n-width log-U[1e4,5e5], presieve < 1e5) over the p-chunk
+[1e12, 1e12+2^30) with per-job autotuned (m, tile): the jobs run plain
+back-to-back and then as one graph-captured batch; per-job hit sets must be
+equal.
+serving: jobs of the same distribution each consume consecutive 2^30-wide
+p-chunks from 1e12 (the per-job bound is 1e12 + C*2^30, C = --serve-chunks,
+scaled down from 1e13 so the run takes seconds); the continuous-batching
+server runs them saturated (all arrive at once) and under Poisson arrivals,
+reporting p50/p99 job latency and aggregate primes/s against the same jobs
+plain back-to-back; per-job hit sets must equal the plain runs.
 """
 import argparse
 import json
@@ -53,6 +65,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-chunks", type=int, default=4)
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="Poisson arrival rate as a fraction of the plain job rate")
+    ap.add_argument("--serve-active", type=int, default=16)
+    ap.add_argument("--serve-budget-gib", type=float, default=4.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -181,6 +202,9 @@ def main():
                          f"same job and m, {dt:.2f}s"}
         log(f"cpu: {n} primes in {dt:.2f}s on {args.cpu_threads} threads")
 
+    batched = None if args.no_batch else bench_batched(args, rs, p0, chunk)
+    serving = None if args.no_serve else bench_serving(args, rs, p0, chunk)
+
     out = {
         "metric": "primes/sec sieved (22 k, n 180k-250k) plain vs batched-serve; p99 job latency",
         "value": prange_rate,
@@ -208,8 +232,169 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "autotune": tune,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def tune_jobs(rs, plans, p0, buf, sample):
+    ms, tiles = [], []
+    for pl in plans:
+        m, t, _ = rs.autotune(pl, p0, buf, count=min(buf.count, sample), effort=1)
+        ms.append(m)
+        tiles.append(t)
+    return ms, tiles
+
+
+def bench_batched(args, rs, p0, chunk):
+    """Heterogeneous jobs: plain back-to-back vs one graph-captured batch."""
+    import workloads
+
+    t = time.time()
+    specs = workloads.job_specs(args.batch_jobs, seed=4)
+    plans = [workloads.job_plan(sp) for sp in specs]
+    synth_s = time.time() - t
+    gpg = rs.GpuPrimeGen(p0 + chunk + 1)
+    buf = gpg.device_range(p0, p0 + chunk)
+    t = time.time()
+    ms, tiles = tune_jobs(rs, plans, p0, buf, 1 << 21)
+    tune_s = time.time() - t
+    log(f"batched: {len(plans)} jobs synthesized in {synth_s:.1f}s, tuned in {tune_s:.1f}s")
+    svs = [rs.Sieve(pl, m=m, tile=tl, max_batch=1 << 10, hit_capacity=1 << 20)
+           for pl, m, tl in zip(plans, ms, tiles)]
+    plain_hits = []
+    t0 = time.perf_counter()
+    for sv in svs:
+        sv.submit_device(p0, buf)
+        plain_hits.append(sv.collect())
+    plain_s = time.perf_counter() - t0
+    for sv in svs:
+        sv.close()
+    log(f"batched: plain back-to-back {plain_s:.3f}s")
+    bat = rs.Batch(plans, ms, hit_capacity=1 << 22, tiles=tiles)
+    for j in range(len(plans)):
+        bat.add_chunk(j, p0, buf)
+    t0 = time.perf_counter()
+    bat.run()  # captures the graph (one launch per occupancy class), then launches it
+    hits = bat.collect()
+    batched_s = time.perf_counter() - t0
+    log(f"batched: one graph {batched_s:.3f}s")
+    per_job = [[] for _ in plans]
+    for (j, p, k, n) in hits:
+        per_job[j].append((p, k, n))
+    if per_job != plain_hits:
+        raise RuntimeError("batched hits differ from the plain runs")
+    bat.close()
+    primes = buf.count * len(plans)
+    buf.free()
+    return {"workload": f"{len(plans)} heterogeneous jobs (base U[3,1030], k-count log-U[1,200], "
+                        "n-width log-U[1e4,5e5] from n=1e5, presieve < 1e5; seed 4), each over "
+                        "the p-chunk [1e12, 1e12+2^30)",
+            "plain_s": plain_s, "batched_s": batched_s,
+            "plain_primes_per_sec": primes / plain_s,
+            "batched_primes_per_sec": primes / batched_s,
+            "batched_vs_plain": plain_s / batched_s,
+            "factors": len(hits), "hits_equal_plain": True,
+            "ms": ms, "tiles": tiles, "tune_s": tune_s}
+
+
+def serve_run(rs, plans, ms, tiles, p0, p_hi, chunk, arrivals, args):
+    """Runs the jobs through the server; arrivals are times in seconds from
+    the start.  Returns per-job latency (s), results, makespan and primes."""
+    import numpy as np
+
+    srv = rs.Server(hbm_budget=int(args.serve_budget_gib * 2**30), max_active=args.serve_active,
+                    chunk_width=chunk, lds_cap=65536, hit_capacity=1 << 22)
+    n = len(plans)
+    ids, done_at = [None] * n, [None] * n
+    nxt, max_active = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[nxt] <= now:
+            ids[nxt] = srv.submit(plans[nxt], p0, p_hi, m=ms[nxt], tile=tiles[nxt])
+            nxt += 1
+        active, queued, _ = srv.step()
+        max_active = max(max_active, active)
+        now = time.perf_counter() - t0
+        for i in range(nxt):
+            if done_at[i] is None and srv.poll(ids[i])[0] == srv.DONE:
+                done_at[i] = now
+        if nxt == n and all(d is not None for d in done_at):
+            break
+        if active == 0 and queued == 0 and nxt < n:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+    results = [srv.results(j) for j in ids]
+    primes = srv.stats()["primes"]
+    srv.close()
+    lat = np.array([d - a for d, a in zip(done_at, arrivals)])
+    makespan = max(done_at) - min(arrivals)
+    return lat, results, makespan, primes, max_active
+
+
+def bench_serving(args, rs, p0, chunk):
+    """Continuous batching vs the same jobs plain back-to-back."""
+    import numpy as np
+
+    import workloads
+
+    specs = workloads.job_specs(args.serve_jobs, seed=6)
+    plans = [workloads.job_plan(sp) for sp in specs]
+    p_hi = p0 + args.serve_chunks * chunk
+    gpg = rs.GpuPrimeGen(p0 + chunk + 1)
+    buf = gpg.device_range(p0, p0 + chunk)
+    ms, tiles = tune_jobs(rs, plans, p0, buf, 1 << 21)
+    buf.free()
+    # plain back-to-back: one job at a time, device-generated primes per chunk
+    plain_hits, plain_job_s = [], []
+    t0 = time.perf_counter()
+    for pl, m, tl in zip(plans, ms, tiles):
+        ts = time.perf_counter()
+        sv = rs.Sieve(pl, m=m, tile=tl, max_batch=1 << 10, hit_capacity=1 << 20)
+        for c in range(args.serve_chunks):
+            sv.submit_range(p0 + c * chunk, p0 + (c + 1) * chunk)
+        plain_hits.append(sv.collect())
+        primes_job = sv.timing()["primes"]
+        sv.close()
+        plain_job_s.append(time.perf_counter() - ts)
+    plain_s = time.perf_counter() - t0
+    total_primes = primes_job * len(plans)
+    log(f"serving: {len(plans)} jobs x {args.serve_chunks} chunks plain back-to-back {plain_s:.2f}s")
+    out = {"workload": f"{len(plans)} jobs of the batched distribution (seed 6), each over "
+                       f"[1e12, 1e12+{args.serve_chunks}*2^30) in 2^30 chunks; HBM budget "
+                       f"{args.serve_budget_gib} GiB, at most {args.serve_active} active",
+           "plain_s": plain_s, "plain_primes_per_sec": total_primes / plain_s,
+           "plain_job_s_p50": float(np.percentile(plain_job_s, 50)),
+           "plain_job_s_p99": float(np.percentile(plain_job_s, 99))}
+    # saturated: every job arrives at t = 0
+    lat, res, span, primes, act = serve_run(rs, plans, ms, tiles, p0, p_hi, chunk,
+                                            [0.0] * len(plans), args)
+    if res != plain_hits or primes != total_primes:
+        raise RuntimeError("served hits differ from the plain runs")
+    out["saturated"] = {"aggregate_primes_per_sec": primes / span, "makespan_s": span,
+                        "latency_p50_s": float(np.percentile(lat, 50)),
+                        "latency_p99_s": float(np.percentile(lat, 99)),
+                        "max_active": act,
+                        "vs_plain": plain_s / span}
+    log(f"serving: saturated makespan {span:.2f}s p99 {np.percentile(lat, 99):.2f}s")
+    # Poisson arrivals at a fraction of the plain job rate
+    rate = args.serve_load * len(plans) / plain_s
+    arr = workloads.poisson_arrivals(len(plans), rate, seed=7)
+    lat, res, span, primes, act = serve_run(rs, plans, ms, tiles, p0, p_hi, chunk, arr, args)
+    if res != plain_hits:
+        raise RuntimeError("served hits differ from the plain runs")
+    out["poisson"] = {"rate_jobs_per_s": rate, "load": args.serve_load,
+                      "aggregate_primes_per_sec": primes / span, "makespan_s": span,
+                      "latency_p50_s": float(np.percentile(lat, 50)),
+                      "latency_p99_s": float(np.percentile(lat, 99)),
+                      "max_active": act}
+    log(f"serving: poisson rate {rate:.2f}/s p50 {np.percentile(lat, 50):.2f}s "
+        f"p99 {np.percentile(lat, 99):.2f}s")
+    out["hits_equal_plain"] = True
+    out["factors"] = sum(len(h) for h in plain_hits)
+    out["ms"], out["tiles"] = ms, tiles
+    return out
 
 
 if __name__ == "__main__":

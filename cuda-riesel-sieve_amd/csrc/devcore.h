@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <vector>
 
 #include "../host/host.h"
@@ -44,6 +45,23 @@ struct Launcher {
     uint32_t hit_cap = 0;
 };
 
+// Launch resources shared by jobs of different LDS layouts: one persistent
+// launch per occupancy class (jobs whose layouts give the same grid shape),
+// each sized for its largest member, all sharing one scratch, work counter
+// and hit buffer (the launches run in order on one stream).
+struct MultiLauncher {
+    Launcher base;                // owns the buffers
+    int w_cap = 0;                // largest record (words) the scratch holds
+    size_t max_waves = 0;         // resident waves of the smallest layout
+    std::map<int, Launcher> geo;  // per-wave LDS words -> grid shape
+};
+
+struct ClassRange {
+    Launcher L;                   // class grid shape + shared buffers
+    int chunk0 = 0, nchunks = 0;  // slice of the class-ordered chunk table
+    long long ngroups = 0;
+};
+
 struct Timing {
     hipEvent_t a, b;
     long long primes;
@@ -59,6 +77,18 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj);
 size_t job_device_bytes(const rs_plan* pl, bool dat);
 void free_launcher(Launcher& L);
 rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity);
+// Grid shape for a per-wave LDS layout (no allocation).
+rs_status launch_geometry(int device, int lds_words, Launcher* L);
+rs_status init_multi(MultiLauncher* M, int device, int w_cap, int64_t hit_capacity);
+// Orders `chunks` (each with .job indexing `jobs`) by occupancy class,
+// numbering groups from 0 within each class, and returns one range per
+// non-empty class.
+rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
+                       std::vector<ChunkDev>* chunks, std::vector<ClassRange>* out);
+// Enqueues one persistent launch per class; d_chunks holds the ordered table.
+rs_status launch_classes(const std::vector<ClassRange>& cls, hipStream_t stream,
+                         const JobDev* jobs, const ChunkDev* d_chunks);
+void free_multi(MultiLauncher& M);
 rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const ChunkDev& c0,
                  const JobDev* jobs, const ChunkDev* chunks, int nchunks, long long ngroups);
 rs_status drain_hits(const Launcher& L, hipStream_t stream, std::vector<HitDev>* out,

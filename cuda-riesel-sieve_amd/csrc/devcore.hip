@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
+#include <map>
 #include <vector>
 
 #include "../host/host.h"
@@ -121,28 +123,31 @@ void free_launcher(Launcher& L) {
 
 // Sizes the grid for lds_words per wave and record size W (both maxima over
 // the jobs of the launch) and allocates scratch, counter and hits.
-rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity) {
+rs_status launch_geometry(int device, int lds_words, Launcher* L) {
     L->device = device;
     int max_lds = 0, cus = 0;
     HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     const size_t per_wave = (size_t)lds_words * 4;
-    if (per_wave > (size_t)max_lds) return RS_ERR_ARG;
+    if (lds_words < 1 || per_wave > (size_t)max_lds) return RS_ERR_ARG;
     // waves of a block are independent (one table each); up to 4 per block
     L->wpb = (int)std::min<size_t>(kMaxWavesPerBlock, (size_t)max_lds / per_wave);
     L->lds_words = lds_words;
     L->lds_bytes = per_wave * L->wpb;
+    // the attribute is per function: always the device limit, so launches
+    // of different LDS sizes never restrict one another
     if (L->lds_bytes > 65536)
         HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(rs_sieve_kernel),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)L->lds_bytes));
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
     int per_cu = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, reinterpret_cast<const void*>(rs_sieve_kernel), 64 * L->wpb, L->lds_bytes));
     L->grid = std::max(1, cus * std::max(1, per_cu));
-    L->scratch_words = 64 * W;
-    const size_t waves = (size_t)L->grid * L->wpb;
-    if (hipMalloc(&L->scratch, waves * L->scratch_words * 8) != hipSuccess) return RS_ERR_NOMEM;
+    return RS_OK;
+}
+
+static rs_status alloc_buffers(Launcher* L, size_t scratch_words_total, int64_t hit_capacity) {
+    if (hipMalloc(&L->scratch, scratch_words_total * 8) != hipSuccess) return RS_ERR_NOMEM;
     HIPCHK(hipMalloc(&L->work, 8));
     L->hit_cap = (uint32_t)hit_capacity;
     if (hipMalloc(&L->hits, (size_t)hit_capacity * sizeof(HitDev)) != hipSuccess)
@@ -152,8 +157,100 @@ rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t h
     return RS_OK;
 }
 
-// Zeroes the work counter and launches the persistent kernel on `stream`.
-// Kernel-only (no memset nodes), so it can be captured in a graph.
+rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity) {
+    rs_status st = launch_geometry(device, lds_words, L);
+    if (st != RS_OK) return st;
+    L->scratch_words = 64 * W;
+    return alloc_buffers(L, (size_t)L->grid * L->wpb * L->scratch_words, hit_capacity);
+}
+
+rs_status init_multi(MultiLauncher* M, int device, int w_cap, int64_t hit_capacity) {
+    M->w_cap = w_cap;
+    // the smallest layout gives the most resident waves: size scratch for it
+    Launcher g;
+    rs_status st = launch_geometry(device, 64, &g);
+    if (st != RS_OK) return st;
+    M->base.device = device;
+    M->max_waves = (size_t)g.grid * g.wpb;
+    return alloc_buffers(&M->base, M->max_waves * 64 * (size_t)w_cap, hit_capacity);
+}
+
+rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
+                       std::vector<ChunkDev>* chunks, std::vector<ClassRange>* out) {
+    out->clear();
+    // occupancy key of each job's own layout
+    std::vector<std::pair<long long, int>> key(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        auto it = M->geo.find(jobs[j].lds_words);
+        if (it == M->geo.end()) {
+            Launcher g;
+            rs_status st = launch_geometry(M->base.device, jobs[j].lds_words, &g);
+            if (st != RS_OK) return st;
+            it = M->geo.emplace(jobs[j].lds_words, g).first;
+        }
+        if (jobs[j].W > M->w_cap) return RS_ERR_ARG;
+        key[j] = {(long long)it->second.grid * kMaxWavesPerBlock + it->second.wpb, 0};
+    }
+    // classes in order of decreasing occupancy; a class takes the largest
+    // layout and record of its members (same grid shape by construction)
+    std::vector<long long> keys;
+    for (auto& k : key) keys.push_back(k.first);
+    std::sort(keys.begin(), keys.end(), std::greater<long long>());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    std::vector<ChunkDev> sorted;
+    sorted.reserve(chunks->size());
+    for (long long k : keys) {
+        ClassRange cr{};
+        int lds = 0, W = 0;
+        for (size_t j = 0; j < jobs.size(); ++j)
+            if (key[j].first == k) {
+                lds = std::max(lds, jobs[j].lds_words);
+                W = std::max(W, jobs[j].W);
+            }
+        cr.chunk0 = (int)sorted.size();
+        long long g = 0;
+        for (const ChunkDev& c : *chunks) {
+            if (key[c.job].first != k || c.count == 0) continue;
+            ChunkDev d = c;
+            d.group0 = g;
+            g += (c.count + 63) / 64;
+            sorted.push_back(d);
+        }
+        cr.nchunks = (int)sorted.size() - cr.chunk0;
+        cr.ngroups = g;
+        if (cr.nchunks == 0) continue;
+        cr.L = M->geo.at(lds);
+        cr.L.scratch_words = 64 * W;
+        cr.L.scratch = M->base.scratch;
+        cr.L.work = M->base.work;
+        cr.L.hits = M->base.hits;
+        cr.L.hit_count = M->base.hit_count;
+        cr.L.hit_cap = M->base.hit_cap;
+        if ((size_t)cr.L.grid * cr.L.wpb * cr.L.scratch_words > M->max_waves * 64 * (size_t)M->w_cap)
+            return RS_ERR_ARG;
+        out->push_back(cr);
+    }
+    *chunks = std::move(sorted);
+    return RS_OK;
+}
+
+rs_status launch_classes(const std::vector<ClassRange>& cls, hipStream_t stream,
+                         const JobDev* jobs, const ChunkDev* d_chunks) {
+    for (const ClassRange& c : cls) {
+        JobDev j0{};
+        ChunkDev c0{};
+        rs_status st = launch(c.L, stream, j0, c0, jobs, d_chunks + c.chunk0, c.nchunks,
+                              c.ngroups);
+        if (st != RS_OK) return st;
+    }
+    return RS_OK;
+}
+
+void free_multi(MultiLauncher& M) {
+    free_launcher(M.base);
+    M.geo.clear();
+}
+
 rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const ChunkDev& c0,
                  const JobDev* jobs, const ChunkDev* chunks, int nchunks, long long ngroups) {
     hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)L.work, 2);

@@ -311,58 +311,129 @@ extern "C" void rs_sieve_destroy(rs_sieve* s) { free_sieve(s); }
 
 // ------------------------------------------------------------ autotuner
 
-extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
-                                 const uint32_t* offsets_dev, int64_t count, int* best_m,
-                                 int* best_tile, int* cand_m, int* cand_tile,
-                                 double* ns_per_prime, int cap, int* ncand) {
-    if (!pl || !offsets_dev || count < 1 || !best_m || !best_tile) return RS_ERR_ARG;
-    int max_lds = 0;
-    HIPCHK(hipSetDevice(device));
-    HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-    const int m0 = rs_plan_auto_m(pl);
-    std::vector<int> ms;
-    for (int num = 4; num <= 16; ++num) {  // m0 * num / 8, rounded to 64
-        const int m = std::max(64, (m0 * num / 8 + 32) / 64 * 64);
-        if (std::find(ms.begin(), ms.end(), m) == ms.end()) ms.push_back(m);
-    }
+namespace {
+
+struct Tuner {
+    int device;
+    const rs_plan* pl;
+    uint64_t low;
+    const uint32_t* offs;
+    int64_t count;
+    int max_lds;
+    bool warm = false;
+    int n = 0, cap;
+    int *cand_m, *cand_tile;
+    double* ns_out;
     double best = 1e300;
-    int n = 0;
+    int best_m = 0, best_tile = 0;
+
+    // Mean kernel ns per prime of (m, tile) on the sample; RS_ERR_ARG when
+    // the layout does not fit one wavefront's LDS limit.
+    rs_status time(int m, int tile, double* ns) {
+        rs_sieve* s = nullptr;
+        rs_status st = rs_sieve_create(device, pl, m, tile, 1, 1 << 16, &s);
+        if (st != RS_OK) return st;
+        if (s->job.J.lds_words * 4 > max_lds) {
+            rs_sieve_destroy(s);
+            return RS_ERR_ARG;
+        }
+        double ms_time = 0;
+        int64_t primes = 0;
+        if (!warm) {  // first launch of the process pays code-object loading
+            st = rs_sieve_submit_device(s, low, offs, count);
+            if (st == RS_OK) st = rs_sieve_timing(s, nullptr, nullptr);
+            warm = true;
+        }
+        if (st == RS_OK) st = rs_sieve_submit_device(s, low, offs, count);
+        if (st == RS_OK) st = rs_sieve_timing(s, &ms_time, &primes);
+        rs_sieve_destroy(s);
+        if (st != RS_OK) return st;
+        *ns = ms_time * 1e6 / (double)count;
+        if (n < cap) {
+            if (cand_m) cand_m[n] = m;
+            if (cand_tile) cand_tile[n] = tile;
+            if (ns_out) ns_out[n] = *ns;
+        }
+        ++n;
+        if (*ns < best) {
+            best = *ns;
+            best_m = m;
+            best_tile = tile;
+        }
+        return RS_OK;
+    }
+};
+
+int round64(int m) { return std::max(64, (m + 32) / 64 * 64); }
+
+}  // namespace
+
+extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
+                                 const uint32_t* offsets_dev, int64_t count, int effort,
+                                 int* best_m, int* best_tile, int* cand_m, int* cand_tile,
+                                 double* ns_per_prime, int cap, int* ncand) {
+    if (!pl || !offsets_dev || count < 1 || !best_m || !best_tile || effort < 0 || effort > 1)
+        return RS_ERR_ARG;
+    HIPCHK(hipSetDevice(device));
+    Tuner tu{device, pl, low, offsets_dev, count, 0};
+    HIPCHK(hipDeviceGetAttribute(&tu.max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    tu.cap = cap;
+    tu.cand_m = cand_m;
+    tu.cand_tile = cand_tile;
+    tu.ns_out = ns_per_prime;
+    const int m0 = rs_plan_auto_m(pl);
     *best_m = 0;
     *best_tile = 0;
-    for (int m : ms) {
-        for (int tile : {1, 2, 4}) {
-            rs_sieve* s = nullptr;
-            rs_status st = rs_sieve_create(device, pl, m, tile, 1, 1 << 16, &s);
-            if (st == RS_ERR_ARG) break;  // m does not fit: larger m will not either
-            if (st != RS_OK) return st;
-            if (s->job.J.lds_words * 4 > max_lds) {
-                rs_sieve_destroy(s);
-                break;
+    double ns = 0;
+    if (effort == 0) {
+        // full grid: m0 * num / 8 for num = 4..16 (rounded to 64) x tiles
+        std::vector<int> ms;
+        for (int num = 4; num <= 16; ++num) {
+            const int m = round64(m0 * num / 8);
+            if (std::find(ms.begin(), ms.end(), m) == ms.end()) ms.push_back(m);
+        }
+        for (int m : ms) {
+            bool fits = true;
+            for (int tile : {1, 2, 4}) {
+                rs_status st = tu.time(m, tile, &ns);
+                if (st == RS_ERR_ARG) {  // larger m will not fit either
+                    fits = false;
+                    break;
+                }
+                if (st != RS_OK) return st;
             }
-            double ms_time = 0;
-            int64_t primes = 0;
-            // one untimed pass, then the timed one
-            st = rs_sieve_submit_device(s, low, offsets_dev, count);
-            if (st == RS_OK) st = rs_sieve_timing(s, nullptr, nullptr);
-            if (st == RS_OK) st = rs_sieve_submit_device(s, low, offsets_dev, count);
-            if (st == RS_OK) st = rs_sieve_timing(s, &ms_time, &primes);
-            rs_sieve_destroy(s);
-            if (st != RS_OK) return st;
-            const double ns = ms_time * 1e6 / (double)count;
-            if (n < cap) {
-                if (cand_m) cand_m[n] = m;
-                if (cand_tile) cand_tile[n] = tile;
-                if (ns_per_prime) ns_per_prime[n] = ns;
+            if (!fits) break;
+        }
+    } else {
+        // coordinate descent: the tile at m0 (or the largest m that fits),
+        // then m in steps of m0/4 up and down while the time improves
+        int m = m0;
+        rs_status st = RS_ERR_ARG;
+        while (m >= 64) {
+            st = tu.time(m, 2, &ns);
+            if (st != RS_ERR_ARG) break;
+            m = m > 64 ? round64(m * 3 / 4) : 0;
+        }
+        if (st != RS_OK) return st;
+        for (int tile : {1, 4})
+            if ((st = tu.time(m, tile, &ns)) != RS_OK && st != RS_ERR_ARG) return st;
+        const int tile = tu.best_tile;
+        const int step = std::max(64, round64(m / 4));
+        for (int dir : {+1, -1}) {
+            double prev = tu.best;
+            for (int mm = m + dir * step; mm >= 64; mm += dir * step) {
+                st = tu.time(mm, tile, &ns);
+                if (st == RS_ERR_ARG) break;
+                if (st != RS_OK) return st;
+                if (ns >= prev) break;
+                prev = ns;
             }
-            ++n;
-            if (ns < best) {
-                best = ns;
-                *best_m = m;
-                *best_tile = tile;
-            }
+            if (tu.best_m != m) break;  // improved going up: skip going down
         }
     }
-    if (ncand) *ncand = n;
+    if (ncand) *ncand = tu.n;
+    *best_m = tu.best_m;
+    *best_tile = tu.best_tile;
     return *best_m ? RS_OK : RS_ERR_ARG;
 }
 
@@ -372,12 +443,12 @@ struct rs_batch {
     int device = 0;
     std::vector<DevJob> jobs;
     JobDev* d_jobs = nullptr;
-    Launcher L;
+    MultiLauncher M;
+    std::vector<ClassRange> classes;
     hipStream_t stream = nullptr;
-    std::vector<ChunkDev> chunks;      // host copy, group0 filled at run
+    std::vector<ChunkDev> chunks;      // as added; ordered by class at capture
     ChunkDev* d_chunks = nullptr;
     size_t d_chunks_cap = 0;
-    long long ngroups = 0;
     bool dirty = true;                 // chunk table changed since capture
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -396,7 +467,7 @@ static void free_batch(rs_batch* b) {
     for (auto& j : b->jobs) free_job(j);
     (void)hipFree(b->d_jobs);
     (void)hipFree(b->d_chunks);
-    free_launcher(b->L);
+    free_multi(b->M);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->ev_a) (void)hipEventDestroy(b->ev_a);
@@ -416,15 +487,12 @@ extern "C" rs_status rs_batch_create(int device, int njobs, const rs_plan* const
     b->device = device;
     b->jobs.resize(njobs);
     rs_status st = RS_OK;
-    int lds_words = 0, W = 0;
+    int W = 0;
     for (int i = 0; i < njobs && st == RS_OK; ++i) {
         st = make_job(plans[i], ms ? ms[i] : 0, tiles ? tiles[i] : 0, false, &b->jobs[i]);
-        if (st == RS_OK) {
-            lds_words = std::max(lds_words, b->jobs[i].J.lds_words);
-            W = std::max(W, b->jobs[i].J.W);
-        }
+        if (st == RS_OK) W = std::max(W, b->jobs[i].J.W);
     }
-    if (st == RS_OK) st = init_launcher(&b->L, device, lds_words, W, hit_capacity);
+    if (st == RS_OK) st = init_multi(&b->M, device, W, hit_capacity);
     if (st == RS_OK) {
         std::vector<JobDev> host(njobs);
         for (int i = 0; i < njobs; ++i) host[i] = b->jobs[i].J;
@@ -476,22 +544,22 @@ extern "C" rs_status rs_batch_clear(rs_batch* b) {
     return RS_OK;
 }
 
-// Uploads the chunk table and captures zero + kernel into a graph.
+// Orders the chunk table by occupancy class, uploads it and captures one
+// zero + kernel pair per class into a graph.
 static rs_status batch_capture(rs_batch* b) {
-    long long g = 0;
-    for (auto& c : b->chunks) {
-        c.group0 = g;
-        g += (c.count + 63) / 64;
-    }
-    b->ngroups = g;
-    if (b->chunks.size() > b->d_chunks_cap) {
+    std::vector<JobDev> host(b->jobs.size());
+    for (size_t i = 0; i < b->jobs.size(); ++i) host[i] = b->jobs[i].J;
+    std::vector<ChunkDev> ordered = b->chunks;
+    rs_status st = plan_classes(&b->M, host, &ordered, &b->classes);
+    if (st != RS_OK) return st;
+    if (ordered.size() > b->d_chunks_cap) {
         (void)hipFree(b->d_chunks);
         b->d_chunks = nullptr;
-        b->d_chunks_cap = std::max<size_t>(64, b->chunks.size());
+        b->d_chunks_cap = std::max<size_t>(64, ordered.size());
         if (hipMalloc(&b->d_chunks, sizeof(ChunkDev) * b->d_chunks_cap) != hipSuccess)
             return RS_ERR_NOMEM;
     }
-    HIPCHK(hipMemcpy(b->d_chunks, b->chunks.data(), sizeof(ChunkDev) * b->chunks.size(),
+    HIPCHK(hipMemcpy(b->d_chunks, ordered.data(), sizeof(ChunkDev) * ordered.size(),
                      hipMemcpyHostToDevice));
     if (b->exec) {
         HIPCHK(hipGraphExecDestroy(b->exec));
@@ -502,9 +570,7 @@ static rs_status batch_capture(rs_batch* b) {
         b->graph = nullptr;
     }
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    ChunkDev c0{};
-    rs_status st = launch(b->L, b->stream, b->jobs[0].J, c0, b->d_jobs, b->d_chunks,
-                          (int)b->chunks.size(), b->ngroups);
+    st = launch_classes(b->classes, b->stream, b->d_jobs, b->d_chunks);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &graph);
     if (st != RS_OK) return st;
@@ -535,7 +601,7 @@ extern "C" rs_status rs_batch_run(rs_batch* b) {
 
 static rs_status batch_drain(rs_batch* b) {
     HIPCHK(hipSetDevice(b->device));
-    rs_status st = drain_hits(b->L, b->stream, &b->held, &b->lost);
+    rs_status st = drain_hits(b->M.base, b->stream, &b->held, &b->lost);
     if (st != RS_OK) return st;
     if (b->timed) {
         float ms = 0;

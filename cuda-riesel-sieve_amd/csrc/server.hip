@@ -65,7 +65,7 @@ struct rs_server {
     int max_active = 0;
     uint64_t width = 0;
     int lds_cap_words = 0, w_cap = 0;
-    Launcher L;
+    MultiLauncher M;
     hipStream_t stream = nullptr;
     rs_gpu_primegen* gpg = nullptr;
     std::vector<uint32_t*> bufs;        // one prime buffer per slot
@@ -91,7 +91,7 @@ static void free_server(rs_server* s) {
     if (s->h_counts) (void)hipHostFree(s->h_counts);
     (void)hipFree(s->d_jobs);
     (void)hipFree(s->d_chunks);
-    free_launcher(s->L);
+    free_multi(s->M);
     if (s->gpg) rs_gpu_primegen_free(s->gpg);
     if (s->ev_a) (void)hipEventDestroy(s->ev_a);
     if (s->ev_b) (void)hipEventDestroy(s->ev_b);
@@ -116,7 +116,9 @@ extern "C" rs_status rs_server_create(int device, int64_t hbm_budget, int max_ac
     s->lds_cap_words = lds_cap_bytes / 4;
     s->w_cap = 128;  // record words: 20 + Q + T + 2 ceil(K/64)
     s->buf_cap = gpu_primes_capacity(chunk_width);
-    rs_status st = init_launcher(&s->L, device, s->lds_cap_words, s->w_cap, hit_capacity);
+    Launcher g;
+    rs_status st = launch_geometry(device, s->lds_cap_words, &g);  // cap fits the device
+    if (st == RS_OK) st = init_multi(&s->M, device, s->w_cap, hit_capacity);
     if (st == RS_OK &&
         (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
          hipEventCreate(&s->ev_a) != hipSuccess || hipEventCreate(&s->ev_b) != hipSuccess ||
@@ -257,36 +259,38 @@ extern "C" rs_status rs_server_step(rs_server* s, int* active, int* queued,
         wins[k].count = s->h_counts[k];
         if ((int64_t)wins[k].count > s->buf_cap) return RS_ERR_OVERFLOW;
     }
-    // 3. one persistent launch over every active job's window
+    // 3. every active job's window: one persistent launch per occupancy
+    //    class of the active jobs' LDS layouts
     std::vector<ChunkDev> chunks;
-    long long g = 0, step_primes = 0;
+    std::vector<JobDev> jhost;
+    long long step_primes = 0;
     for (size_t a = 0; a < s->active.size(); ++a) {
+        jhost.push_back(s->jobs.at(s->active[a]).dj.J);
         const Window& w = wins[win_of[a]];
         if (w.count == 0) continue;
         ChunkDev c{};
         c.low = w.lo;
         c.offs = w.buf;
         c.count = w.count;
-        c.group0 = g;
         c.job = (int)a;
-        g += (c.count + 63) / 64;
         step_primes += c.count;
         chunks.push_back(c);
     }
     if (!chunks.empty()) {
+        std::vector<ClassRange> cls;
+        rs_status st = plan_classes(&s->M, jhost, &chunks, &cls);
+        if (st != RS_OK) return st;
         HIPCHK(hipMemcpyAsync(s->d_chunks, chunks.data(), sizeof(ChunkDev) * chunks.size(),
                               hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipEventRecord(s->ev_a, s->stream));
-        ChunkDev c0{};
-        rs_status st = launch(s->L, s->stream, s->jobs.at(s->active[0]).dj.J, c0, s->d_jobs,
-                              s->d_chunks, (int)chunks.size(), g);
+        st = launch_classes(cls, s->stream, s->d_jobs, s->d_chunks);
         if (st != RS_OK) return st;
         HIPCHK(hipEventRecord(s->ev_b, s->stream));
     }
     // 4. hits to their jobs, progress, retirement
     std::vector<HitDev> hits;
     int64_t lost = 0;
-    rs_status st = drain_hits(s->L, s->stream, &hits, &lost);
+    rs_status st = drain_hits(s->M.base, s->stream, &hits, &lost);
     if (st != RS_OK) return st;
     if (lost) return RS_ERR_OVERFLOW;
     if (!chunks.empty()) {

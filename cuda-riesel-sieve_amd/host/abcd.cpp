@@ -201,11 +201,13 @@ extern "C" rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_
                                    uint32_t presieve_bound, uint64_t seed,
                                    rs_abcd** out) {
     if (!out || base < 2 || nk < 1 || kmin < 1 || kmax < kmin || nmin < 1 ||
-        nmax < nmin || kmax > 0x7fffffffu)
+        nmax < nmin || kmax > 0x7fffffffu || odd_k < 0 || odd_k > 2)
         return RS_ERR_ARG;
     *out = nullptr;
     const uint64_t span = (uint64_t)(kmax - kmin) + 1;
-    const uint64_t avail = odd_k ? (span + (kmin & 1)) / 2 : span;
+    // k of the requested parity in [kmin, kmax]: odd_k 1 odd, 2 even, 0 any
+    const uint32_t want_par = odd_k == 1 ? 1u : 0u;
+    const uint64_t avail = odd_k ? (span + ((kmin & 1) == want_par ? 1 : 0)) / 2 : span;
     if ((uint64_t)nk > avail) return RS_ERR_ARG;
     std::vector<uint32_t> qs;
     for (uint32_t q : rsh::primes_upto(presieve_bound ? presieve_bound - 1 : 0))
@@ -223,7 +225,7 @@ extern "C" rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_
         while ((int)(chosen.size() + batch.size()) < nk) {
             if (++draws > 64 * nk + 1024) return RS_ERR_ARG;  // cannot fill
             uint32_t k = kmin + (uint32_t)rng.below(span);
-            if (odd_k && (k & 1) == 0) { if (k + 1 > kmax) continue; k += 1; }
+            if (odd_k && (k & 1) != want_par) { if (k + 1 > kmax) continue; k += 1; }
             if (used.count(k)) continue;
             used.insert(k);
             batch.push_back(k);

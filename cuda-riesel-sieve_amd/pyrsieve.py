@@ -73,7 +73,7 @@ _SIGS = {
     "rs_sieve_sync": (i32, [vp]),
     "rs_sieve_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
     "rs_sieve_destroy": (None, [vp]),
-    "rs_autotune": (i32, [i32, vp, u64, vp, i64, P(i32), P(i32), P(i32), P(i32),
+    "rs_autotune": (i32, [i32, vp, u64, vp, i64, i32, P(i32), P(i32), P(i32), P(i32),
                           P(ctypes.c_double), i32, P(i32)]),
     "rs_batch_create": (i32, [i32, i32, P(vp), P(i32), P(i32), i64, P(vp)]),
     "rs_batch_job_info": (i32, [vp, i32, P(i32), P(i32), P(i32), P(i32)]),
@@ -173,6 +173,7 @@ class Abcd:
 
     @classmethod
     def synth(cls, base, nk, kmin, kmax, nmin, nmax, presieve_bound, seed, odd_k=True):
+        """odd_k: True/1 odd k only, 2 even k only, False/0 any parity."""
         h = vp()
         _chk("rs_abcd_synth", lib().rs_abcd_synth(base, nk, kmin, kmax, int(odd_k), nmin, nmax,
                                                   presieve_bound, seed, ctypes.byref(h)))
@@ -355,6 +356,16 @@ class GpuPrimeGen:
         self.h = h
         self.device = device
 
+    def device_range(self, lo, hi):
+        """Primes of [lo, hi) as offsets from lo in a DeviceBuffer (stays in HBM)."""
+        cap = int(2 * (hi - lo) / max(1.0, np.log(max(hi - lo, 3)))) + 64
+        buf = DeviceBuffer.empty(self.device, cap)
+        n = i64()
+        _chk("rs_gpu_primes_range", lib().rs_gpu_primes_range(self.h, lo, hi, buf.ptr, cap,
+                                                              ctypes.byref(n)))
+        buf.count = n.value
+        return buf
+
     def range(self, lo, hi):
         """Primes of [lo, hi) as a host uint32 array (through a device buffer)."""
         cap = int(2 * (hi - lo) / max(1.0, np.log(max(hi - lo, 3)))) + 64
@@ -400,6 +411,15 @@ class DeviceBuffer:
         if a.nbytes:
             _chk("rs_dev_upload", lib().rs_dev_upload(device, self.ptr, a.ctypes.data_as(vp),
                                                       a.nbytes))
+
+    @classmethod
+    def empty(cls, device, count):
+        """Uninitialised device buffer for `count` uint32 offsets."""
+        self = cls.__new__(cls)
+        self.device, self.nbytes, self.count = device, 4 * count, count
+        self.ptr = vp()
+        _chk("rs_dev_alloc", lib().rs_dev_alloc(device, max(4 * count, 4), ctypes.byref(self.ptr)))
+        return self
 
     def free(self):
         if self.ptr:
@@ -541,15 +561,17 @@ class Batch:
             self.close()
 
 
-def autotune(plan, low, dbuf, count=None, device=0):
-    """(best_m, best_tile, [(m, tile, ns_per_prime), ...]) measured on dbuf."""
+def autotune(plan, low, dbuf, count=None, device=0, effort=0):
+    """(best_m, best_tile, [(m, tile, ns_per_prime), ...]) measured on dbuf.
+    effort 0: full (m, tile) grid; 1: coordinate descent."""
     count = dbuf.count if count is None else count
     bm, bt, n = i32(), i32(), i32()
     cap = 256
     cm = np.zeros(cap, np.int32)
     ct = np.zeros(cap, np.int32)
     ns = np.zeros(cap, np.float64)
-    _chk("rs_autotune", lib().rs_autotune(device, plan.h, low, dbuf.ptr, count, ctypes.byref(bm),
+    _chk("rs_autotune", lib().rs_autotune(device, plan.h, low, dbuf.ptr, count, effort,
+                                          ctypes.byref(bm),
                                           ctypes.byref(bt), _p(cm, i32), _p(ct, i32),
                                           _p(ns, ctypes.c_double), cap, ctypes.byref(n)))
     k = min(n.value, cap)

@@ -76,7 +76,8 @@ void rs_abcd_free(rs_abcd* a);
 
 /* Synthetic input (no reference counterpart: the reference only reads ABCD
  * made by srsieve, sr_745.abcd:1).  Draws nk distinct k uniformly from
- * [kmin, kmax] (odd only if odd_k) with a seeded splitmix64 stream, keeps
+ * [kmin, kmax] (odd_k: 0 any parity, 1 odd only, 2 even only, as Riesel
+ * k*b^n-1 with odd b needs even k) with a seeded splitmix64 stream, keeps
  * n in [nmin, nmax] such that no odd prime q < presieve_bound with q ∤ b
  * divides k*b^n-1, and redraws a k whose list comes out empty. */
 rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_t kmax,
@@ -197,12 +198,14 @@ void rs_sieve_destroy(rs_sieve* s);
 /* Autotuner (no reference counterpart: the reference takes m from -m).
  * Times candidate (m, tile) pairs for a plan on `count` primes
  * p = low + offsets_dev[i] (device memory) and returns the fastest.
- * Candidates: multiples of 64 from 1/2 to 2 times the cost model's m that
- * fit one wavefront's LDS limit, each with tiles 1, 2 and 4.  cand_m,
+ * effort 0 times the full grid: multiples of 64 from 1/2 to 2 times the
+ * cost model's m that fit one wavefront's LDS limit, each with tiles 1, 2
+ * and 4.  effort 1 is a coordinate descent (tile at the model's m, then m
+ * in steps of m/4 while the time improves), typically 5-8 timings.  cand_m,
  * cand_tile, ns_per_prime (optional, capacity cap) receive the measured
  * candidates, *ncand their number. */
 rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
-                      const uint32_t* offsets_dev, int64_t count, int* best_m,
+                      const uint32_t* offsets_dev, int64_t count, int effort, int* best_m,
                       int* best_tile, int* cand_m, int* cand_tile,
                       double* ns_per_prime, int cap, int* ncand);
 
