@@ -72,6 +72,8 @@ def main():
                     help="Poisson arrival rate as a fraction of the plain job rate")
     ap.add_argument("--serve-active", type=int, default=16)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
+    ap.add_argument("--serve-quantum-ms", type=float, default=40.0,
+                    help="device time per job per server step (0: whole 2^30 windows)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
@@ -239,12 +241,15 @@ def main():
 
 
 def tune_jobs(rs, plans, p0, buf, sample):
-    ms, tiles = [], []
+    """Per-job (m, tile) by the autotuner's coordinate descent, and the
+    measured kernel ns per prime of the choice."""
+    ms, tiles, ns = [], [], []
     for pl in plans:
-        m, t, _ = rs.autotune(pl, p0, buf, count=min(buf.count, sample), effort=1)
+        m, t, cands = rs.autotune(pl, p0, buf, count=min(buf.count, sample), effort=1)
         ms.append(m)
         tiles.append(t)
-    return ms, tiles
+        ns.append(min(c[2] for c in cands if c[0] == m and c[1] == t))
+    return ms, tiles, ns
 
 
 def bench_batched(args, rs, p0, chunk):
@@ -258,7 +263,7 @@ def bench_batched(args, rs, p0, chunk):
     gpg = rs.GpuPrimeGen(p0 + chunk + 1)
     buf = gpg.device_range(p0, p0 + chunk)
     t = time.time()
-    ms, tiles = tune_jobs(rs, plans, p0, buf, 1 << 21)
+    ms, tiles, _ = tune_jobs(rs, plans, p0, buf, 1 << 21)
     tune_s = time.time() - t
     log(f"batched: {len(plans)} jobs synthesized in {synth_s:.1f}s, tuned in {tune_s:.1f}s")
     svs = [rs.Sieve(pl, m=m, tile=tl, max_batch=1 << 10, hit_capacity=1 << 20)
@@ -299,13 +304,14 @@ def bench_batched(args, rs, p0, chunk):
             "ms": ms, "tiles": tiles, "tune_s": tune_s}
 
 
-def serve_run(rs, plans, ms, tiles, p0, p_hi, chunk, arrivals, args):
+def serve_run(rs, plans, ms, tiles, ns, p0, p_hi, chunk, arrivals, args):
     """Runs the jobs through the server; arrivals are times in seconds from
     the start.  Returns per-job latency (s), results, makespan and primes."""
     import numpy as np
 
     srv = rs.Server(hbm_budget=int(args.serve_budget_gib * 2**30), max_active=args.serve_active,
-                    chunk_width=chunk, lds_cap=65536, hit_capacity=1 << 22)
+                    chunk_width=chunk, lds_cap=65536, hit_capacity=1 << 22,
+                    quantum_ms=args.serve_quantum_ms)
     n = len(plans)
     ids, done_at = [None] * n, [None] * n
     nxt, max_active = 0, 0
@@ -313,7 +319,8 @@ def serve_run(rs, plans, ms, tiles, p0, p_hi, chunk, arrivals, args):
     while True:
         now = time.perf_counter() - t0
         while nxt < n and arrivals[nxt] <= now:
-            ids[nxt] = srv.submit(plans[nxt], p0, p_hi, m=ms[nxt], tile=tiles[nxt])
+            ids[nxt] = srv.submit(plans[nxt], p0, p_hi, m=ms[nxt], tile=tiles[nxt],
+                                  ns_per_prime=ns[nxt])
             nxt += 1
         active, queued, _ = srv.step()
         max_active = max(max_active, active)
@@ -344,7 +351,7 @@ def bench_serving(args, rs, p0, chunk):
     p_hi = p0 + args.serve_chunks * chunk
     gpg = rs.GpuPrimeGen(p0 + chunk + 1)
     buf = gpg.device_range(p0, p0 + chunk)
-    ms, tiles = tune_jobs(rs, plans, p0, buf, 1 << 21)
+    ms, tiles, ns = tune_jobs(rs, plans, p0, buf, 1 << 21)
     buf.free()
     # plain back-to-back: one job at a time, device-generated primes per chunk
     plain_hits, plain_job_s = [], []
@@ -363,12 +370,13 @@ def bench_serving(args, rs, p0, chunk):
     log(f"serving: {len(plans)} jobs x {args.serve_chunks} chunks plain back-to-back {plain_s:.2f}s")
     out = {"workload": f"{len(plans)} jobs of the batched distribution (seed 6), each over "
                        f"[1e12, 1e12+{args.serve_chunks}*2^30) in 2^30 chunks; HBM budget "
-                       f"{args.serve_budget_gib} GiB, at most {args.serve_active} active",
+                       f"{args.serve_budget_gib} GiB, at most {args.serve_active} active, "
+                       f"windows of ~{args.serve_quantum_ms} ms device time (at most 2^30)",
            "plain_s": plain_s, "plain_primes_per_sec": total_primes / plain_s,
            "plain_job_s_p50": float(np.percentile(plain_job_s, 50)),
            "plain_job_s_p99": float(np.percentile(plain_job_s, 99))}
     # saturated: every job arrives at t = 0
-    lat, res, span, primes, act = serve_run(rs, plans, ms, tiles, p0, p_hi, chunk,
+    lat, res, span, primes, act = serve_run(rs, plans, ms, tiles, ns, p0, p_hi, chunk,
                                             [0.0] * len(plans), args)
     if res != plain_hits or primes != total_primes:
         raise RuntimeError("served hits differ from the plain runs")
@@ -381,7 +389,7 @@ def bench_serving(args, rs, p0, chunk):
     # Poisson arrivals at a fraction of the plain job rate
     rate = args.serve_load * len(plans) / plain_s
     arr = workloads.poisson_arrivals(len(plans), rate, seed=7)
-    lat, res, span, primes, act = serve_run(rs, plans, ms, tiles, p0, p_hi, chunk, arr, args)
+    lat, res, span, primes, act = serve_run(rs, plans, ms, tiles, ns, p0, p_hi, chunk, arr, args)
     if res != plain_hits:
         raise RuntimeError("served hits differ from the plain runs")
     out["poisson"] = {"rate_jobs_per_s": rate, "load": args.serve_load,

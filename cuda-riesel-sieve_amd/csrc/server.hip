@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <deque>
 #include <map>
 #include <vector>
@@ -44,6 +45,7 @@ struct SrvJob {
     int m = 0, tile = 0;
     bool dat = false;
     uint64_t p_lo = 0, p_hi = 0, p_next = 0;
+    double ns_hint = 0, ns = 0;  // kernel ns per prime: caller's hint, estimate used
     int state = ST_QUEUED;
     DevJob dj;
     size_t footprint = 0;
@@ -64,6 +66,7 @@ struct rs_server {
     int64_t budget = 0, used = 0;
     int max_active = 0;
     uint64_t width = 0;
+    double quantum_ms = 0;
     int lds_cap_words = 0, w_cap = 0;
     MultiLauncher M;
     hipStream_t stream = nullptr;
@@ -101,10 +104,11 @@ static void free_server(rs_server* s) {
 
 extern "C" rs_status rs_server_create(int device, int64_t hbm_budget, int max_active,
                                       uint64_t chunk_width, int lds_cap_bytes,
-                                      int64_t hit_capacity, rs_server** out) {
+                                      int64_t hit_capacity, double quantum_ms,
+                                      rs_server** out) {
     if (!out || hbm_budget < 1 || max_active < 1 || max_active > 1024 || chunk_width < 2 ||
         chunk_width > (1ull << 32) || lds_cap_bytes < 4096 || hit_capacity < 1 ||
-        hit_capacity > 0x7fffffff)
+        hit_capacity > 0x7fffffff || !(quantum_ms >= 0) || quantum_ms > 1e6)
         return RS_ERR_ARG;
     *out = nullptr;
     if (hipSetDevice(device) != hipSuccess) return RS_ERR_HIP;
@@ -113,6 +117,7 @@ extern "C" rs_status rs_server_create(int device, int64_t hbm_budget, int max_ac
     s->budget = hbm_budget;
     s->max_active = max_active;
     s->width = chunk_width;
+    s->quantum_ms = quantum_ms;
     s->lds_cap_words = lds_cap_bytes / 4;
     s->w_cap = 128;  // record words: 20 + Q + T + 2 ceil(K/64)
     s->buf_cap = gpu_primes_capacity(chunk_width);
@@ -140,8 +145,10 @@ static size_t footprint(const rs_server* s, const rs_plan& pl, bool dat) {
 }
 
 extern "C" rs_status rs_server_submit(rs_server* s, const rs_plan* pl, int m, int tile, int dat,
-                                      uint64_t p_lo, uint64_t p_hi, int64_t* job_id) {
-    if (!s || !pl || !job_id || p_lo < 3 || p_hi <= p_lo || p_hi > (1ull << 63) - (1ull << 32))
+                                      uint64_t p_lo, uint64_t p_hi, double ns_per_prime,
+                                      int64_t* job_id) {
+    if (!s || !pl || !job_id || p_lo < 3 || p_hi <= p_lo || p_hi > (1ull << 63) - (1ull << 32) ||
+        !(ns_per_prime >= 0))
         return RS_ERR_ARG;
     SrvJob j;
     j.id = s->next_id++;
@@ -151,11 +158,32 @@ extern "C" rs_status rs_server_submit(rs_server* s, const rs_plan* pl, int m, in
     j.dat = dat != 0;
     j.p_lo = j.p_next = p_lo;
     j.p_hi = p_hi;
+    j.ns_hint = ns_per_prime;
     j.footprint = footprint(s, j.plan, j.dat);
     *job_id = j.id;
     s->queue.push_back(j.id);
     s->jobs.emplace(j.id, std::move(j));
     return RS_OK;
+}
+
+// Kernel ns per prime of a job on MI355X, fitted to rs_sieve_kernel timings
+// of 64 jobs of mixed shape (base 3..1030, 1..200 k, n-widths 1e4..5e5;
+// median error 8%): a fixed part, baby steps, the Q*T table of giant-step
+// starts and A*T giant steps with A = rows/2 Jacobi survivors.
+static double est_ns_per_prime(const rs_plan& pl, const JobDev& J) {
+    const double A = 0.5 * (double)pl.rows_k.size();
+    return 0.282 + 0.001445 * J.m + 0.000274 * (double)pl.Q * J.T + 0.002097 * A * J.T;
+}
+
+// The job's next window: chunk_width, or about quantum_ms of kernel time.
+static uint64_t window_width(const rs_server* s, const SrvJob& j) {
+    if (s->quantum_ms <= 0) return s->width;
+    // primes per unit of p near p_next ~ 1/ln p
+    const double lnp = std::log((double)j.p_next + 16.0);
+    const double w = s->quantum_ms * 1e6 / j.ns * lnp;
+    uint64_t width = w >= (double)s->width ? s->width : (uint64_t)w;
+    width = std::max<uint64_t>(width & ~0xffffull, std::min<uint64_t>(1ull << 20, s->width));
+    return width;
 }
 
 // Builds the job's device state with the largest m (from the requested or
@@ -168,6 +196,7 @@ static rs_status admit(rs_server* s, SrvJob& j) {
         if (st == RS_OK && dj.J.lds_words <= s->lds_cap_words && dj.J.W <= s->w_cap) {
             j.dj = dj;
             j.m = m;
+            j.ns = j.ns_hint > 0 ? j.ns_hint : est_ns_per_prime(j.plan, dj.J);
             return RS_OK;
         }
         free_job(dj);
@@ -231,7 +260,7 @@ extern "C" rs_status rs_server_step(rs_server* s, int* active, int* queued,
     std::vector<int> win_of(s->active.size());
     for (size_t a = 0; a < s->active.size(); ++a) {
         const SrvJob& j = s->jobs.at(s->active[a]);
-        const uint64_t lo = j.p_next, hi = std::min(j.p_next + s->width, j.p_hi);
+        const uint64_t lo = j.p_next, hi = std::min(j.p_next + window_width(s, j), j.p_hi);
         int w = -1;
         for (size_t k = 0; k < wins.size(); ++k)
             if (wins[k].lo == lo && wins[k].hi == hi) w = (int)k;

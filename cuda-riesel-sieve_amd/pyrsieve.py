@@ -83,8 +83,8 @@ _SIGS = {
     "rs_batch_collect": (i32, [vp, P(JobFactor), i64, P(i64), P(i64)]),
     "rs_batch_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
     "rs_batch_destroy": (None, [vp]),
-    "rs_server_create": (i32, [i32, i64, i32, u64, i32, i64, P(vp)]),
-    "rs_server_submit": (i32, [vp, vp, i32, i32, i32, u64, u64, P(i64)]),
+    "rs_server_create": (i32, [i32, i64, i32, u64, i32, i64, ctypes.c_double, P(vp)]),
+    "rs_server_submit": (i32, [vp, vp, i32, i32, i32, u64, u64, ctypes.c_double, P(i64)]),
     "rs_server_step": (i32, [vp, P(i32), P(i32), P(i64)]),
     "rs_server_poll": (i32, [vp, i64, P(i32), P(u64), P(i64)]),
     "rs_server_results": (i32, [vp, i64, P(Factor), i64, P(i64)]),
@@ -287,17 +287,17 @@ class Server:
     h = vp()
 
     def __init__(self, hbm_budget, max_active=8, chunk_width=1 << 30, lds_cap=16384,
-                 hit_capacity=1 << 20, device=0):
+                 hit_capacity=1 << 20, device=0, quantum_ms=0.0):
         h = vp()
         _chk("rs_server_create", lib().rs_server_create(device, hbm_budget, max_active,
                                                         chunk_width, lds_cap, hit_capacity,
-                                                        ctypes.byref(h)))
+                                                        quantum_ms, ctypes.byref(h)))
         self.h = h
 
-    def submit(self, plan, p_lo, p_hi, m=0, tile=0, dat=False):
+    def submit(self, plan, p_lo, p_hi, m=0, tile=0, dat=False, ns_per_prime=0.0):
         jid = i64()
         _chk("rs_server_submit", lib().rs_server_submit(self.h, plan.h, m, tile, int(dat), p_lo,
-                                                        p_hi, ctypes.byref(jid)))
+                                                        p_hi, ns_per_prime, ctypes.byref(jid)))
         return jid.value
 
     def step(self):

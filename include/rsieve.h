@@ -254,21 +254,31 @@ void rs_batch_destroy(rs_batch* b);
  * while a slot is free and their device footprint (row tables, n-list
  * bitmaps with dat, one prime buffer of chunk_width) fits hbm_budget,
  * generates on the GPU the primes of every distinct p-window of the active
- * jobs (jobs at equal progress share one), sieves all active jobs in one
- * persistent launch, moves hits to their jobs and retires finished jobs.
- * Job state stays resident on the device between steps.  lds_cap_bytes
- * bounds one wavefront's LDS (a job's m is lowered to fit).  A job larger
- * than the whole budget is rejected (state -1). */
+ * jobs (jobs at equal progress share one), sieves all active jobs (one
+ * persistent launch per occupancy class of their LDS layouts), moves hits
+ * to their jobs and retires finished jobs.  Job state stays resident on
+ * the device between steps.  lds_cap_bytes bounds one wavefront's LDS (a
+ * job's m is lowered to fit).  A job larger than the whole budget is
+ * rejected (state -1).
+ *
+ * Windows: with quantum_ms = 0 every job advances chunk_width per step.
+ * With quantum_ms > 0 each job's window is sized to about quantum_ms of
+ * kernel time from its cost estimate (ns per prime: the submit hint, else
+ * a cost model of m, Q*T and rows*T), at most chunk_width and at least
+ * 2^20, so cheap and costly jobs get equal device time per step (processor
+ * sharing) and short jobs finish early. */
 typedef struct rs_server rs_server;
 
 rs_status rs_server_create(int device, int64_t hbm_budget, int max_active,
                            uint64_t chunk_width, int lds_cap_bytes,
-                           int64_t hit_capacity, rs_server** out);
+                           int64_t hit_capacity, double quantum_ms,
+                           rs_server** out);
 /* Queues a job: the plan is copied; m/tile as for rs_sieve_create; dat != 0
- * keeps only n in the job's lists; primes p in [p_lo, p_hi). */
+ * keeps only n in the job's lists; primes p in [p_lo, p_hi).  ns_per_prime
+ * is the job's measured kernel cost (rs_autotune), 0 if unknown. */
 rs_status rs_server_submit(rs_server* s, const rs_plan* pl, int m, int tile,
                            int dat, uint64_t p_lo, uint64_t p_hi,
-                           int64_t* job_id);
+                           double ns_per_prime, int64_t* job_id);
 /* One scheduling step (synchronous).  Reports active and queued jobs after
  * the step and the primes sieved in it. */
 rs_status rs_server_step(rs_server* s, int* active, int* queued,

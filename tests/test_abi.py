@@ -33,6 +33,24 @@ def test_binding_covers_every_declared_symbol(rs):
     assert sorted(rs._SIGS) == declared_functions()
 
 
+def declared_arities():
+    """{function: number of parameters} from the header's prototypes."""
+    text = open(os.path.join(ROOT, "include", "rsieve.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b(rs_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_binding_arity_matches_header(rs):
+    # a ctypes argtypes list of the wrong length would pass garbage silently
+    ar = declared_arities()
+    bad = {n: (len(rs._SIGS[n][1]), ar[n]) for n in ar if len(rs._SIGS[n][1]) != ar[n]}
+    assert not bad, bad
+
+
 def test_no_device_is_not_fatal(rs):
     # device_count() must answer (0 here) without aborting the process
     assert rs.device_count() >= 0

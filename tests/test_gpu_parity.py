@@ -389,10 +389,12 @@ def test_cli_gpu_generated_primes(tmp_path, gpu):
     assert outs[0] == outs[1] and f"{KNOWN_P} | 18300*745^242670-1" in outs[1]
 
 
-def test_server_results_alone_and_in_the_mix(rs, orc, gpu):
+@pytest.mark.parametrize("quantum_ms", [0.0, 0.05])
+def test_server_results_alone_and_in_the_mix(rs, orc, gpu, quantum_ms):
     """Jobs arriving between steps are admitted as slots and budget free up;
     each job's hits equal the job sieved alone (and the oracle), datless and
-    with device-resident n-lists (dat)."""
+    with device-resident n-lists (dat); with a time quantum the windows of
+    each job are sized by its cost (2^20 up to 2^22 here)."""
     specs = [(745, 6, 10**6, 20000, 30000, 1000), (2, 9, 10**8, 100000, 140000, 2000),
              (5, 3, 10**5, 5000, 7000, 500), (1030, 12, 10**7, 40000, 90000, 3000),
              (3, 1, 10**4, 1000, 100000, 200), (31, 4, 10**6, 30000, 60000, 1000)]
@@ -405,7 +407,7 @@ def test_server_results_alone_and_in_the_mix(rs, orc, gpu):
     ranges[1] = ranges[0]  # two jobs on the same windows share their primes
     per_job = 4 * (2 * width / np.log(width) + 64)  # each job's prime buffer
     srv = rs.Server(hbm_budget=int(2.5 * per_job), max_active=3, chunk_width=width,
-                    lds_cap=16384)
+                    lds_cap=16384, quantum_ms=quantum_ms)
     ids, pending = [], list(range(len(plans)))
     steps = 0
     while pending or any(srv.poll(j)[0] in (0, 1) for j in ids):
@@ -414,7 +416,7 @@ def test_server_results_alone_and_in_the_mix(rs, orc, gpu):
             ids.append(srv.submit(plans[i], *ranges[i], dat=(i == 3)))
         srv.step()
         steps += 1
-        assert steps < 200
+        assert steps < 400
     for i, j in enumerate(ids):
         st, done, nh = srv.poll(j)
         assert st == srv.DONE and done == ranges[i][1] - ranges[i][0]
