@@ -11,9 +11,10 @@
 
 extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
                                            const ChunkDev* chunks, int nchunks,
-                                           long long ngroups, int lds_words, int scratch_words,
-                                           uint64_t* scratch, unsigned long long* work,
-                                           HitDev* hits, uint32_t* hit_count, uint32_t hit_cap);
+                                           long long ngroups, int lds_words, int shared_words,
+                                           int scratch_words, uint64_t* scratch,
+                                           unsigned long long* work, HitDev* hits,
+                                           uint32_t* hit_count, uint32_t hit_cap);
 extern "C" __global__ void rs_zero_u32(uint32_t*, int);
 
 namespace rsdev {
@@ -23,7 +24,7 @@ const int kMaxWavesPerBlock = 4;
 // Job constants and device row tables built from a plan.
 struct DevJob {
     JobDev J{};
-    uint32_t* d_kvals = nullptr;
+    uint4* d_kvals = nullptr;    // Jacobi constants of each distinct k
     uint32_t* d_rowinfo = nullptr;
     uint32_t* d_rowk = nullptr;
     uint32_t* d_bitmap = nullptr;
@@ -35,7 +36,8 @@ struct Launcher {
     int device = 0;
     int grid = 0;
     int wpb = 1;
-    int lds_words = 0;
+    int lds_words = 0;       // per wavefront
+    int shared_words = 0;    // block-shared words before the wavefront regions
     size_t lds_bytes = 0;
     int scratch_words = 0;  // per wavefront: 64 records
     uint64_t* scratch = nullptr;
@@ -68,6 +70,7 @@ struct Timing {
 };
 
 uint32_t magic_div(int d);
+uint4 jacobi_consts(uint32_t a);
 void free_job(DevJob& j);
 // Job constants from a plan, table size m (0: cost model), giant-step tile
 // (1, 2 or 4; 0: 2); with dat, the plan's n-list bitmaps go to the device
@@ -76,9 +79,11 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj);
 // Device bytes make_job allocates for a plan (row tables, bitmaps if dat).
 size_t job_device_bytes(const rs_plan* pl, bool dat);
 void free_launcher(Launcher& L);
-rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity);
+// shared_words: block-shared LDS (the row table of a single-job launch).
+rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity,
+                        int shared_words = 0);
 // Grid shape for a per-wave LDS layout (no allocation).
-rs_status launch_geometry(int device, int lds_words, Launcher* L);
+rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words = 0);
 rs_status init_multi(MultiLauncher* M, int device, int w_cap, int64_t hit_capacity);
 // Orders `chunks` (each with .job indexing `jobs`) by occupancy class,
 // numbering groups from 0 within each class, and returns one range per

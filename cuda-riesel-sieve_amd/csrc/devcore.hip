@@ -19,9 +19,10 @@
 
 extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
                                            const ChunkDev* chunks, int nchunks,
-                                           long long ngroups, int lds_words, int scratch_words,
-                                           uint64_t* scratch, unsigned long long* work,
-                                           HitDev* hits, uint32_t* hit_count, uint32_t hit_cap);
+                                           long long ngroups, int lds_words, int shared_words,
+                                           int scratch_words, uint64_t* scratch,
+                                           unsigned long long* work, HitDev* hits,
+                                           uint32_t* hit_count, uint32_t hit_cap);
 extern "C" __global__ void rs_zero_u32(uint32_t*, int);
 
 namespace rsdev {
@@ -31,12 +32,22 @@ void free_job(DevJob& j) {
     (void)hipFree(j.d_rowinfo);
     (void)hipFree(j.d_rowk);
     (void)hipFree(j.d_bitmap);
-    j.d_kvals = j.d_rowinfo = j.d_rowk = j.d_bitmap = nullptr;
+    j.d_kvals = nullptr;
+    j.d_rowinfo = j.d_rowk = j.d_bitmap = nullptr;
 }
 
 size_t job_device_bytes(const rs_plan* pl, bool dat) {
     const size_t rows = pl->rows_k.size();
     return 12 * rows + (dat ? 4 * rows * (size_t)pl->words : 0) + 256;
+}
+
+// Jacobi constants of a 32-bit a > 0: odd part n, 2-adic valuation z and
+// floor((2^64 - 1) / n) split in 32-bit halves.
+uint4 jacobi_consts(uint32_t a) {
+    uint32_t z = 0;
+    while ((a & 1) == 0) a >>= 1, ++z;
+    const uint64_t rn = ~0ull / a;
+    return make_uint4(a, z, (uint32_t)rn, (uint32_t)(rn >> 32));
 }
 
 uint32_t magic_div(int d) {
@@ -70,7 +81,10 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.Q = pl->Q;
     J.m = m;
     J.x0 = pl->x0;
-    J.T = (pl->x1 - pl->x0 + 1 + m - 1) / m;
+    if (tw == 3) tw = 2;
+    // T in whole giant-step tiles: the extra steps search x past x1, whose
+    // n exceed maxN and are dropped
+    J.T = ((pl->x1 - pl->x0 + 1 + m - 1) / m + tw - 1) / tw * tw;
     J.minN = pl->minN;
     J.maxN = pl->maxN;
     J.nrows = nrows;
@@ -81,8 +95,8 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.logNB = std::max(logNB, 2);
     if (J.Q * J.T >= 65536 || J.T > 65535) return RS_ERR_ARG;
     J.invT = magic_div(J.T);
-    if (tw == 3) tw = 2;
     J.tw = tw;
+    J.lazy = kvals.back() < (1u << 24);
     // flattened giant-step indices of a row slice (and their division by T)
     // stay below 2^16
     if ((long long)std::min<int>(nrows, ROW_SLICE) * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;
@@ -90,13 +104,17 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + J.T;
     J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.T, nrows, J.W);
-    HIPCHK(hipMalloc(&dj->d_kvals, kvals.size() * 4));
+    std::vector<uint4> kjac(kvals.size());
+    for (size_t i = 0; i < kvals.size(); ++i) kjac[i] = jacobi_consts(kvals[i]);
+    J.bjac = jacobi_consts(pl->base);
+    HIPCHK(hipMalloc(&dj->d_kvals, kjac.size() * sizeof(uint4)));
     HIPCHK(hipMalloc(&dj->d_rowinfo, rowinfo.size() * 4));
     HIPCHK(hipMalloc(&dj->d_rowk, rowk.size() * 4));
-    HIPCHK(hipMemcpy(dj->d_kvals, kvals.data(), kvals.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dj->d_kvals, kjac.data(), kjac.size() * sizeof(uint4),
+                     hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dj->d_rowinfo, rowinfo.data(), rowinfo.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dj->d_rowk, rowk.data(), rowk.size() * 4, hipMemcpyHostToDevice));
-    J.kvals = dj->d_kvals;
+    J.kjac = dj->d_kvals;
     J.rowinfo = dj->d_rowinfo;
     J.rowk = dj->d_rowk;
     J.words = pl->words;
@@ -123,17 +141,19 @@ void free_launcher(Launcher& L) {
 
 // Sizes the grid for lds_words per wave and record size W (both maxima over
 // the jobs of the launch) and allocates scratch, counter and hits.
-rs_status launch_geometry(int device, int lds_words, Launcher* L) {
+rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words) {
     L->device = device;
     int max_lds = 0, cus = 0;
     HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    const size_t per_wave = (size_t)lds_words * 4;
-    if (lds_words < 1 || per_wave > (size_t)max_lds) return RS_ERR_ARG;
+    const size_t per_wave = (size_t)lds_words * 4, shared = (size_t)shared_words * 4;
+    if (lds_words < 1 || shared_words < 0 || per_wave + shared > (size_t)max_lds)
+        return RS_ERR_ARG;
     // waves of a block are independent (one table each); up to 4 per block
-    L->wpb = (int)std::min<size_t>(kMaxWavesPerBlock, (size_t)max_lds / per_wave);
+    L->wpb = (int)std::min<size_t>(kMaxWavesPerBlock, ((size_t)max_lds - shared) / per_wave);
     L->lds_words = lds_words;
-    L->lds_bytes = per_wave * L->wpb;
+    L->shared_words = shared_words;
+    L->lds_bytes = per_wave * L->wpb + shared;
     // the attribute is per function: always the device limit, so launches
     // of different LDS sizes never restrict one another
     if (L->lds_bytes > 65536)
@@ -157,8 +177,9 @@ static rs_status alloc_buffers(Launcher* L, size_t scratch_words_total, int64_t 
     return RS_OK;
 }
 
-rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity) {
-    rs_status st = launch_geometry(device, lds_words, L);
+rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity,
+                        int shared_words) {
+    rs_status st = launch_geometry(device, lds_words, L, shared_words);
     if (st != RS_OK) return st;
     L->scratch_words = 64 * W;
     return alloc_buffers(L, (size_t)L->grid * L->wpb * L->scratch_words, hit_capacity);
@@ -258,6 +279,7 @@ rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const 
     const int grid = (int)std::min<long long>(L.grid, (ngroups + L.wpb - 1) / L.wpb);
     hipLaunchKernelGGL(rs_sieve_kernel, dim3(std::max(1, grid)), dim3(64 * L.wpb), L.lds_bytes,
                        stream, j0, c0, jobs, chunks, nchunks, ngroups, L.lds_words,
+                       jobs ? 0 : L.shared_words,
                        L.scratch_words, L.scratch, L.work, L.hits, L.hit_count, L.hit_cap);
     HIPCHK(hipGetLastError());
     return RS_OK;

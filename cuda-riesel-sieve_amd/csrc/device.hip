@@ -104,7 +104,11 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int t
     s->device = device;
     s->max_batch = max_batch;
     rs_status st = make_job(pl, m, tile, false, &s->job);
-    if (st == RS_OK) st = init_launcher(&s->L, device, s->job.J.lds_words, s->job.J.W, hit_capacity);
+    // the row table goes to block-shared LDS when it fits one row slice
+    const int nrows = s->job.J.nrows;
+    const int shared = nrows <= ROW_SLICE ? (2 * nrows + 3) & ~3 : 0;
+    if (st == RS_OK)
+        st = init_launcher(&s->L, device, s->job.J.lds_words, s->job.J.W, hit_capacity, shared);
     if (st == RS_OK &&
         (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
          hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess))

@@ -23,19 +23,23 @@ struct JobDev {
     uint32_t base;
     int Q;
     int m;        // baby steps, multiple of 64
-    int T;        // giant steps per row: ceil((x1 - x0 + 1) / m)
+    int T;        // giant steps per row: ceil((x1 - x0 + 1) / m), rounded up to tw
     int x0;       // floor(minN / Q)
     int minN, maxN;
     int nrows;
     int nk;       // distinct k
     int logNB;    // baby table: NB = 2^logNB buckets of 8 16-bit fingerprints
     uint32_t invT;  // ceil(2^32 / T) for T >= 2, else 0: idx / T for idx < 2^16
-    int tw;         // giant-step tile: consecutive (row, t) items per lane, 1, 2 or 4
+    int tw;         // giant-step tile: consecutive t of one row per lane, 1, 2 or 4
+                    // (T is rounded up to a multiple of tw)
+    int lazy;       // giant-step products left in [0, 2p): every k < 2^24
     int W;        // record words
     int off_gt;   // REC_BH + Q
     int off_j;    // off_gt + T
     int lds_words;  // 32-bit LDS words per wave (table, counts, Y, rows, record)
-    const uint32_t* kvals;    // distinct k, ascending [nk]
+    const uint4* kjac;        // per distinct k (ascending): odd part n, 2-adic
+                              // valuation z, floor((2^64-1)/n) as lo, hi [nk]
+    uint4 bjac;               // the same for the base
     const uint32_t* rowinfo;  // per row: kidx | r << 16
     const uint32_t* rowk;     // per row: k
     const uint32_t* bitmap;   // optional n-list bitmaps (rows x words, MSB first)
@@ -72,7 +76,7 @@ static inline int rec_words(int Q, int T, int nk) {
 //   [0, 4 NB)                 table: NB buckets x 8 uint16 fingerprints (0 = empty)
 //   [4 NB, 4 NB + NB/2)       bucket fill counts, two uint16 per word
 //   then 2 Q T                Y products (uint64)
-//   then 2 min(nrows, ROW_SLICE)  active rows of one slice (k, row << 8 | r)
+//   then 2 min(nrows, ROW_SLICE)  active rows of one slice (k, 8 r T | row << 19)
 //   then 2 W                  the prime's record (uint64)
 static inline int lds_words_per_wave(int logNB, int Q, int T, int nrows, int W) {
     const int NB = 1 << logNB;

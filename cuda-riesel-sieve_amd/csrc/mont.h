@@ -35,11 +35,12 @@ __device__ __forceinline__ u64 mont_mprime(u64 p) {
     return 0ull - y;
 }
 
-// r - p if r >= p, else r, for r < 2p and p < 2^63: r - p wraps to at
-// least 2^64 - p > 2^63 exactly when r < p, so the sign bit decides.
+// r - p if r >= p, else r (r < 2p): the borrow of the 64-bit subtraction
+// selects, so the compare is the subtraction itself.
 __device__ __forceinline__ u64 csub(u64 r, u64 p) {
-    const u64 d = r - p;
-    return (long long)d < 0 ? r : d;
+    u64 d;
+    const bool borrow = __builtin_sub_overflow(r, p, &d);
+    return borrow ? r : d;
 }
 
 // Montgomery product a*b*2^-64 mod p.  Needs b < p and a < 2^64 (a may be
@@ -68,15 +69,23 @@ __device__ __forceinline__ u64 montmul(u64 a, u64 b, u64 p, u64 mp) {
     return csub(t3 + s3, p);
 }
 
-// k*y*2^-32 mod p for 32-bit k and y < p: one 32-bit Montgomery step.
-// k*y < 2^32 p and m*p < 2^32 p, so (k*y + m*p)/2^32 < 2p fits in 64 bits.
-__device__ __forceinline__ u64 halfmul(u32 k, u64 y, u64 p, u32 mp0) {
+// k*y*2^-32 mod p before the final subtraction, in [0, 2p), for 32-bit k
+// and y < p: one 32-bit Montgomery step.  With k*y = a1*2^32 + lo(a0) and
+// m*p = m*hi(p)*2^32 + m*lo(p), the quotient (k*y + m*p)/2^32 is
+// a1 + m*hi(p) + hi(m*lo(p) + lo(a0)) (the low word of that last sum is 0).
+// It is < 2p < 2^64, so the partial sum a1 + m*hi(p) cannot overflow and
+// takes a1 as the 64-bit addend of one multiply-add.
+__device__ __forceinline__ u64 halfmul_lazy(u32 k, u64 y, u64 p, u32 mp0) {
     const u64 a0 = mad32(k, lo32(y), 0);
     const u64 a1 = mad32(k, hi32(y), hi32(a0));
     const u32 m = lo32(a0) * mp0;
     const u64 s0 = mad32(m, lo32(p), lo32(a0));     // low word is 0
-    const u64 s1 = mad32(m, hi32(p), hi32(s0));
-    return csub(a1 + s1, p);
+    return mad32(m, hi32(p), a1) + hi32(s0);
+}
+
+// k*y*2^-32 mod p for 32-bit k and y < p.
+__device__ __forceinline__ u64 halfmul(u32 k, u64 y, u64 p, u32 mp0) {
+    return csub(halfmul_lazy(k, y, p, mp0), p);
 }
 
 // x^e in Montgomery form (x and one = R mod p in Montgomery form).
@@ -88,6 +97,49 @@ __device__ __forceinline__ u64 mont_pow(u64 x, u32 e, u64 one, u64 p, u64 mp) {
         if (e) x = montmul(x, x, p, mp);
     }
     return r;
+}
+
+// High 64 bits of x*y.
+__device__ __forceinline__ u64 umulhi64(u64 x, u64 y) {
+    const u64 t0 = mad32(lo32(x), lo32(y), 0);
+    const u64 t1 = mad32(lo32(x), hi32(y), hi32(t0));
+    const u64 t2 = mad32(hi32(x), lo32(y), lo32(t1));
+    return mad32(hi32(x), hi32(y), hi32(t1)) + hi32(t2);
+}
+
+// Binary Jacobi loop: t * (a/n) for odd n > 1 and a < 2^32.
+__device__ __forceinline__ int jacobi_loop(u32 a, u32 n, int t) {
+    while (a != 0) {
+        const u32 s = __builtin_ctz(a);
+        a >>= s;
+        const u32 n8 = n & 7;
+        if ((s & 1) && (n8 == 3 || n8 == 5)) t = -t;
+        if (a < n) {
+            const u32 tmp = a;
+            a = n;
+            n = tmp;
+            if ((a & 3) == 3 && (n & 3) == 3) t = -t;
+        }
+        a -= n;
+    }
+    return n == 1 ? t : 0;
+}
+
+// Jacobi symbol (a/p) for a = 2^z * n (n odd, both 32-bit) and odd p > 1,
+// with rn = floor((2^64 - 1) / n) precomputed: (2/p)^z, then reciprocity
+// (n/p) = (p/n) (-1)^((n-1)/2 (p-1)/2) and p mod n by a Barrett quotient
+// (at most 2 below the true one).
+__device__ __forceinline__ int jacobi_pre(u32 n, u32 z, u64 rn, u64 p) {
+    int t = 1;
+    const u32 p8 = lo32(p) & 7;
+    if ((z & 1) && (p8 == 3 || p8 == 5)) t = -t;
+    if (n == 1) return t;
+    if ((n & 3) == 3 && (p8 & 3) == 3) t = -t;
+    const u64 q = umulhi64(p, rn);
+    u64 r = p - mad32(lo32(q), n, (u64)(hi32(q) * n) << 32);
+    if (r >= n) r -= n;
+    if (r >= n) r -= n;
+    return jacobi_loop((u32)r, n, t);
 }
 
 // Jacobi symbol (a/p) for 32-bit a and odd p > 1, in {-1, 0, 1}.

@@ -59,10 +59,16 @@ __device__ __forceinline__ int div_T(u32 idx, u32 invT) {
     return invT ? (int)__umulhi(idx, invT) : (int)idx;
 }
 
-// 16-bit table fingerprint of a residue: 15 bits above the bucket index
-// with the top bit set, so it is never 0, the empty-slot value.
-__device__ __forceinline__ u32 fingerprint(u64 v, int logNB) {
-    return (lo32(v >> logNB) & 0x7fffu) | 0x8000u;
+// Table hash of a residue: the bucket is taken from bits 15 and up of its
+// low word, the 16-bit fingerprint is its low 15 bits with the top bit set,
+// so it is never 0, the empty-slot value.
+__device__ __forceinline__ u32 bucket_of(u64 v, u32 nbm) { return (lo32(v) >> 15) & nbm; }
+__device__ __forceinline__ u32 fingerprint(u64 v) { return (lo32(v) & 0x7fffu) | 0x8000u; }
+
+// Active-row entries of the giant steps: .x = k, .y = 8*r*T (byte offset of
+// the row's Y run, < 2^19) | (row - slice start) << 19.
+__device__ __forceinline__ u32 row_entry(u32 r, u32 T, u32 rowrel) {
+    return (r * T * 8u) | (rowrel << 19);
 }
 
 // Nonzero iff some 16-bit half of x is zero (exact as a boolean: the
@@ -72,14 +78,16 @@ __device__ __forceinline__ u32 has_zero_half(u32 x) {
 }
 
 // Emits every baby step j with g^j == target (all 64 lanes take part:
-// lane l checks j = l + 64 v) as a hit of giant step tt of row `row`
-// (k, r).  With n-list bitmaps only n in the row's list is emitted.
-__device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32 k, u32 ry,
-                                       int tt, u64 vstart, u64 g64, u64 p, u64 mp,
+// lane l checks j = l + 64 v) as a hit of giant step tt of the row whose
+// entry is (k, yent) in the slice starting at row r0.  With n-list bitmaps
+// only n in the row's list is emitted.
+__device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32 k, u32 yent,
+                                       int r0, int tt, u64 vstart, u64 g64, u64 p, u64 mp,
                                        HitDev* __restrict__ hits, u32* __restrict__ hit_count,
                                        u32 hit_cap) {
     const int lane = threadIdx.x & 63;
-    const u32 r = ry & 0xffu, row = ry >> 8;
+    const u32 r = (u32)div_T((yent & 0x7ffffu) >> 3, J.invT);
+    const u32 row = (u32)r0 + (yent >> 19);
     u64 w = vstart;
     for (int v = 0; v < (J.m >> 6); ++v) {
         if (w == target) {
@@ -108,86 +116,141 @@ __device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32
     }
 }
 
-// Giant steps of one prime: the A*T grid of (active row a, step t), t
-// fastest, is cut into runs of TW consecutive items per lane and 64*TW per
-// round.  Each item is one 32x64 Montgomery product s = k*Y[r][t] and one
-// bucket read; all TW of a lane are issued before any is tested.  A full
-// bucket (slot 7 taken) continues into the next bucket, and a fingerprint
-// match is resolved by rescan(); both are rare and handled after the fast
-// path.  The loop is wave-uniform because rescan() needs every lane.
-template <int TW>
-__device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A,
+// Lane mask of a per-lane condition (straight to a scalar register pair).
+__device__ __forceinline__ u64 lanes(bool c) { return __builtin_amdgcn_ballot_w64(c); }
+
+// Giant steps of one prime: the A*T grid of (active row a, step t) is cut
+// into runs of TW consecutive t of one row (T is a multiple of TW), one run
+// per lane and 64 runs per round.  A lane reads its row entry once and its
+// TW Y products at fixed offsets; each item is one 32x64 Montgomery
+// product s = k*Y[r][t] and one bucket read, all TW issued before any is
+// tested.  With LAZY the product is left in [0, 2p) (the case s >= p has
+// probability about k/2^33) and such items are re-probed after reduction;
+// with CHAIN (some bucket of this prime overflowed) an item whose bucket
+// is full also follows the chain to the next buckets.  Both go through the
+// slow path below, as does the exact check of a fingerprint match
+// (rescan()).  The loop is wave-uniform because rescan() needs every lane.
+template <int TW, bool CHAIN, bool LAZY>
+__device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int r0,
                                             const uint4* __restrict__ tab,
-                                            const u64* __restrict__ Y,
+                                            const char* __restrict__ Yb,
                                             const uint2* __restrict__ list, u64 vstart, u64 g64,
                                             u64 p, u64 mp, HitDev* __restrict__ hits,
                                             u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
-    const u32 NB = 1u << J.logNB;
+    const u32 nbm = (1u << J.logNB) - 1;
     const int T = J.T;
     const int total = A * T;
     const u32 mp0 = lo32(mp);
     for (int base = 0; base < total; base += 64 * TW) {
         const int i0 = base + TW * lane;
-        int a = div_T((u32)i0, J.invT), t = i0 - a * T;
+        const bool ok = i0 < total;
+        const u64 okm = lanes(ok);
+        int a = div_T((u32)i0, J.invT), t0 = i0 - a * T;
+        if (!ok) a = A - 1, t0 = 0;
+        const uint2 row = list[a];
+        const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * t0);
         u64 s[TW];
         uint4 e[TW];
-        u32 fp2[TW], bk[TW], ku[TW], ru[TW];
-        int tu[TW];
-        bool ok[TW];
+        u32 fp2[TW], bk[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            ok[u] = i0 + u < total;
-            const int ac = min(a, A - 1);
-            const uint2 row = list[ac];
-            ku[u] = row.x;
-            ru[u] = row.y;
-            tu[u] = t;
-            s[u] = halfmul(row.x, Y[(row.y & 0xffu) * T + min(t, T - 1)], p, mp0);
-            bk[u] = lo32(s[u]) & (NB - 1);
-            fp2[u] = fingerprint(s[u], J.logNB) * 0x10001u;
-            if (++t == T) { t = 0; ++a; }
+            s[u] = LAZY ? halfmul_lazy(row.x, yp[u], p, mp0) : halfmul(row.x, yp[u], p, mp0);
+            bk[u] = bucket_of(s[u], nbm);
+            const u32 fp = fingerprint(s[u]);
+            fp2[u] = fp | (fp << 16);
         }
 #pragma unroll
         for (int u = 0; u < TW; ++u) e[u] = tab[bk[u]];
-        u32 mbits = 0, fbits = 0;
+        u64 mm[TW], slow[TW];
+        u64 anys = 0;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             const u32 z = has_zero_half(e[u].x ^ fp2[u]) | has_zero_half(e[u].y ^ fp2[u]) |
                           has_zero_half(e[u].z ^ fp2[u]) | has_zero_half(e[u].w ^ fp2[u]);
-            mbits |= (u32)(ok[u] && z != 0) << u;
-            fbits |= (u32)(ok[u] && e[u].w >= 0x10000u) << u;  // slot 7 taken
+            mm[u] = lanes(z != 0) & okm;
+            slow[u] = 0;
+            if (LAZY) {
+                const u64 big = lanes(s[u] >= p) & okm;  // probed with the wrong bits
+                mm[u] &= ~big;
+                slow[u] |= big;
+            }
+            if (CHAIN) slow[u] |= lanes(e[u].w >= 0x10000u) & okm;  // slot 7 taken
+            anys |= slow[u];
         }
-        // full buckets: follow the chain to the first bucket with slot 7 free
-        if (__ballot(fbits != 0)) {
+        // slow path: reduce, then probe the bucket and its overflow chain
+        if ((CHAIN || LAZY) && anys) {
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
-                if ((fbits >> u) & 1) {
-                    u32 b = bk[u];
+                bool hit = false;
+                if ((slow[u] >> lane) & 1) {
+                    s[u] = csub(s[u], p);
+                    u32 b = bucket_of(s[u], nbm);
+                    const u32 fp = fingerprint(s[u]);
+                    const u32 f2 = fp | (fp << 16);
                     while (true) {
-                        b = (b + 1) & (NB - 1);
                         const uint4 f = tab[b];
-                        const u32 z = has_zero_half(f.x ^ fp2[u]) | has_zero_half(f.y ^ fp2[u]) |
-                                      has_zero_half(f.z ^ fp2[u]) | has_zero_half(f.w ^ fp2[u]);
-                        if (z) mbits |= 1u << u;
+                        const u32 z = has_zero_half(f.x ^ f2) | has_zero_half(f.y ^ f2) |
+                                      has_zero_half(f.z ^ f2) | has_zero_half(f.w ^ f2);
+                        if (z) hit = true;
                         if (f.w < 0x10000u) break;
+                        b = (b + 1) & nbm;
                     }
                 }
+                mm[u] |= lanes(hit);
             }
         }
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            u64 mb = __ballot((mbits >> u) & 1);
+            u64 mb = mm[u];
             while (mb) {
                 const int src = __builtin_ctzll(mb);
                 mb &= mb - 1;
-                rescan(J, job, readlane64(s[u], src), __builtin_amdgcn_readlane(ku[u], src),
-                       __builtin_amdgcn_readlane(ru[u], src),
-                       __builtin_amdgcn_readlane(tu[u], src), vstart, g64, p, mp, hits,
+                rescan(J, job, csub(readlane64(s[u], src), p), __builtin_amdgcn_readlane(row.x, src),
+                       __builtin_amdgcn_readlane(row.y, src), r0,
+                       __builtin_amdgcn_readlane(t0, src) + u, vstart, g64, p, mp, hits,
                        hit_count, hit_cap);
             }
         }
     }
+}
+
+// Rows [r0, r1) that pass the Jacobi filter of this prime (record words at
+// R), compacted in order into `list` as giant-step entries; returns their
+// number.  The row table (k, kidx | r << 16) is read from block-shared LDS
+// at rows_lds (FROM_LDS) or from the job's global arrays.
+template <bool FROM_LDS>
+__device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_lds,
+                                            const u64* __restrict__ R, uint2* __restrict__ list,
+                                            int r0, int r1, int T) {
+    const int lane = threadIdx.x & 63;
+    int A = 0;
+    for (int c = r0; c < r1; c += 64) {
+        const int row = c + lane;
+        bool pass = false;
+        u32 k = 0, r = 0;
+        if (row < r1) {
+            u32 info;
+            if (FROM_LDS) {
+                const uint2 rw = rows_lds[row];
+                k = rw.x;
+                info = rw.y;
+            } else {
+                info = J.rowinfo[row];
+                k = J.rowk[row];
+            }
+            const u32 kidx = info & 0xffffu;
+            r = info >> 16;
+            const u64 w = R[J.off_j + 2 * (kidx >> 6) + (r & 1)];
+            pass = (w >> (kidx & 63)) & 1;
+        }
+        const u64 mask = lanes(pass);
+        if (pass)
+            list[A + __popcll(mask & ((1ull << lane) - 1))] =
+                make_uint2(k, row_entry(r, (u32)T, (u32)(row - r0)));
+        A += __popcll(mask);
+    }
+    return A;
 }
 
 // Set-up of one prime (one lane): writes the record described in kernels.h.
@@ -230,12 +293,15 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
         gt = montmul(gt, G, p, mp);
     }
     // Jacobi bits; a prime dividing b gets none, so no row is searched
-    const int jb = B == 0 ? 0 : jacobi32(J.base, p);
+    const int jb = B == 0 ? 0
+                          : jacobi_pre(J.bjac.x, J.bjac.y,
+                                       ((u64)J.bjac.w << 32) | J.bjac.z, p);
     for (int w = 0; w < J.nk; w += 64) {
         u64 jk = 0, jkb = 0;
         const int e = min(64, J.nk - w);
         for (int i2 = 0; i2 < e; ++i2) {
-            const int jkv = jb == 0 ? 0 : jacobi32(J.kvals[w + i2], p);
+            const uint4 kj = J.kjac[w + i2];
+            const int jkv = jb == 0 ? 0 : jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
             jk |= (u64)(jkv == 1) << i2;
             jkb |= (u64)(jkv * jb == 1) << i2;
         }
@@ -252,10 +318,12 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
 // 0..63 (lane l has word l), the rest are read from Rg; LDS at wbase.
 __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
                                            const u64* __restrict__ Rg, u32* __restrict__ wbase,
-                                           HitDev* __restrict__ hits,
+                                           const uint2* __restrict__ rows_lds,
+                                           bool rows_in_lds, HitDev* __restrict__ hits,
                                            u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
     const u32 NB = 1u << J.logNB;
+    const u32 nbm = NB - 1;
     const int Q = J.Q, T = J.T, m = J.m, W = J.W;
     uint4* const tab = (uint4*)wbase;
     uint16_t* const tab16 = (uint16_t*)wbase;
@@ -279,9 +347,10 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     // first free slot of its bucket (or of the next non-full bucket)
     const u64 vstart = montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp);
     u64 v = vstart;
+    bool ovf = false;  // some insert went past a full bucket
     for (int s = 0; s < L; ++s) {
-        u32 b = lo32(v) & (NB - 1);
-        const u32 fp = fingerprint(v, J.logNB);
+        u32 b = bucket_of(v, nbm);
+        const u32 fp = fingerprint(v);
         while (true) {
             const u32 sh = (b & 1) * 16;
             const u32 c = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
@@ -289,10 +358,12 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
                 tab16[b * 8 + c] = (uint16_t)fp;
                 break;
             }
-            b = (b + 1) & (NB - 1);
+            ovf = true;
+            b = (b + 1) & nbm;
         }
         if (s + 1 < L) v = montmul(v, g64, p, mp);
     }
+    const bool chain = __ballot(ovf) != 0;
 
     // Y[r*T + t] = b^r H G^t 2^32 (Montgomery form)
     for (int idx = lane; idx < Q * T; idx += 64) {
@@ -304,47 +375,54 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     // are compacted in order as (k, row << 8 | r), then their giant steps run
     for (int r0 = 0; r0 < J.nrows; r0 += ROW_SLICE) {
         const int r1 = min(J.nrows, r0 + ROW_SLICE);
-        int A = 0;
-        for (int c = r0; c < r1; c += 64) {
-            const int row = c + lane;
-            bool pass = false;
-            u32 k = 0, r = 0;
-            if (row < r1) {
-                const u32 info = J.rowinfo[row];
-                const u32 kidx = info & 0xffffu;
-                r = info >> 16;
-                k = J.rowk[row];
-                const u64 w = R[J.off_j + 2 * (kidx >> 6) + (r & 1)];
-                pass = (w >> (kidx & 63)) & 1;
-            }
-            const u64 mask = __ballot(pass);
-            if (pass)
-                list[A + __popcll(mask & ((1ull << lane) - 1))] = make_uint2(k, ((u32)row << 8) | r);
-            A += __popcll(mask);
-        }
+        const int A = rows_in_lds ? compact_rows<true>(J, rows_lds, R, list, r0, r1, T)
+                                  : compact_rows<false>(J, rows_lds, R, list, r0, r1, T);
         wave_sync();
-        if (J.tw >= 4)
-            giant_steps<4>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
-        else if (J.tw == 2)
-            giant_steps<2>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
-        else
-            giant_steps<1>(J, job, A, tab, Y, list, vstart, g64, p, mp, hits, hit_count, hit_cap);
+        const char* const Yb = (const char*)Y;
+#define RS_GIANT(TWV, CH, LZ)                                                                  \
+    giant_steps<TWV, CH, LZ>(J, job, A, r0, tab, Yb, list, vstart, g64, p, mp, hits, hit_count, \
+                             hit_cap)
+#define RS_GIANT_TW(CH, LZ)          \
+    if (J.tw >= 4)                   \
+        RS_GIANT(4, CH, LZ);         \
+    else if (J.tw == 2)              \
+        RS_GIANT(2, CH, LZ);         \
+    else                             \
+        RS_GIANT(1, CH, LZ);
+        if (J.lazy) {
+            if (chain) { RS_GIANT_TW(true, true) } else { RS_GIANT_TW(false, true) }
+        } else {
+            if (chain) { RS_GIANT_TW(true, false) } else { RS_GIANT_TW(false, false) }
+        }
+#undef RS_GIANT_TW
+#undef RS_GIANT
         wave_sync();  // the next slice rewrites the list
     }
     wave_sync();  // the next prime reuses the table
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+// At least 4 wavefronts per SIMD (128 VGPRs): the LDS layouts of typical
+// jobs allow 4, and fewer leave the VALU idle on LDS waits.
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
                 const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
-                int lds_words, int scratch_words, uint64_t* __restrict__ scratch,
-                unsigned long long* __restrict__ work, HitDev* __restrict__ hits,
-                uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
+                int lds_words, int shared_words, int scratch_words,
+                uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
+                HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
     extern __shared__ uint32_t lds[];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wpb = blockDim.x >> 6;
-    u32* const wbase = lds + wv * lds_words;
+    // single-job launches keep the job's row table (k, kidx | r << 16) in
+    // block-shared LDS (shared_words >= 2 * nrows), read by every prime
+    uint2* const rows_lds = (uint2*)lds;
+    const bool rows_in_lds = shared_words > 0 && !jobs;
+    if (rows_in_lds) {
+        for (int i = threadIdx.x; i < j0.nrows; i += blockDim.x)
+            rows_lds[i] = make_uint2(j0.rowk[i], j0.rowinfo[i]);
+        __syncthreads();
+    }
+    u32* const wbase = lds + shared_words + wv * lds_words;
     u64* const S = scratch + ((long long)blockIdx.x * wpb + wv) * scratch_words;
 
     while (true) {
@@ -380,7 +458,8 @@ rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
         for (int i = 0; i < n; ++i) {
             const u64 cur = pre;
             if (i + 1 < n) pre = lane < J.W ? S[(i + 1) * J.W + lane] : 0;
-            bsgs_prime(J, C.job, cur, S + i * J.W, wbase, hits, hit_count, hit_cap);
+            bsgs_prime(J, C.job, cur, S + i * J.W, wbase, rows_lds, rows_in_lds, hits,
+                       hit_count, hit_cap);
         }
         wave_sync_global();  // records are rewritten by the next group
     }
