@@ -25,6 +25,8 @@ const int kMaxWavesPerBlock = 4;
 struct DevJob {
     JobDev J{};
     uint4* d_kvals = nullptr;    // Jacobi constants of each distinct k
+    uint32_t* d_kw0 = nullptr;   // row-mask words of each k
+    uint64_t* d_kmask = nullptr;
     uint32_t* d_rowinfo = nullptr;
     uint32_t* d_rowk = nullptr;
     uint32_t* d_bitmap = nullptr;

@@ -29,6 +29,10 @@ namespace rsdev {
 
 void free_job(DevJob& j) {
     (void)hipFree(j.d_kvals);
+    (void)hipFree(j.d_kw0);
+    (void)hipFree(j.d_kmask);
+    j.d_kw0 = nullptr;
+    j.d_kmask = nullptr;
     (void)hipFree(j.d_rowinfo);
     (void)hipFree(j.d_rowk);
     (void)hipFree(j.d_bitmap);
@@ -38,7 +42,8 @@ void free_job(DevJob& j) {
 
 size_t job_device_bytes(const rs_plan* pl, bool dat) {
     const size_t rows = pl->rows_k.size();
-    return 12 * rows + (dat ? 4 * rows * (size_t)pl->words : 0) + 256;
+    // row tables, and per k (at most one per row) Jacobi constants and masks
+    return 8 * rows + 52 * rows + (dat ? 4 * rows * (size_t)pl->words : 0) + 256;
 }
 
 // Jacobi constants of a 32-bit a > 0: odd part n, 2-adic valuation z and
@@ -100,10 +105,21 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     // flattened giant-step indices of a row slice (and their division by T)
     // stay below 2^16
     if ((long long)std::min<int>(nrows, ROW_SLICE) * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;
-    J.W = rec_words(J.Q, J.T, J.nk);
+    J.W = rec_words(J.Q, J.T, nrows);
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + J.T;
     J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.T, nrows, J.W);
+    // per k: the 64-row word of its first row and its even-r / odd-r rows
+    // in that word and the next (a k has at most Q < 64 rows)
+    std::vector<uint32_t> kw0(kvals.size(), 0xffffffffu);
+    std::vector<uint64_t> kmask(4 * kvals.size(), 0);
+    for (int i = 0; i < nrows; ++i) {
+        const uint32_t kidx = rowinfo[i] & 0xffffu, r = rowinfo[i] >> 16;
+        if (kw0[kidx] == 0xffffffffu) kw0[kidx] = (uint32_t)(i >> 6);
+        const uint32_t dw = (uint32_t)(i >> 6) - kw0[kidx];
+        if (dw > 1) return RS_ERR_ARG;  // rows of one k are contiguous and < 64
+        kmask[4 * kidx + 2 * (r & 1) + dw] |= 1ull << (i & 63);
+    }
     std::vector<uint4> kjac(kvals.size());
     for (size_t i = 0; i < kvals.size(); ++i) kjac[i] = jacobi_consts(kvals[i]);
     J.bjac = jacobi_consts(pl->base);
@@ -115,6 +131,12 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     HIPCHK(hipMemcpy(dj->d_rowinfo, rowinfo.data(), rowinfo.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dj->d_rowk, rowk.data(), rowk.size() * 4, hipMemcpyHostToDevice));
     J.kjac = dj->d_kvals;
+    HIPCHK(hipMalloc(&dj->d_kw0, kw0.size() * 4));
+    HIPCHK(hipMalloc(&dj->d_kmask, kmask.size() * 8));
+    HIPCHK(hipMemcpy(dj->d_kw0, kw0.data(), kw0.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dj->d_kmask, kmask.data(), kmask.size() * 8, hipMemcpyHostToDevice));
+    J.kw0 = dj->d_kw0;
+    J.kmask = dj->d_kmask;
     J.rowinfo = dj->d_rowinfo;
     J.rowk = dj->d_rowk;
     J.words = pl->words;

@@ -6,14 +6,16 @@
 // Record of one prime written by the set-up phase and read by the BSGS
 // phase (64-bit words).  All residues are Montgomery forms (R = 2^64).
 //   [0] p   [1] mp = -p^-1 mod 2^64   [2] one = R mod p   [3] g^64
-//   [4 .. 12)   g^c for c = 0..7          (g = b^Q)
-//   [12 .. 20)  g^(8a) for a = 0..7
-//   [20 .. 20+Q)        BH[r] = b^r * H * 2^32, H = g^(x0 + m - 1)
-//   [20+Q .. 20+Q+T)    GT[t] = G^t, G = g^m
-//   [20+Q+T .. +2*KW)   Jacobi bits: for each 64-k word w, word 2w has bit
-//                       i set iff (k/p) = +1 and word 2w+1 iff (k*b/p) = +1
-enum { REC_P = 0, REC_MP = 1, REC_ONE = 2, REC_G64 = 3, REC_GP = 4, REC_G8P = 12,
-       REC_BH = 20 };
+//   [4] g^64 mod p (plain)   [5] its Shoup constant floor(g^64 2^64 / p)
+//   [6 .. 14)   g^c for c = 0..7          (g = b^Q)
+//   [14 .. 22)  g^(8a) for a = 0..7
+//   [22 .. 22+Q)        BH[r] = b^r * H * 2^32, H = g^(x0 + m - 1)
+//   [22+Q .. 22+Q+T)    GT[t] = G^t, G = g^m
+//   [22+Q+T .. +NW)     row masks: bit i of word w is set iff row 64w+i passes
+//                       the Jacobi filter, (k/p) = +1 for even r and
+//                       (k*b/p) = +1 for odd r (NW = ceil(nrows / 64))
+enum { REC_P = 0, REC_MP = 1, REC_ONE = 2, REC_G64 = 3, REC_GW = 4, REC_GWS = 5, REC_GP = 6,
+       REC_G8P = 14, REC_BH = 22 };
 
 // Rows handled per compaction pass of the BSGS phase: bounds the LDS list
 // of active rows whatever the job's row count.
@@ -35,11 +37,14 @@ struct JobDev {
     int lazy;       // giant-step products left in [0, 2p): every k < 2^24
     int W;        // record words
     int off_gt;   // REC_BH + Q
-    int off_j;    // off_gt + T
+    int off_j;    // off_gt + T: the row masks
     int lds_words;  // 32-bit LDS words per wave (table, counts, Y, rows, record)
     const uint4* kjac;        // per distinct k (ascending): odd part n, 2-adic
                               // valuation z, floor((2^64-1)/n) as lo, hi [nk]
     uint4 bjac;               // the same for the base
+    const uint32_t* kw0;      // per distinct k: 64-row word holding its first row
+    const uint64_t* kmask;    // per distinct k: its even-r rows in words w0 and
+                              // w0 + 1, then its odd-r rows in the same words
     const uint32_t* rowinfo;  // per row: kidx | r << 16
     const uint32_t* rowk;     // per row: k
     const uint32_t* bitmap;   // optional n-list bitmaps (rows x words, MSB first)
@@ -68,8 +73,8 @@ struct HitDev {
     uint32_t pad;
 };
 
-static inline int rec_words(int Q, int T, int nk) {
-    return REC_BH + Q + T + 2 * ((nk + 63) / 64);
+static inline int rec_words(int Q, int T, int nrows) {
+    return REC_BH + Q + T + (nrows + 63) / 64;
 }
 
 // LDS layout of one wavefront, in 32-bit words from its base:

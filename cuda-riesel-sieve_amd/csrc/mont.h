@@ -88,6 +88,31 @@ __device__ __forceinline__ u64 halfmul(u32 k, u64 y, u64 p, u32 mp0) {
     return csub(halfmul_lazy(k, y, p, mp0), p);
 }
 
+// High 64 bits of x*y.
+__device__ __forceinline__ u64 umulhi64(u64 x, u64 y) {
+    const u64 t0 = mad32(lo32(x), lo32(y), 0);
+    const u64 t1 = mad32(lo32(x), hi32(y), hi32(t0));
+    const u64 t2 = mad32(hi32(x), lo32(y), lo32(t1));
+    return mad32(hi32(x), hi32(y), hi32(t1)) + hi32(t2);
+}
+
+// Multiplication by a fixed residue (Shoup): x*w mod p for x < 2^64 and a
+// plain w < p with ws = floor(w 2^64 / p).  q = hi64(x ws) is the quotient
+// or one less, so x*w - q*p (computed mod 2^64) lies in [0, 2p).  With x a
+// Montgomery form the result is the Montgomery form of the product.  For
+// w~ = w R mod p (w's Montgomery form), w = redc(w~) and ws = w~ * mp mod
+// 2^64 exactly, because w 2^64 = ws p + w~ and mp = -p^-1 mod 2^64.
+__device__ __forceinline__ u64 shoup_mul(u64 x, u64 w, u64 ws, u64 p) {
+    const u64 q = umulhi64(x, ws);
+    return csub(x * w - q * p, p);
+}
+
+// The Shoup pair (w, ws) of a Montgomery form wm.
+__device__ __forceinline__ void shoup_pair(u64 wm, u64 p, u64 mp, u64* w, u64* ws) {
+    *w = montmul(wm, 1, p, mp);
+    *ws = wm * mp;
+}
+
 // x^e in Montgomery form (x and one = R mod p in Montgomery form).
 __device__ __forceinline__ u64 mont_pow(u64 x, u32 e, u64 one, u64 p, u64 mp) {
     u64 r = one;
@@ -97,14 +122,6 @@ __device__ __forceinline__ u64 mont_pow(u64 x, u32 e, u64 one, u64 p, u64 mp) {
         if (e) x = montmul(x, x, p, mp);
     }
     return r;
-}
-
-// High 64 bits of x*y.
-__device__ __forceinline__ u64 umulhi64(u64 x, u64 y) {
-    const u64 t0 = mad32(lo32(x), lo32(y), 0);
-    const u64 t1 = mad32(lo32(x), hi32(y), hi32(t0));
-    const u64 t2 = mad32(hi32(x), lo32(y), lo32(t1));
-    return mad32(hi32(x), hi32(y), hi32(t1)) + hi32(t2);
 }
 
 // Binary Jacobi loop: t * (a/n) for odd n > 1 and a < 2^32.

@@ -119,7 +119,7 @@ extern "C" rs_status rs_server_create(int device, int64_t hbm_budget, int max_ac
     s->width = chunk_width;
     s->quantum_ms = quantum_ms;
     s->lds_cap_words = lds_cap_bytes / 4;
-    s->w_cap = 128;  // record words: 20 + Q + T + 2 ceil(K/64)
+    s->w_cap = 256;  // record words: 22 + Q + T + ceil(rows/64)
     s->buf_cap = gpu_primes_capacity(chunk_width);
     Launcher g;
     rs_status st = launch_geometry(device, s->lds_cap_words, &g);  // cap fits the device

@@ -215,10 +215,11 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
     }
 }
 
-// Rows [r0, r1) that pass the Jacobi filter of this prime (record words at
-// R), compacted in order into `list` as giant-step entries; returns their
-// number.  The row table (k, kidx | r << 16) is read from block-shared LDS
-// at rows_lds (FROM_LDS) or from the job's global arrays.
+// Rows [r0, r1) that pass the Jacobi filter of this prime (row masks at
+// R + off_j, r0 a multiple of 64), compacted in order into `list` as
+// giant-step entries; returns their number.  The row table (k, kidx |
+// r << 16) is read from block-shared LDS at rows_lds (FROM_LDS) or from the
+// job's global arrays, by the passing lanes only.
 template <bool FROM_LDS>
 __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_lds,
                                             const u64* __restrict__ R, uint2* __restrict__ list,
@@ -226,11 +227,13 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
     const int lane = threadIdx.x & 63;
     int A = 0;
     for (int c = r0; c < r1; c += 64) {
-        const int row = c + lane;
-        bool pass = false;
-        u32 k = 0, r = 0;
-        if (row < r1) {
-            u32 info;
+        const u64 wm = R[J.off_j + (c >> 6)];
+        const u32 mlo = (u32)__builtin_amdgcn_readfirstlane(lo32(wm));
+        const u32 mhi = (u32)__builtin_amdgcn_readfirstlane(hi32(wm));
+        const u64 mask = ((u64)mhi << 32) | mlo;
+        if ((mask >> lane) & 1) {
+            const int row = c + lane;
+            u32 k, info;
             if (FROM_LDS) {
                 const uint2 rw = rows_lds[row];
                 k = rw.x;
@@ -239,15 +242,10 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
                 info = J.rowinfo[row];
                 k = J.rowk[row];
             }
-            const u32 kidx = info & 0xffffu;
-            r = info >> 16;
-            const u64 w = R[J.off_j + 2 * (kidx >> 6) + (r & 1)];
-            pass = (w >> (kidx & 63)) & 1;
+            const u32 below = __builtin_amdgcn_mbcnt_hi(hi32(mask),
+                                                        __builtin_amdgcn_mbcnt_lo(lo32(mask), 0));
+            list[A + below] = make_uint2(k, row_entry(info >> 16, (u32)T, (u32)(row - r0)));
         }
-        const u64 mask = lanes(pass);
-        if (pass)
-            list[A + __popcll(mask & ((1ull << lane) - 1))] =
-                make_uint2(k, row_entry(r, (u32)T, (u32)(row - r0)));
         A += __popcll(mask);
     }
     return A;
@@ -265,49 +263,73 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     const u64 g = mont_pow(B, (u32)J.Q, one, p, mp);
     // baby-step start powers: lane l of the BSGS wave starts at
     // g^(l & 7) * g^(8 (l >> 3))
+    u64 w, ws;
+    shoup_pair(g, p, mp, &w, &ws);
     u64 x = one;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         R[REC_GP + c] = x;
-        x = montmul(x, g, p, mp);
+        x = shoup_mul(x, w, ws, p);
     }
     const u64 g8 = x;
+    shoup_pair(g8, p, mp, &w, &ws);
     x = one;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
         R[REC_G8P + a] = x;
-        x = montmul(x, g8, p, mp);
+        x = shoup_mul(x, w, ws, p);
     }
     const u64 g64 = x;
+    shoup_pair(g64, p, mp, &w, &ws);
+    R[REC_GW] = w;
+    R[REC_GWS] = ws;
     const u64 G = mont_pow(g64, (u32)(J.m >> 6), one, p, mp);
     const u64 H = mont_pow(g, (u32)(J.x0 + J.m - 1), one, p, mp);
     const u64 c32 = montmul(1ull << 32, r2, p, mp);  // Mont(2^32)
     u64 bh = montmul(H, c32, p, mp);
+    shoup_pair(B, p, mp, &w, &ws);
     for (int r = 0; r < J.Q; ++r) {
         R[REC_BH + r] = bh;
-        bh = montmul(bh, B, p, mp);
+        bh = shoup_mul(bh, w, ws, p);
     }
+    shoup_pair(G, p, mp, &w, &ws);
     u64 gt = one;
     for (int t = 0; t < J.T; ++t) {
         R[J.off_gt + t] = gt;
-        gt = montmul(gt, G, p, mp);
+        gt = shoup_mul(gt, w, ws, p);
     }
     // Jacobi bits; a prime dividing b gets none, so no row is searched
     const int jb = B == 0 ? 0
                           : jacobi_pre(J.bjac.x, J.bjac.y,
                                        ((u64)J.bjac.w << 32) | J.bjac.z, p);
-    for (int w = 0; w < J.nk; w += 64) {
-        u64 jk = 0, jkb = 0;
-        const int e = min(64, J.nk - w);
-        for (int i2 = 0; i2 < e; ++i2) {
-            const uint4 kj = J.kjac[w + i2];
-            const int jkv = jb == 0 ? 0 : jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
-            jk |= (u64)(jkv == 1) << i2;
-            jkb |= (u64)(jkv * jb == 1) << i2;
+    // row masks: k (in order) adds its even-r rows if (k/p) = +1 and its
+    // odd-r rows if (kb/p) = +1 to words w0, w0 + 1; words are written once,
+    // in order, when no later k can reach them
+    const int nw = (J.nrows + 63) >> 6;
+    u64* const RM = R + J.off_j;
+    int cur = 0;
+    u64 lo = 0, hi = 0;  // words cur, cur + 1
+    for (int i = 0; i < J.nk; ++i) {
+        const int w0 = (int)J.kw0[i];
+        if (w0 != cur) {
+            RM[cur] = lo;
+            if (w0 == cur + 1) {
+                lo = hi;
+            } else {
+                RM[cur + 1] = hi;
+                lo = 0;
+            }
+            hi = 0;
+            cur = w0;
         }
-        R[J.off_j + 2 * (w >> 6)] = jk;
-        R[J.off_j + 2 * (w >> 6) + 1] = jkb;
+        const uint4 kj = J.kjac[i];
+        const int jkv = jb == 0 ? 0 : jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
+        const u64* const km = J.kmask + 4 * i;
+        if (jkv == 1) lo |= km[0], hi |= km[1];
+        if (jkv * jb == 1) lo |= km[2], hi |= km[3];
     }
+    RM[cur] = lo;
+    if (cur + 1 < nw) RM[cur + 1] = hi;
     R[REC_P] = p;
     R[REC_MP] = mp;
     R[REC_ONE] = one;
@@ -337,6 +359,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     for (int w = 64 + lane; w < W; w += 64) R[w] = Rg[w];
     const u64 p = readlane64(pre, REC_P), mp = readlane64(pre, REC_MP);
     const u64 g64 = readlane64(pre, REC_G64);
+    const u64 gw = readlane64(pre, REC_GW), gws = readlane64(pre, REC_GWS);
 
     // clear table and counts
     for (u32 b = lane; b < NB; b += 64) tab[b] = make_uint4(0, 0, 0, 0);
@@ -361,7 +384,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
             ovf = true;
             b = (b + 1) & nbm;
         }
-        if (s + 1 < L) v = montmul(v, g64, p, mp);
+        if (s + 1 < L) v = shoup_mul(v, gw, gws, p);
     }
     const bool chain = __ballot(ovf) != 0;
 
@@ -428,8 +451,8 @@ rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
     while (true) {
         unsigned long long grp = 0;
         if (lane == 0) grp = atomicAdd(work, 1ull);
-        grp = ((unsigned long long)__builtin_amdgcn_readfirstlane((u32)(grp >> 32)) << 32) |
-              __builtin_amdgcn_readfirstlane((u32)grp);
+        grp = ((unsigned long long)(u32)__builtin_amdgcn_readfirstlane((u32)(grp >> 32)) << 32) |
+              (u32)__builtin_amdgcn_readfirstlane((u32)grp);
         if ((long long)grp >= ngroups) break;
         // the chunk holding this group (chunks sorted by group0)
         int ci = 0;
