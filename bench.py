@@ -75,6 +75,10 @@ def main():
     ap.add_argument("--serve-quantum-ms", type=float, default=40.0,
                     help="device time per job per server step (0: whole 2^30 windows)")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-real", action="store_true")
+    ap.add_argument("--no-edge", action="store_true")
+    ap.add_argument("--real-chunks", type=int, default=8)
+    ap.add_argument("--edge-chunks", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -153,15 +157,17 @@ def main():
     hbm = rs.dev_copy_rate(copy_bytes)
     # HBM bytes per prime from the committed counter summary (FETCH_SIZE and
     # WRITE_SIZE scaled by the copy-probe calibration), times primes per launch
-    traffic = None
+    traffic, prof_cfg = None, "?"
     prof = os.path.join(ROOT, "profiles", "rs_sieve_kernel_r745_pmc.txt")
     if os.path.exists(prof):
         for line in open(prof):
             if line.startswith("traffic_bytes_per_prime:"):
                 traffic = float(line.split(":")[1]) * per_launch_primes
+            if line.startswith("config:"):
+                prof_cfg = line.split(":", 1)[1].strip()
     roof = {"bound": "valu", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "Gmul/s",
             "frac": achieved / peak, "traffic": traffic,
-            "traffic_source": "profiles/rs_sieve_kernel_r745_pmc.txt (m=576, tile=2)",
+            "traffic_source": f"profiles/rs_sieve_kernel_r745_pmc.txt ({prof_cfg})",
             "kernel": "rs_sieve_kernel", "muls_per_prime": muls_per_prime,
             "primes_per_launch": per_launch_primes,
             "mean_launch_ms": bsgs_s_per_launch * 1e3,
@@ -203,7 +209,19 @@ def main():
                "sample": f"first {n} primes of [1e12, 1e12+2^30) (p-range {span}), "
                          f"same job and m, {dt:.2f}s"}
         log(f"cpu: {n} primes in {dt:.2f}s on {args.cpu_threads} threads")
+        # one core, on a sample sized to about a quarter of the budget
+        n1 = max(1000, min(len(sample_offs), int(n / args.cpu_threads / 4)))
+        ts = time.perf_counter()
+        oracle.bsgs_many(job, p0, sample_offs[:n1], threads=1)
+        dt1 = time.perf_counter() - ts
+        span1 = int(sample_offs[n1 - 1]) + 1
+        cpu["one_core"] = {"value": span1 / dt1, "unit": "p/sec", "cores": 1,
+                           "primes_per_sec": n1 / dt1,
+                           "sample": f"first {n1} primes of [1e12, 1e12+2^30), {dt1:.2f}s"}
+        log(f"cpu: {n1} primes in {dt1:.2f}s on 1 thread")
 
+    real = None if args.no_real else bench_real_r745(args, rs, chunk)
+    edge = None if args.no_edge else bench_edge(args, rs, chunk)
     batched = None if args.no_batch else bench_batched(args, rs, p0, chunk)
     serving = None if args.no_serve else bench_serving(args, rs, p0, chunk)
 
@@ -234,10 +252,93 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "autotune": tune,
+        "r745_abcd": real,
+        "edge_base2": edge,
         "batched": batched,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def timed_device_chunks(rs, sv, lows, bufs):
+    """Kernel-resident rate of a sieve over device-resident chunks."""
+    sv.submit_device(lows[0], bufs[0])
+    sv.collect()
+    sv.timing()
+    t0 = time.perf_counter()
+    for lo, b in zip(lows, bufs):
+        sv.submit_device(lo, b)
+    hits = sv.collect()
+    dt = time.perf_counter() - t0
+    return hits, dt, sv.timing()
+
+
+def bench_real_r745(args, rs, chunk):
+    """The reference's own input, sr_745.abcd, at its README flags (-m 8,
+    m = 512; -Q 18), from its default start p = 102297149770000, with the
+    primes generated on the device beforehand."""
+    path = os.path.join(ROOT, "tests", "golden", "sr_745.abcd")
+    plan = rs.Plan(rs.Abcd.read(path), 18)
+    p0 = 102297149770000
+    K = args.real_chunks
+    gpg = rs.GpuPrimeGen(p0 + K * chunk + 1)
+    lows = [p0 + i * chunk for i in range(K)]
+    bufs = [gpg.device_range(lo, lo + chunk) for lo in lows]
+    nprimes = sum(b.count for b in bufs)
+    out = {"workload": f"sr_745.abcd (the reference's test input: 22 even k, {plan.nrows} rows "
+                       f"at Q=18), p in [{p0}, +{K}*2^30)", "primes": nprimes}
+    m_t, t_t, _ = rs.autotune(plan, p0, bufs[0], count=min(bufs[0].count, 1 << 22))
+    for name, m, tile in (("readme_m512", 512, 0), ("autotuned", m_t, t_t)):
+        sv = rs.Sieve(plan, m=m, tile=tile, max_batch=1 << 10, hit_capacity=1 << 20)
+        hits, dt, tm = timed_device_chunks(rs, sv, lows, bufs)
+        out[name] = {"m": sv.m, "tile": sv.tile, "T": sv.T, "value": K * chunk / dt,
+                     "unit": "p/sec", "primes_per_sec": nprimes / dt,
+                     "vs_a100_published": K * chunk / dt / A100_PUBLISHED,
+                     "factors": len(hits), "kernel_ms": tm["kernel_ms"]}
+        if plan.contains(18300, 242670):
+            raise RuntimeError("sr_745.abcd unexpectedly lists the datless debug factor")
+        sv.close()
+        log(f"sr_745.abcd {name}: m={out[name]['m']} tile={out[name]['tile']} "
+            f"{out[name]['value']:.3e} p/s, {len(hits)} factors")
+    for b in bufs:
+        b.free()
+    return out
+
+
+def bench_edge(args, rs, chunk):
+    """Many-k / large-p edge: base 2, 500 odd k in [3, 1e9], n in [1e6, 2e6]
+    presieved by odd primes < 1e6, p just below 2^62 (device-generated,
+    Miller-Rabin above the sieving bound).  The baby table never leaves
+    LDS: the autotuner trades m against occupancy."""
+    t = time.time()
+    plan = rs.Plan(rs.Abcd.synth(2, 500, 3, 10**9, 1000000, 2000000, 10**6, seed=3))
+    synth_s = time.time() - t
+    K = args.edge_chunks
+    hi = 1 << 62
+    lows = [hi - (K - i) * chunk for i in range(K)]
+    gpg = rs.GpuPrimeGen(hi + 1)
+    t = time.time()
+    bufs = [gpg.device_range(lo, lo + chunk) for lo in lows]
+    gen_s = time.time() - t
+    nprimes = sum(b.count for b in bufs)
+    m, tile, cands = rs.autotune(plan, lows[0], bufs[0], count=min(bufs[0].count, 1 << 20),
+                                 effort=1)
+    sv = rs.Sieve(plan, m=m, tile=tile, max_batch=1 << 10, hit_capacity=1 << 20)
+    hits, dt, tm = timed_device_chunks(rs, sv, lows, bufs)
+    out = {"workload": f"base 2, 500 odd k in [3,1e9] (seed 3), n in [1e6,2e6] presieved by "
+                       f"odd primes < 1e6 ({plan.nrows} rows, Q={plan.Q}), "
+                       f"p in [2^62 - {K}*2^30, 2^62)",
+           "value": K * chunk / dt, "unit": "p/sec", "primes_per_sec": nprimes / dt,
+           "primes": nprimes, "m": sv.m, "tile": sv.tile, "T": sv.T,
+           "lds_bytes_per_wave": sv.lds_bytes, "factors": len(hits),
+           "kernel_ms": tm["kernel_ms"], "gpu_primegen_s": gen_s, "synth_s": synth_s,
+           "autotune": [[cm, ct, round(ns, 3)] for cm, ct, ns in cands]}
+    log(f"edge: m={sv.m} tile={sv.tile} T={sv.T} lds/wave={sv.lds_bytes}B "
+        f"{out['value']:.3e} p/s")
+    sv.close()
+    for b in bufs:
+        b.free()
+    return out
 
 
 def tune_jobs(rs, plans, p0, buf, sample):
