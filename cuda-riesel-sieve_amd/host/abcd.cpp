@@ -18,6 +18,7 @@
 #include <fstream>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <unordered_set>
 
 #include "host.h"
@@ -164,6 +165,47 @@ extern "C" rs_status rs_abcd_write(const rs_abcd* a, const char* path) {
         for (int j = s + 1; j < e; ++j) fprintf(f, "%d\n", a->ns[j] - a->ns[j - 1]);
     }
     return fclose(f) == 0 ? RS_OK : RS_ERR_IO;
+}
+
+extern "C" rs_status rs_abcd_remove(rs_abcd* a, const rs_factor* factors, int64_t count,
+                                   int64_t* removed) {
+    if (!a || count < 0 || (count > 0 && !factors)) return RS_ERR_ARG;
+    // n to remove per k, sorted
+    std::unordered_map<uint32_t, std::vector<int32_t>> drop;
+    for (int64_t i = 0; i < count; ++i) drop[factors[i].k].push_back((int32_t)factors[i].n);
+    for (auto& kv : drop) {
+        std::sort(kv.second.begin(), kv.second.end());
+        kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
+    }
+    std::vector<int32_t> ks, kstart{0}, ns;
+    ns.reserve(a->ns.size());
+    int64_t gone = 0;
+    for (size_t i = 0; i < a->ks.size(); ++i) {
+        const int s0 = a->kstart[i], e0 = a->kstart[i + 1];
+        auto it = drop.find((uint32_t)a->ks[i]);
+        const size_t before = ns.size();
+        for (int j = s0; j < e0; ++j) {
+            const int32_t n = a->ns[j];
+            if (it != drop.end() && std::binary_search(it->second.begin(), it->second.end(), n)) {
+                ++gone;
+                continue;
+            }
+            ns.push_back(n);
+        }
+        if (ns.size() == before) continue;  // list emptied: drop the k
+        ks.push_back(a->ks[i]);
+        kstart.push_back((int32_t)ns.size());
+    }
+    a->ks.swap(ks);
+    a->kstart.swap(kstart);
+    a->ns.swap(ns);
+    a->minN = a->maxN = 0;
+    if (!a->ns.empty()) {
+        a->minN = *std::min_element(a->ns.begin(), a->ns.end());
+        a->maxN = *std::max_element(a->ns.begin(), a->ns.end());
+    }
+    if (removed) *removed = gone;
+    return RS_OK;
 }
 
 extern "C" void rs_abcd_free(rs_abcd* a) { delete a; }

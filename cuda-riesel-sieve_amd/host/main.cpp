@@ -7,7 +7,9 @@
 // SCALE (primes per launch = 128*2^(b-1) * 128*2^(t-1), :817-818, :849-851),
 // -s SCALE (at least 4096*s sieving primes, :1274), -v (no verification:
 // datless output), -g (generate the primes on the GPU; the default is the
-// CPU generator of the reference's design), -h.  Unlike :705, -h and -v
+// CPU generator of the reference's design), -R FILE (write the input ABCD
+// with every verified factor's candidate removed; the reference only
+// appends factor lines), -h.  Unlike :705, -h and -v
 // take no argument.  With no
 // -p/-P the default test range of :837-839 is used.  The sieved range is
 // clipped to [P0, P1] (the reference runs past P1 with its last batch).
@@ -42,6 +44,7 @@ static void usage() {
         "-s SCALE : Use at least 4096*SCALE small primes for the prime generator. Default 256.\n"
         "-v       : Don't verify factors. This implies running datless.\n"
         "-g       : Generate the primes on the GPU instead of the CPU generator.\n"
+        "-R FILE  : Write the ABCD file minus the candidates with a verified factor to FILE.\n"
         "-d DEV   : GPU device index (default 0).\n"
         "-h       : Prints this help\n");
 }
@@ -56,9 +59,10 @@ int main(int argc, char* argv[]) {
     int smallPrimeScale = 256, device = 0, gpuPrimes = 0;
     const char* abcdFile = "sr_745.abcd";
     const char* factorFile = "factors.txt";
+    const char* removedFile = nullptr;
     unsigned long long low = 0, high = 0;
     int opt;
-    while ((opt = getopt(argc, argv, "b:hi:m:p:P:Q:O:s:t:vd:g")) != -1) {
+    while ((opt = getopt(argc, argv, "b:hi:m:p:P:Q:O:R:s:t:vd:g")) != -1) {
         switch (opt) {
             case 'b': blockScale = (int)strtol(optarg, nullptr, 0); break;
             case 'h': usage(); return 0;
@@ -79,8 +83,13 @@ int main(int argc, char* argv[]) {
             case 'v': verify = 0; break;
             case 'd': device = (int)strtol(optarg, nullptr, 0); break;
             case 'g': gpuPrimes = 1; break;
+            case 'R': removedFile = optarg; break;
             default: usage(); return 1;
         }
+    }
+    if (removedFile && !verify) {
+        printf("Bad input parameters : -R needs verified factors (it cannot be used with -v).\n");
+        return 1;
     }
     if (blockScale < 1 || blockScale > 31 || threadScale < 1 || threadScale > 31) {
         printf("Bad input parameter : 1 <= BlockScale, ThreadScale <= 31.\n");
@@ -156,6 +165,7 @@ int main(int argc, char* argv[]) {
     const unsigned long long startLow = low;
     std::vector<uint32_t> cur((size_t)arraySize), nxt((size_t)arraySize);
     std::vector<rs_factor> hits(1 << 20);
+    std::vector<rs_factor> verified;  // candidates to remove with -R
     // one batch: the first arraySize primes >= lo (lo odd), clipped to <= high
     auto make_batch = [&](uint64_t lo, std::vector<uint32_t>& buf, long long* n,
                           uint64_t* nextLow) -> rs_status {
@@ -216,6 +226,7 @@ int main(int argc, char* argv[]) {
                     return 1;
                 }
             }
+            if (removedFile) verified.push_back(f);
             char line[128];
             rs_format_factor(&f, base, line, sizeof line);
             fputs(line, stdout);
@@ -253,6 +264,16 @@ int main(int argc, char* argv[]) {
     printf("Factors found: %lld\n", factorCount);
     printf("Time per kernel: %g\n", total / kernelCount);
     printf("Progress = %llu at %.0f p/sec\n\n", swept, swept / total);
+    if (removedFile) {
+        int64_t removed = 0;
+        if ((st = rs_abcd_remove(abcd, verified.data(), (int64_t)verified.size(), &removed)) !=
+                RS_OK ||
+            (st = rs_abcd_write(abcd, removedFile)) != RS_OK)
+            return fail("writing the reduced ABCD file", st);
+        rs_abcd_info(abcd, nullptr, &nk, &nn, nullptr, nullptr);
+        printf("Removed %lld candidates; %d k-values, %d candidates left in %s\n",
+               (long long)removed, nk, nn, removedFile);
+    }
     rs_primegen_free(gen);
     rs_sieve_destroy(sv);
     rs_plan_free(plan);

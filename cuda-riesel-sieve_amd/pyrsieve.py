@@ -46,6 +46,7 @@ _SIGS = {
     "rs_abcd_info": (i32, [vp, P(u32), P(i32), P(i32), P(i32), P(i32)]),
     "rs_abcd_lists": (i32, [vp, P(i32), P(i32), P(i32)]),
     "rs_abcd_write": (i32, [vp, ctypes.c_char_p]),
+    "rs_abcd_remove": (i32, [vp, vp, i64, P(i64)]),
     "rs_abcd_free": (None, [vp]),
     "rs_abcd_synth": (i32, [u32, i32, u32, u32, i32, i32, i32, u32, u64, P(vp)]),
     "rs_plan_create": (i32, [vp, i32, P(vp)]),
@@ -178,6 +179,15 @@ class Abcd:
         _chk("rs_abcd_synth", lib().rs_abcd_synth(base, nk, kmin, kmax, int(odd_k), nmin, nmax,
                                                   presieve_bound, seed, ctypes.byref(h)))
         return cls(h.value)
+
+    def remove(self, factors):
+        """Drops the candidates (k, n) of factors [(p, k, n), ...]; returns how many."""
+        arr = (Factor * max(1, len(factors)))()
+        for i, (p, k, n) in enumerate(factors):
+            arr[i] = Factor(p, k, n)
+        gone = i64()
+        _chk("rs_abcd_remove", lib().rs_abcd_remove(self.h, arr, len(factors), ctypes.byref(gone)))
+        return gone.value
 
     def info(self):
         b, nk, nn, lo, hi = u32(), i32(), i32(), i32(), i32()

@@ -72,6 +72,15 @@ rs_status rs_abcd_lists(const rs_abcd* a, int32_t* ks, int32_t* kstart,
                         int32_t* ns);
 /* Writes the ABCD text form (srsieve layout, one header per k). */
 rs_status rs_abcd_write(const rs_abcd* a, const char* path);
+/* Factor removal ("dat" mode; no reference counterpart: the reference only
+ * appends factor lines, :1606-1611, :1832).  Removes every candidate (k, n)
+ * named by factors[0..count) from the lists (pairs not in the lists are
+ * ignored), drops k whose list becomes empty, recomputes minN/maxN and sets
+ * *removed (may be NULL) to the number of candidates removed.  Removing
+ * every candidate leaves an image with no k, which rs_abcd_write writes as
+ * an empty file and rs_plan_create rejects. */
+rs_status rs_abcd_remove(rs_abcd* a, const rs_factor* factors, int64_t count,
+                         int64_t* removed);
 void rs_abcd_free(rs_abcd* a);
 
 /* Synthetic input (no reference counterpart: the reference only reads ABCD

@@ -312,6 +312,38 @@ def test_cli_default_range_known_answer(tmp_path, gpu):
     assert line not in r.stdout
 
 
+def test_factor_removal_after_sieving(rs, orc, tmp_path, gpu):
+    """Dat mode: verified factors' candidates leave the ABCD; sieving the
+    rewritten file over the same primes finds none of its candidates, and
+    the CLI's -R writes the same file."""
+    abcd = rs.Abcd.synth(5, 10, 3, 10**5, 2000, 9000, 200, seed=4, odd_k=2)
+    before = abcd.info()["nn"]
+    path = tmp_path / "in.abcd"
+    abcd.write(path)
+    plan = rs.Plan(abcd)
+    low = 10**6 + 1
+    offs = rs.PrimeGen(low + 2 * 10**5).range(low, low + 2 * 10**5)
+    hits = run_device(rs, plan, low, offs, m=256)
+    assert hits == run_oracle(orc, plan, 256, low, offs)
+    found = [h for h in hits if plan.contains(h[1], h[2])]
+    assert found and all(rs.verify_factor(p, k, 5, n) for p, k, n in found)
+    gone = abcd.remove(found)
+    assert gone == len({(k, n) for _, k, n in found})
+    assert abcd.info()["nn"] == before - gone
+    plan2 = rs.Plan(abcd)
+    again = run_device(rs, plan2, low, offs, m=256)
+    assert not [h for h in again if plan2.contains(h[1], h[2])]
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+    out = tmp_path / "left.abcd"
+    r = subprocess.run([exe, "-i", str(path), "-p", str(low), "-P", str(low + 2 * 10**5 - 1),
+                        "-O", str(tmp_path / "f.txt"), "-m", "7", "-R", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert f"Removed {gone} candidates" in r.stdout
+    for x, y in zip(abcd.lists(), rs.Abcd.read(out).lists()):
+        assert np.array_equal(x, y)
+
+
 def test_batched_jobs_equal_alone_and_oracle(rs, orc, gpu):
     """Heterogeneous jobs (bases, k counts, n windows, m) in one launch: each
     job's hits equal its plain run and the oracle; rerun replays the graph."""

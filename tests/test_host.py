@@ -127,6 +127,28 @@ def test_primes_true_above_qmax_squared(rs):
         assert (v in set(vals)) == isp
 
 
+def test_factor_removal(rs, tmp_path):
+    a = rs.Abcd.from_lists(745, {18: [100, 118, 136], 40: [101, 119], 66: [104]})
+    # (18, 118) and (66, 104) go, (40, 7) is not a candidate, (18, 118) twice
+    gone = a.remove([(1009, 18, 118), (1013, 66, 104), (1019, 40, 7), (1021, 18, 118)])
+    assert gone == 2
+    ks, kstart, ns = a.lists()
+    assert ks.tolist() == [18, 40] and kstart.tolist() == [0, 2, 4]
+    assert ns.tolist() == [100, 136, 101, 119]
+    inf = a.info()
+    assert (inf["nk"], inf["nn"], inf["minN"], inf["maxN"]) == (2, 4, 100, 136)
+    path = tmp_path / "left.abcd"
+    a.write(path)
+    b = rs.Abcd.read(path)
+    for x, y in zip(a.lists(), b.lists()):
+        assert np.array_equal(x, y)
+    assert a.remove([]) == 0
+    assert a.remove([(3, 18, 100), (3, 18, 136), (3, 40, 101), (3, 40, 119)]) == 4
+    assert a.info()["nk"] == 0
+    with pytest.raises(rs.RsError):
+        rs.Plan(a)
+
+
 def test_synth_presieve_exact(rs):
     a = rs.Abcd.synth(745, 5, 3, 10**6, 10000, 12000, 1000, seed=11)
     ks, kstart, ns = a.lists()
