@@ -213,7 +213,7 @@ def main():
         import oracle
 
         rk, rr, _ = plan.rows()
-        job = oracle.Job(plan.base, plan.Q, sv.m, plan.minN, plan.maxN, rk, rr)
+        job = oracle.Job(plan.base, plan.Q, sv.m, plan.minN, plan.maxN, rk, rr, plan.c)
         n = 2000
         while True:
             ts = time.perf_counter()

@@ -102,6 +102,7 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.invT = magic_div(J.T);
     J.tw = tw;
     J.lazy = kvals.back() < (1u << 24);
+    J.plus = pl->c == 1;
     // flattened giant-step indices of a row slice (and their division by T)
     // stay below 2^16
     if ((long long)std::min<int>(nrows, ROW_SLICE) * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;

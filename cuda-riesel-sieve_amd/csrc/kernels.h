@@ -35,6 +35,7 @@ struct JobDev {
     int tw;         // giant-step tile: consecutive t of one row per lane, 1, 2 or 4
                     // (T is rounded up to a multiple of tw)
     int lazy;       // giant-step products left in [0, 2p): every k < 2^24
+    int plus;       // 1: k*b^n+1 (Sierpinski), 0: k*b^n-1
     int W;        // record words
     int off_gt;   // REC_BH + Q
     int off_j;    // off_gt + T: the row masks

@@ -286,7 +286,9 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     const u64 G = mont_pow(g64, (u32)(J.m >> 6), one, p, mp);
     const u64 H = mont_pow(g, (u32)(J.x0 + J.m - 1), one, p, mp);
     const u64 c32 = montmul(1ull << 32, r2, p, mp);  // Mont(2^32)
+    // k*b^n+1: solve (-k) b^r g^x == 1, i.e. negate the giant-step starts
     u64 bh = montmul(H, c32, p, mp);
+    if (J.plus && bh != 0) bh = p - bh;
     shoup_pair(B, p, mp, &w, &ws);
     for (int r = 0; r < J.Q; ++r) {
         R[REC_BH + r] = bh;
@@ -302,9 +304,11 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     const int jb = B == 0 ? 0
                           : jacobi_pre(J.bjac.x, J.bjac.y,
                                        ((u64)J.bjac.w << 32) | J.bjac.z, p);
-    // row masks: k (in order) adds its even-r rows if (k/p) = +1 and its
-    // odd-r rows if (kb/p) = +1 to words w0, w0 + 1; words are written once,
-    // in order, when no later k can reach them
+    // row masks: k (in order) adds its even-r rows if (k/p) = w and its
+    // odd-r rows if (kb/p) = w to words w0, w0 + 1, where w = +1, or (-1/p)
+    // for k*b^n+1 (-k b^r must be a square); words are written once, in
+    // order, when no later k can reach them
+    const int want = J.plus && (lo32(p) & 3) == 3 ? -1 : 1;
     const int nw = (J.nrows + 63) >> 6;
     u64* const RM = R + J.off_j;
     int cur = 0;
@@ -325,8 +329,8 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
         const uint4 kj = J.kjac[i];
         const int jkv = jb == 0 ? 0 : jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
         const u64* const km = J.kmask + 4 * i;
-        if (jkv == 1) lo |= km[0], hi |= km[1];
-        if (jkv * jb == 1) lo |= km[2], hi |= km[3];
+        if (jkv == want) lo |= km[0], hi |= km[1];
+        if (jkv * jb == want) lo |= km[2], hi |= km[3];
     }
     RM[cur] = lo;
     if (cur + 1 < nw) RM[cur + 1] = hi;

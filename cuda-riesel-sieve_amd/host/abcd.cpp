@@ -76,11 +76,17 @@ extern "C" rs_status rs_abcd_read(const char* path, rs_abcd** out) {
                 delete a;
                 return RS_ERR_FORMAT;
             }
-            // only Riesel (-1) sequences: "...^$a-1"
-            if (rest.find("-1", car) == std::string::npos || rest.find("-1", car) > lb) {
+            // "...^$a-1" (Riesel, the reference's form) or "...^$a+1"
+            // (Sierpinski); one form per file
+            const size_t mi = rest.find("$a-1", car), pl = rest.find("$a+1", car);
+            int c = 0;
+            if (mi != std::string::npos && mi < lb) c = -1;
+            if (pl != std::string::npos && pl < lb) c = c ? 0 : 1;
+            if (c == 0 || (have && c != a->c)) {
                 delete a;
                 return RS_ERR_FORMAT;
             }
+            a->c = c;
             long long k, b, n0;
             if (!parse_int(rest.substr(0, st).c_str(), &k) || k < 1 ||
                 !parse_int(rest.substr(st + 1, car - st - 1).c_str(), &b) || b < 2 ||
@@ -146,6 +152,18 @@ extern "C" rs_status rs_abcd_info(const rs_abcd* a, uint32_t* base, int* nk, int
     return RS_OK;
 }
 
+extern "C" rs_status rs_abcd_form(const rs_abcd* a, int* c) {
+    if (!a || !c) return RS_ERR_ARG;
+    *c = a->c;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_abcd_set_form(rs_abcd* a, int c) {
+    if (!a || (c != -1 && c != 1)) return RS_ERR_ARG;
+    a->c = c;
+    return RS_OK;
+}
+
 extern "C" rs_status rs_abcd_lists(const rs_abcd* a, int32_t* ks, int32_t* kstart,
                                    int32_t* ns) {
     if (!a) return RS_ERR_ARG;
@@ -161,7 +179,8 @@ extern "C" rs_status rs_abcd_write(const rs_abcd* a, const char* path) {
     if (!f) return RS_ERR_IO;
     for (size_t i = 0; i < a->ks.size(); ++i) {
         int s = a->kstart[i], e = a->kstart[i + 1];
-        fprintf(f, "ABCD %d*%u^$a-1 [%d]\n", a->ks[i], a->base, a->ns[s]);
+        fprintf(f, "ABCD %d*%u^$a%s [%d]\n", a->ks[i], a->base, a->c == 1 ? "+1" : "-1",
+                a->ns[s]);
         for (int j = s + 1; j < e; ++j) fprintf(f, "%d\n", a->ns[j] - a->ns[j - 1]);
     }
     return fclose(f) == 0 ? RS_OK : RS_ERR_IO;
@@ -239,11 +258,11 @@ uint32_t order_mod(uint64_t b, uint32_t q) {
 // k.  Work is split over q across threads, which mark one shared removal
 // byte per (k, n).
 extern "C" rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_t kmax,
-                                   int odd_k, int nmin, int nmax,
+                                   int odd_k, int c, int nmin, int nmax,
                                    uint32_t presieve_bound, uint64_t seed,
                                    rs_abcd** out) {
     if (!out || base < 2 || nk < 1 || kmin < 1 || kmax < kmin || nmin < 1 ||
-        nmax < nmin || kmax > 0x7fffffffu || odd_k < 0 || odd_k > 2)
+        nmax < nmin || kmax > 0x7fffffffu || odd_k < 0 || odd_k > 2 || (c != -1 && c != 1))
         return RS_ERR_ARG;
     *out = nullptr;
     const uint64_t span = (uint64_t)(kmax - kmin) + 1;
@@ -311,7 +330,9 @@ extern "C" rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_
                         const uint64_t k = batch[i] % q;
                         if (k == 0) continue;  // k*b^n-1 == -1 mod q
                         ++want;
+                        // q | k*b^n + c  <=>  b^n == -c * k^-1 (mod q)
                         uint64_t v = inv_mod(k, q);
+                        if (c == 1) v = q - v;
                         for (uint32_t j = 0; j < m; ++j, v = v * bq % q) {
                             uint32_t h = ((uint32_t)v * 2654435761u) >> (32 - hb);
                             while (hidx[h] != 0xffffffffu && hval[h] != (uint32_t)v)
@@ -372,6 +393,7 @@ extern "C" rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_
               [&](int x, int y) { return chosen[x] < chosen[y]; });
     rs_abcd* a = new rs_abcd();
     a->base = base;
+    a->c = c;
     for (int i : order) {
         a->kstart.push_back((int32_t)a->ns.size());
         a->ks.push_back((int32_t)chosen[i]);

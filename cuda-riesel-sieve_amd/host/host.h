@@ -8,6 +8,7 @@
 
 struct rs_abcd {
     uint32_t base = 0;
+    int c = -1;                   // k*b^n + c: -1 Riesel, +1 Sierpinski
     std::vector<int32_t> ks;      // one entry per header line
     std::vector<int32_t> kstart;  // CSR offsets into ns, size ks.size() + 1
     std::vector<int32_t> ns;      // strictly increasing per k
@@ -16,6 +17,7 @@ struct rs_abcd {
 
 struct rs_plan {
     uint32_t base = 0;
+    int c = -1;                   // form of the candidates, as in rs_abcd
     int Q = 0;
     int minN = 0, maxN = 0;
     int x0 = 0, x1 = 0;           // floor(minN/Q), floor(maxN/Q)

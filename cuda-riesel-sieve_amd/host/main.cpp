@@ -14,7 +14,8 @@
 // -p/-P the default test range of :837-839 is used.  The sieved range is
 // clipped to [P0, P1] (the reference runs past P1 with its last batch).
 // Output lines keep the reference's formats: factors "p | k*b^n-1"
-// (:1833) and a progress line every 20 launches (:1759).
+// (:1833; "+1" for a Sierpinski file) and a progress line every 20
+// launches (:1759).
 #include <getopt.h>
 
 #include <algorithm>
@@ -134,6 +135,9 @@ int main(int argc, char* argv[]) {
     rs_abcd_info(abcd, &base, &nk, &nn, &minN, &maxN);
     printf("The base is %u\n%d k-values, %d candidates\nMinN = %d and MaxN = %d\n", base, nk, nn,
            minN, maxN);
+    int form = -1;
+    rs_abcd_form(abcd, &form);
+    if (form == 1) printf("Sierpinski form k*b^n+1\n");
     rs_plan* plan = nullptr;
     if ((st = rs_plan_create(abcd, Qin, &plan)) != RS_OK) return fail("planning", st);
     int Q, nrows, words;
@@ -220,15 +224,16 @@ int main(int argc, char* argv[]) {
             const rs_factor& f = hits[i];
             if (verify) {
                 if (!rs_plan_contains(plan, f.k, f.n)) continue;              // :1620-1647
-                if (!rs_verify_factor(f.p, f.k, base, f.n)) {
-                    fprintf(stderr, "Error: device reported %llu | %u*%u^%u-1, which fails "
-                            "verification\n", (unsigned long long)f.p, f.k, base, f.n);
+                if (!rs_verify_factor(f.p, f.k, base, f.n, form)) {
+                    fprintf(stderr, "Error: device reported %llu | %u*%u^%u%s, which fails "
+                            "verification\n", (unsigned long long)f.p, f.k, base, f.n,
+                            form == 1 ? "+1" : "-1");
                     return 1;
                 }
             }
             if (removedFile) verified.push_back(f);
             char line[128];
-            rs_format_factor(&f, base, line, sizeof line);
+            rs_format_factor(&f, base, form, line, sizeof line);
             fputs(line, stdout);
             if (!ff) ff = fopen(factorFile, "a");
             if (!ff) {

@@ -44,6 +44,7 @@ extern "C" rs_status rs_plan_create(const rs_abcd* a, int Qin, rs_plan** out) {
     const int Q = bestQ ? bestQ : bestEvenQ;
     rs_plan* pl = new rs_plan();
     pl->base = a->base;
+    pl->c = a->c;
     pl->Q = Q;
     pl->minN = minN;
     pl->maxN = maxN;
@@ -74,6 +75,12 @@ extern "C" rs_status rs_plan_create(const rs_abcd* a, int Qin, rs_plan** out) {
             pl->bitmaps[row * pl->words + xp / 32] |= 1u << (31 - (xp & 31));
         }
     *out = pl;
+    return RS_OK;
+}
+
+extern "C" rs_status rs_plan_form(const rs_plan* pl, int* c) {
+    if (!pl || !c) return RS_ERR_ARG;
+    *c = pl->c;
     return RS_OK;
 }
 

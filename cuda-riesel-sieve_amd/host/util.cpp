@@ -68,16 +68,18 @@ int hw_threads(int requested) {
 
 // kernel_v0.24.6.cu:1656-1671: b^n mod p by square-and-multiply in 128-bit,
 // then k*b^n == 1 (mod p).
-extern "C" int rs_verify_factor(uint64_t p, uint32_t k, uint32_t b, uint32_t n) {
-    if (p < 2) return 0;
-    return rsh::mulmod(rsh::powmod(b, n, p), k % p, p) == 1 % p ? 1 : 0;
+extern "C" int rs_verify_factor(uint64_t p, uint32_t k, uint32_t b, uint32_t n, int c) {
+    if (p < 2 || (c != -1 && c != 1)) return 0;
+    const uint64_t want = c == 1 ? p - 1 : 1 % p;  // k*b^n == -c (mod p)
+    return rsh::mulmod(rsh::powmod(b, n, p), k % p, p) == want ? 1 : 0;
 }
 
-// writeFactorToFile's line, kernel_v0.24.6.cu:1833: "%llu | %d*%d^%d-1\n".
-extern "C" int rs_format_factor(const rs_factor* f, uint32_t b, char* buf, int cap) {
-    if (!f || !buf || cap <= 0) return RS_ERR_ARG;
-    int n = snprintf(buf, (size_t)cap, "%llu | %u*%u^%u-1\n", (unsigned long long)f->p,
-                     f->k, b, f->n);
+// writeFactorToFile's line, kernel_v0.24.6.cu:1833: "%llu | %d*%d^%d-1\n"
+// (with "+1" for the Sierpinski form).
+extern "C" int rs_format_factor(const rs_factor* f, uint32_t b, int c, char* buf, int cap) {
+    if (!f || !buf || cap <= 0 || (c != -1 && c != 1)) return RS_ERR_ARG;
+    int n = snprintf(buf, (size_t)cap, "%llu | %u*%u^%u%s\n", (unsigned long long)f->p,
+                     f->k, b, f->n, c == 1 ? "+1" : "-1");
     if (n < 0) return RS_ERR_ARG;
     if (n >= cap) return RS_ERR_OVERFLOW;
     return n;

@@ -48,8 +48,11 @@ _SIGS = {
     "rs_abcd_write": (i32, [vp, ctypes.c_char_p]),
     "rs_abcd_remove": (i32, [vp, vp, i64, P(i64)]),
     "rs_abcd_free": (None, [vp]),
-    "rs_abcd_synth": (i32, [u32, i32, u32, u32, i32, i32, i32, u32, u64, P(vp)]),
+    "rs_abcd_synth": (i32, [u32, i32, u32, u32, i32, i32, i32, i32, u32, u64, P(vp)]),
+    "rs_abcd_form": (i32, [vp, P(i32)]),
+    "rs_abcd_set_form": (i32, [vp, i32]),
     "rs_plan_create": (i32, [vp, i32, P(vp)]),
+    "rs_plan_form": (i32, [vp, P(i32)]),
     "rs_plan_info": (i32, [vp, P(u32), P(i32), P(i32), P(i32), P(i32), P(i32)]),
     "rs_plan_rows": (i32, [vp, P(i32), P(i32), P(u32)]),
     "rs_plan_contains": (i32, [vp, u32, u32]),
@@ -63,8 +66,8 @@ _SIGS = {
     "rs_gpu_primegen_create": (i32, [i32, u64, P(vp)]),
     "rs_gpu_primes_range": (i32, [vp, u64, u64, vp, i64, P(i64)]),
     "rs_gpu_primegen_free": (None, [vp]),
-    "rs_verify_factor": (i32, [u64, u32, u32, u32]),
-    "rs_format_factor": (i32, [P(Factor), u32, ctypes.c_char_p, i32]),
+    "rs_verify_factor": (i32, [u64, u32, u32, u32, i32]),
+    "rs_format_factor": (i32, [P(Factor), u32, i32, ctypes.c_char_p, i32]),
     "rs_sieve_create": (i32, [i32, vp, i32, i32, i64, i64, P(vp)]),
     "rs_sieve_info": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32)]),
     "rs_sieve_submit": (i32, [vp, u64, P(u32), i64]),
@@ -173,12 +176,23 @@ class Abcd:
         return cls(h.value)
 
     @classmethod
-    def synth(cls, base, nk, kmin, kmax, nmin, nmax, presieve_bound, seed, odd_k=True):
-        """odd_k: True/1 odd k only, 2 even k only, False/0 any parity."""
+    def synth(cls, base, nk, kmin, kmax, nmin, nmax, presieve_bound, seed, odd_k=True, c=-1):
+        """odd_k: True/1 odd k only, 2 even k only, False/0 any parity;
+        c: -1 for k*b^n-1, +1 for k*b^n+1."""
         h = vp()
-        _chk("rs_abcd_synth", lib().rs_abcd_synth(base, nk, kmin, kmax, int(odd_k), nmin, nmax,
+        _chk("rs_abcd_synth", lib().rs_abcd_synth(base, nk, kmin, kmax, int(odd_k), c, nmin, nmax,
                                                   presieve_bound, seed, ctypes.byref(h)))
         return cls(h.value)
+
+    @property
+    def c(self):
+        """Form k*b^n + c of the candidates (-1 or +1)."""
+        v = i32()
+        _chk("rs_abcd_form", lib().rs_abcd_form(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_form(self, c):
+        _chk("rs_abcd_set_form", lib().rs_abcd_set_form(self.h, c))
 
     def remove(self, factors):
         """Drops the candidates (k, n) of factors [(p, k, n), ...]; returns how many."""
@@ -229,6 +243,9 @@ class Plan:
                                                 ctypes.byref(lo), ctypes.byref(hi)))
         self.base, self.Q, self.nrows, self.words = b.value, Q.value, nr.value, w.value
         self.minN, self.maxN = lo.value, hi.value
+        cc = i32()
+        _chk("rs_plan_form", lib().rs_plan_form(self.h, ctypes.byref(cc)))
+        self.c = cc.value
 
     def rows(self):
         rk = np.zeros(self.nrows, np.int32)
@@ -395,14 +412,14 @@ class GpuPrimeGen:
             self.h = vp()
 
 
-def verify_factor(p, k, b, n):
-    return bool(lib().rs_verify_factor(p, k, b, n))
+def verify_factor(p, k, b, n, c=-1):
+    return bool(lib().rs_verify_factor(p, k, b, n, c))
 
 
-def format_factor(p, k, b, n):
+def format_factor(p, k, b, n, c=-1):
     f = Factor(p, k, n)
     buf = ctypes.create_string_buffer(128)
-    r = lib().rs_format_factor(ctypes.byref(f), b, buf, 128)
+    r = lib().rs_format_factor(ctypes.byref(f), b, c, buf, 128)
     if r < 0:
         raise RsError("rs_format_factor", r)
     return buf.value.decode()

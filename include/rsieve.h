@@ -67,6 +67,12 @@ rs_status rs_abcd_from_lists(uint32_t base, int nk, const int32_t* ks,
                              rs_abcd** out);
 rs_status rs_abcd_info(const rs_abcd* a, uint32_t* base, int* nk, int* nn,
                        int* minN, int* maxN);
+/* The form k*b^n + c of the file: c = -1 (Riesel, the reference's only form,
+ * :1009-1016) or +1 (Sierpinski, "ABCD k*b^$a+1 [n0]" headers; a file mixing
+ * both is rejected by rs_abcd_read).  set_form changes it for images built
+ * from lists. */
+rs_status rs_abcd_form(const rs_abcd* a, int* c);
+rs_status rs_abcd_set_form(rs_abcd* a, int c);
 /* Copies the lists out (ks[nk], kstart[nk+1], ns[nn]). */
 rs_status rs_abcd_lists(const rs_abcd* a, int32_t* ks, int32_t* kstart,
                         int32_t* ns);
@@ -88,10 +94,11 @@ void rs_abcd_free(rs_abcd* a);
  * [kmin, kmax] (odd_k: 0 any parity, 1 odd only, 2 even only, as Riesel
  * k*b^n-1 with odd b needs even k) with a seeded splitmix64 stream, keeps
  * n in [nmin, nmax] such that no odd prime q < presieve_bound with q ∤ b
- * divides k*b^n-1, and redraws a k whose list comes out empty. */
+ * divides k*b^n+c (c = -1 or +1), and redraws a k whose list comes out
+ * empty. */
 rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_t kmax,
-                        int odd_k, int nmin, int nmax, uint32_t presieve_bound,
-                        uint64_t seed, rs_abcd** out);
+                        int odd_k, int c, int nmin, int nmax,
+                        uint32_t presieve_bound, uint64_t seed, rs_abcd** out);
 
 /* ------------------------------------------------------------ planner */
 /* Replaces :1086-1255.  Q is the even value in [2, 24) (or Qin, even) with
@@ -101,6 +108,8 @@ rs_status rs_abcd_synth(uint32_t base, int nk, uint32_t kmin, uint32_t kmax,
 typedef struct rs_plan rs_plan;
 
 rs_status rs_plan_create(const rs_abcd* a, int Qin, rs_plan** out);
+/* The plan's form c (-1 or +1), from its ABCD image. */
+rs_status rs_plan_form(const rs_plan* pl, int* c);
 rs_status rs_plan_info(const rs_plan* pl, uint32_t* base, int* Q, int* nrows,
                        int* words, int* minN, int* maxN);
 /* rows_k[nrows], rows_r[nrows], bitmaps[nrows*words] (any may be NULL). */
@@ -154,11 +163,12 @@ rs_status rs_gpu_primes_range(rs_gpu_primegen* g, uint64_t lo, uint64_t hi,
 void rs_gpu_primegen_free(rs_gpu_primegen* g);
 
 /* ------------------------------------------------------------ verification */
-/* 1 if p | k*b^n - 1 (128-bit square-and-multiply, :1656-1671). */
-int rs_verify_factor(uint64_t p, uint32_t k, uint32_t b, uint32_t n);
-/* Formats "p | k*b^n-1\n" (writeFactorToFile, :1833) into buf; returns the
- * length or a negative status. */
-int rs_format_factor(const rs_factor* f, uint32_t b, char* buf, int cap);
+/* 1 if p | k*b^n + c, c = -1 (Riesel; 128-bit square-and-multiply as at
+ * :1656-1671) or +1 (Sierpinski). */
+int rs_verify_factor(uint64_t p, uint32_t k, uint32_t b, uint32_t n, int c);
+/* Formats "p | k*b^n-1\n" (writeFactorToFile, :1833; "+1" for c = +1) into
+ * buf; returns the length or a negative status. */
+int rs_format_factor(const rs_factor* f, uint32_t b, int c, char* buf, int cap);
 
 /* ------------------------------------------------------------ device sieve */
 /* One sieve job on one GPU: base, Q, table size m (multiple of 64, or 0 for

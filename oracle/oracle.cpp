@@ -99,10 +99,12 @@ u64 or_powmod(u64 b, u64 e, u64 p) {
     return (u64)r;
 }
 
-// 1 if p | k*b^n - 1.  kernel_v0.24.6.cu:1656-1671.
-int or_verify(u64 p, u64 k, u64 b, u64 n) {
+// 1 if p | k*b^n + c, c = -1 (Riesel, kernel_v0.24.6.cu:1656-1671) or +1
+// (Sierpinski).
+int or_verify(u64 p, u64 k, u64 b, u64 n, int c) {
     u128 v = (u128)or_powmod(b, n, p) * (k % p) % p;
-    return v == 1 % p ? 1 : 0;
+    const u64 want = c == 1 ? (p - 1) % p : 1 % p;  // k*b^n == -c
+    return v == want ? 1 : 0;
 }
 
 // ---------------------------------------------------------------- ABCD input
@@ -254,6 +256,7 @@ struct OrJob {
     int nrows;
     const int* rows_k;   // sorted by (k, r)
     const int* rows_r;
+    int c;               // -1: k*b^n-1 (Riesel), +1: k*b^n+1 (Sierpinski)
 };
 
 static bool hit_less(const OrHit& a, const OrHit& b) {
@@ -276,8 +279,12 @@ static bool hit_less(const OrHit& a, const OrHit& b) {
 //   x = (t+1)m - j, n = Q x + r (:534-621).
 // tLo = floor((x0-1)/m) (0 if x0 == 0) and tHi = ceil(x1/m) cover x in
 // [x0, x1]; x = 0 (only when minN < Q) is tested directly.  Hits with n
-// outside [minN, maxN] are dropped (:1606).  Precondition: p odd, p < 2^63,
-// p does not divide b.  Returns the number of hits (all written if <= cap).
+// outside [minN, maxN] are dropped (:1606).  For c = +1 (k*b^n == -1) k is
+// negated before the product with beta, the one-line change :505-506 notes,
+// and the filter compares the Jacobi symbols with (-1/p) instead of +1
+// (-k*b^r must be a square; the reference's unchanged filter would skip
+// solvable rows for p = 3 mod 4).  Precondition: p odd, p < 2^63, p does
+// not divide b.  Returns the number of hits (all written if <= cap).
 int or_bsgs_prime(const OrJob* J, u64 p, OrHit* out, int cap,
                   long long* montmuls) {
     long long mm = 0;
@@ -306,6 +313,7 @@ int or_bsgs_prime(const OrJob* J, u64 p, OrHit* out, int cap,
     u64 beta = one;
     for (int t = 0; t < tLo; ++t) { beta = or_montmul(beta, c1, p, mp); ++mm; }
     const int jb = or_jacobi(J->base, p);
+    const int want = J->c == 1 ? ((p & 3) == 1 ? 1 : -1) : 1;  // (-1/p) for +1
     int nh = 0;
     int lastk = -1, prevRem = 0;
     bool j1 = false, j2 = false;
@@ -315,9 +323,10 @@ int or_bsgs_prime(const OrJob* J, u64 p, OrHit* out, int cap,
         if (k != lastk) {
             lastk = k;
             int jk = or_jacobi((u64)k, p);
-            j1 = jk == 1;
-            j2 = jk * jb == 1;
+            j1 = jk == want;
+            j2 = jk * jb == want;
             kM = or_modul64((u64)k % p, 0, p);
+            if (J->c == 1 && kM != 0) kM = p - kM;
             fixedBeta = or_montmul(kM, beta, p, mp); ++mm;
             prevRem = 0;
         }
@@ -357,9 +366,10 @@ int or_bsgs_prime(const OrJob* J, u64 p, OrHit* out, int cap,
 
 // Exhaustive check: every row, every x in [x0, x1], plain 128-bit products,
 // no Jacobi filter and no table.  Defines the complete datless factor set
-// {(p,k,n) : n in [minN,maxN], (k, n mod Q) a row, k*b^n == 1 mod p}.
+// {(p,k,n) : n in [minN,maxN], (k, n mod Q) a row, k*b^n == -c mod p}.
 int or_brute_prime(const OrJob* J, u64 p, OrHit* out, int cap) {
     const int Q = J->Q;
+    const u64 want = J->c == 1 ? (p - 1) % p : 1 % p;
     const int x0 = J->minN / Q, x1 = J->maxN / Q;
     const u64 bQ = or_powmod(J->base, Q, p);
     int nh = 0;
@@ -369,7 +379,7 @@ int or_brute_prime(const OrJob* J, u64 p, OrHit* out, int cap) {
         u64 v = (u64)((u128)(k % p) * or_powmod(J->base, (u64)Q * x0 + r, p) % p);
         for (int x = x0; x <= x1; ++x) {
             long long n = (long long)Q * x + r;
-            if (v == 1 % p && n >= J->minN && n <= J->maxN) {
+            if (v == want && n >= J->minN && n <= J->maxN) {
                 if (nh < cap) out[nh] = {p, (u32)k, (u32)n};
                 ++nh;
             }

@@ -26,7 +26,7 @@ class OrHit(ctypes.Structure):
 class OrJob(ctypes.Structure):
     _fields_ = [("base", u32), ("Q", i32), ("m", i32), ("minN", i32),
                 ("maxN", i32), ("nrows", i32), ("rows_k", P(i32)),
-                ("rows_r", P(i32))]
+                ("rows_r", P(i32)), ("c", i32)]
 
 
 _lib = None
@@ -44,7 +44,7 @@ def lib():
             "or_montmul": (u64, [u64, u64, u64, u64]),
             "or_jacobi": (i32, [u64, u64]),
             "or_powmod": (u64, [u64, u64, u64]),
-            "or_verify": (i32, [u64, u64, u64, u64]),
+            "or_verify": (i32, [u64, u64, u64, u64, i32]),
             "or_abcd_count": (i32, [ctypes.c_char_p, P(i32), P(i32)]),
             "or_abcd_read": (i32, [ctypes.c_char_p, P(u32), P(i32), P(i32), P(i32),
                                    P(i32), P(i32)]),
@@ -78,7 +78,9 @@ mprime = lambda p: lib().or_mprime(p)
 montmul = lambda a, b, p, mp: lib().or_montmul(a, b, p, mp)
 jacobi = lambda a, n: lib().or_jacobi(a, n)
 powmod = lambda b, e, p: lib().or_powmod(b, e, p)
-verify = lambda p, k, b, n: lib().or_verify(p, k, b, n)
+def verify(p, k, b, n, c=-1):
+    """1 if p | k*b^n + c (c = -1 Riesel, +1 Sierpinski)."""
+    return lib().or_verify(p, k, b, n, c)
 inv128 = lambda M: lib().or_inv128(M)
 test_array_factor = lambda s: lib().or_test_array_factor(s)
 
@@ -131,11 +133,11 @@ def tminmax(minN, maxN, Q, m):
 class Job:
     """Keeps the numpy row arrays alive behind an OrJob struct."""
 
-    def __init__(self, base, Q, m, minN, maxN, rows_k, rows_r):
+    def __init__(self, base, Q, m, minN, maxN, rows_k, rows_r, c=-1):
         self.rows_k = np.ascontiguousarray(rows_k, np.int32)
         self.rows_r = np.ascontiguousarray(rows_r, np.int32)
         self.s = OrJob(base, Q, m, minN, maxN, len(self.rows_k),
-                       _ptr(self.rows_k, i32), _ptr(self.rows_r, i32))
+                       _ptr(self.rows_k, i32), _ptr(self.rows_r, i32), c)
 
 
 def _hits(buf, n):

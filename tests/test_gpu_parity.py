@@ -55,7 +55,7 @@ def run_device(rs, plan, low, offs, m=0, max_batch=1 << 16, device=0):
 
 def run_oracle(orc, plan, m, low, offs):
     rk, rr, _ = plan.rows()
-    job = orc.Job(plan.base, plan.Q, m, plan.minN, plan.maxN, rk, rr)
+    job = orc.Job(plan.base, plan.Q, m, plan.minN, plan.maxN, rk, rr, plan.c)
     hits, _ = orc.bsgs_many(job, low, offs, threads=16)
     return hits
 
@@ -153,7 +153,7 @@ def test_tiny_primes_vs_exhaustive(rs, orc, gpu):
     rk, rr, _ = plan.rows()
     for m in (64, 192, 1024):
         got = run_device(rs, plan, low, offs, m=m)
-        job = orc.Job(plan.base, plan.Q, m, plan.minN, plan.maxN, rk, rr)
+        job = orc.Job(plan.base, plan.Q, m, plan.minN, plan.maxN, rk, rr, plan.c)
         brute = []
         for q in primes:
             brute += orc.brute_prime(job, q)
@@ -170,6 +170,73 @@ def test_tiny_primes_vs_exhaustive(rs, orc, gpu):
                 if n % plan.Q == r and k * pow(plan.base, n, q) % q == 1:
                     want.append((q, k, n))
     assert small == sorted(want)
+
+
+def test_sierpinski_tiny_primes_vs_exhaustive(rs, orc, gpu):
+    """k*b^n+1 on small primes (both residues mod 4, orders below m, p | k):
+    device = exhaustive search = oracle BSGS."""
+    a = rs.Abcd.from_lists(6, {5: list(range(100, 700, 3)), 11: list(range(101, 650, 2)),
+                               12: [100, 150, 200, 640], 35: list(range(300, 700))})
+    a.set_form(1)
+    plan = rs.Plan(a)
+    assert plan.c == 1
+    primes = [q for q in range(5, 40000, 2) if is_prime(q)]
+    low = 3
+    offs = np.array([q - low for q in primes], np.uint32)
+    rk, rr, _ = plan.rows()
+    for m in (64, 192):
+        got = run_device(rs, plan, low, offs, m=m)
+        job = orc.Job(plan.base, plan.Q, m, plan.minN, plan.maxN, rk, rr, 1)
+        brute = []
+        for q in primes:
+            brute += orc.brute_prime(job, q)
+        assert got == brute, f"m={m}"
+        assert got == run_oracle(orc, plan, m, low, offs)
+    assert got and all(rs.verify_factor(p, k, 6, n, 1) for p, k, n in got)
+
+
+def test_sierpinski_window_and_mixed_batch(rs, orc, gpu):
+    """A synthetic k*b^n+1 set over a window of p near 1e7 against the
+    oracle, and Riesel and Sierpinski jobs batched together = alone."""
+    sp = rs.Plan(rs.Abcd.synth(2, 40, 3, 10**6, 100000, 130000, 10**4, seed=31, c=1))
+    ri = rs.Plan(rs.Abcd.synth(2, 40, 3, 10**6, 100000, 130000, 10**4, seed=31, c=-1))
+    assert sp.c == 1 and ri.c == -1
+    low = 10**7 + 1
+    offs = rs.PrimeGen(low + 2**20).range(low, low + 2**20)
+    want = run_oracle(orc, sp, 512, low, offs)
+    got = run_device(rs, sp, low, offs, m=512)
+    assert got == want and len(got) > 0
+    assert all(rs.verify_factor(p, k, 2, n, 1) for p, k, n in got)
+    bat = rs.Batch([sp, ri], [512, 384])
+    dev = rs.DeviceBuffer(0, offs)
+    bat.add_chunk(0, low, dev)
+    bat.add_chunk(1, low, dev)
+    bat.run()
+    res = bat.collect()
+    assert [(p, k, n) for j, p, k, n in res if j == 0] == got
+    assert [(p, k, n) for j, p, k, n in res if j == 1] == run_oracle(orc, ri, 384, low, offs)
+    bat.close()
+
+
+def test_cli_sierpinski_file(tmp_path, gpu):
+    import pyrsieve as rsmod
+    a = rsmod.Abcd.synth(2, 12, 3, 10**5, 2000, 6000, 300, seed=8, c=1)
+    path = tmp_path / "s.abcd"
+    a.write(path)
+    assert "^$a+1 [" in path.read_text().splitlines()[0]
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+    out = tmp_path / "f.txt"
+    r = subprocess.run([exe, "-i", str(path), "-p", "1000001", "-P", "1100000", "-O", str(out),
+                        "-m", "7"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = out.read_text().splitlines()
+    assert lines and all(l.endswith("+1") for l in lines)
+    for l in lines:
+        p, rest = l.split(" | ")
+        k, rest = rest.split("*")
+        b, rest = rest.split("^")
+        n = rest[:-2]
+        assert (int(k) * pow(int(b), int(n), int(p)) + 1) % int(p) == 0
 
 
 def test_config1_full(rs, orc, gpu):

@@ -71,11 +71,54 @@ def test_reader_errors(rs, tmp_path):
     assert e.value.status == rs.RS_ERR_IO
     bad = tmp_path / "bad.abcd"
     for text in ("12\n", "ABCD 5*2^$a-1 [10]\nxyz\n", "ABCD 5*2^$a-1\n",
-                 "ABCD 5*2^$a+1 [10]\n3\n", "ABCD 5*2^$a-1 [10]\n0\n"):
+                 "ABCD 5*2^$a-1 [10]\n0\n", "ABCD 5*2^$a-1 [10]\nABCD 7*2^$a+1 [10]\n",
+                 "ABCD 5*2^$a*1 [10]\n"):
         bad.write_text(text)
         with pytest.raises(rs.RsError) as e:
             rs.Abcd.read(bad)
         assert e.value.status == rs.RS_ERR_FORMAT, text
+
+
+def test_sierpinski_form_read_write(rs, tmp_path):
+    src = tmp_path / "s.abcd"
+    src.write_text("ABCD 27*2^$a+1 [100]\n4\n10\nABCD 33*2^$a+1 [101]\n6\n")
+    a = rs.Abcd.read(src)
+    assert a.c == 1
+    ks, kstart, ns = a.lists()
+    assert ks.tolist() == [27, 33] and ns.tolist() == [100, 104, 114, 101, 107]
+    out = tmp_path / "o.abcd"
+    a.write(out)
+    assert out.read_text() == src.read_text()
+    assert rs.Plan(a).c == 1 and rs.Abcd.read(R745).c == -1
+    b = rs.Abcd.from_lists(2, {27: [100, 104]})
+    assert b.c == -1
+    b.set_form(1)
+    assert b.c == 1 and rs.Plan(b).c == 1
+
+
+def test_sierpinski_verify_and_format(rs, orc):
+    # 3*2^5+1 = 97 is prime; 5 | 3*2^3+1 = 25
+    assert rs.verify_factor(97, 3, 2, 5, 1) and not rs.verify_factor(97, 3, 2, 5, -1)
+    assert rs.verify_factor(5, 3, 2, 3, 1) and orc.verify(5, 3, 2, 3, 1)
+    assert rs.format_factor(5, 3, 2, 3, 1) == "5 | 3*2^3+1\n"
+    rng = random.Random(9)
+    for _ in range(500):
+        p = rng.getrandbits(40) | 1
+        k, b, n = rng.getrandbits(31), rng.randint(2, 2000), rng.getrandbits(16)
+        want = (k * pow(b, n, p) + 1) % p == 0
+        assert rs.verify_factor(p, k, b, n, 1) == want == bool(orc.verify(p, k, b, n, 1))
+
+
+def test_synth_presieve_exact_sierpinski(rs):
+    a = rs.Abcd.synth(2, 6, 3, 10**6, 3000, 4500, 2000, seed=7, c=1)
+    assert a.c == 1
+    ks, kstart, ns = a.lists()
+    qs = [q for q in range(3, 2000, 2) if all(q % d for d in range(3, int(q**0.5) + 1, 2))]
+    for i, k in enumerate(ks.tolist()):
+        kept = set(ns[kstart[i]:kstart[i + 1]].tolist())
+        for n in range(3000, 4501, 7):  # a sample of n
+            survives = all((k * pow(2, n, q) + 1) % q for q in qs)
+            assert (n in kept) == survives, (k, n)
 
 
 def test_plan_rejects_odd_q(rs):

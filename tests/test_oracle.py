@@ -184,6 +184,36 @@ def test_bsgs_small_vs_python_pow(orc):
         assert orc.bsgs_prime(job, p)[0] == want
 
 
+def test_bsgs_sierpinski_equals_bruteforce_and_pow(orc):
+    """k*b^n+1: the oracle's BSGS (negated k, Jacobi symbols against (-1/p))
+    = exhaustive search = Python pow, over primes 1 and 3 mod 4, and the
+    Riesel and Sierpinski factor sets of the same rows differ."""
+    rng = random.Random(16)
+    for trial in range(4):
+        base = rng.choice([2, 3, 5, 10])
+        ks = sorted(rng.sample(range(1, 300), 3))
+        Q = rng.choice([2, 4, 6])
+        rows = [(k, r) for k in ks for r in sorted(rng.sample(range(Q), rng.randint(1, Q)))]
+        rows_k, rows_r = [k for k, _ in rows], [r for _, r in rows]
+        minN, maxN = rng.randint(1, 100), rng.randint(200, 900)
+        for m in (1, 32, 128):
+            job = orc.Job(base, Q, m, minN, maxN, rows_k, rows_r, 1)
+            riesel = orc.Job(base, Q, m, minN, maxN, rows_k, rows_r, -1)
+            differ = False
+            for p in small_primes(1500)[1:]:
+                p = int(p)
+                if base % p == 0:
+                    continue
+                got, _ = orc.bsgs_prime(job, p)
+                assert got == orc.brute_prime(job, p), (trial, m, p)
+                if m == 32 and p < 400:
+                    want = sorted((p, k, n) for k, r in rows for n in range(minN, maxN + 1)
+                                  if n % Q == r and (k * pow(base, n, p) + 1) % p == 0)
+                    assert got == want
+                differ |= got != orc.bsgs_prime(riesel, p)[0]
+            assert differ
+
+
 def test_datless_known_answer(orc):
     d = orc.read_abcd(R745)
     pl = orc.plan(d["ks"], d["kstart"], d["ns"], d["minN"], d["maxN"])
