@@ -16,6 +16,7 @@ extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev*
                                            unsigned long long* work, HitDev* hits,
                                            uint32_t* hit_count, uint32_t hit_cap);
 extern "C" __global__ void rs_zero_u32(uint32_t*, int);
+extern "C" __global__ void rs_jacobi_table(uint32_t n, uint32_t words, uint32_t* out);
 
 namespace rsdev {
 
@@ -25,6 +26,8 @@ const int kMaxWavesPerBlock = 4;
 struct DevJob {
     JobDev J{};
     uint4* d_kvals = nullptr;    // Jacobi constants of each distinct k
+    uint32_t* d_kjt = nullptr;   // quadratic-character table offsets of each k
+    uint32_t* d_jtab = nullptr;  // the tables
     uint32_t* d_kw0 = nullptr;   // row-mask words of each k
     uint64_t* d_kmask = nullptr;
     uint32_t* d_rowinfo = nullptr;

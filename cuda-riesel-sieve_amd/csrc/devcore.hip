@@ -24,11 +24,15 @@ extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev*
                                            unsigned long long* work, HitDev* hits,
                                            uint32_t* hit_count, uint32_t hit_cap);
 extern "C" __global__ void rs_zero_u32(uint32_t*, int);
+extern "C" __global__ void rs_jacobi_table(uint32_t n, uint32_t words, uint32_t* out);
 
 namespace rsdev {
 
 void free_job(DevJob& j) {
     (void)hipFree(j.d_kvals);
+    (void)hipFree(j.d_kjt);
+    (void)hipFree(j.d_jtab);
+    j.d_kjt = j.d_jtab = nullptr;
     (void)hipFree(j.d_kw0);
     (void)hipFree(j.d_kmask);
     j.d_kw0 = nullptr;
@@ -40,10 +44,38 @@ void free_job(DevJob& j) {
     j.d_rowinfo = j.d_rowk = j.d_bitmap = nullptr;
 }
 
+// Word offsets of the quadratic-character tables of the distinct k (~0u
+// for k without one) and their total size in words.
+static uint64_t jacobi_table_layout(const std::vector<uint32_t>& kvals,
+                                    std::vector<uint32_t>* offs) {
+    uint64_t total = 0;
+    offs->assign(kvals.size(), 0xffffffffu);
+    for (size_t i = 0; i < kvals.size(); ++i) {
+        uint32_t n = kvals[i];
+        while ((n & 1) == 0) n >>= 1;
+        if (n < 3 || n > JTAB_MAX_N) continue;
+        const uint64_t w = (n + 31) / 32;
+        if (total + w > JTAB_MAX_WORDS) continue;
+        (*offs)[i] = (uint32_t)total;
+        total += w;
+    }
+    return total;
+}
+
+static std::vector<uint32_t> distinct_k(const rs_plan* pl) {
+    std::vector<uint32_t> kvals;
+    for (int32_t k : pl->rows_k)
+        if (kvals.empty() || kvals.back() != (uint32_t)k) kvals.push_back((uint32_t)k);
+    return kvals;
+}
+
 size_t job_device_bytes(const rs_plan* pl, bool dat) {
     const size_t rows = pl->rows_k.size();
-    // row tables, and per k (at most one per row) Jacobi constants and masks
-    return 8 * rows + 52 * rows + (dat ? 4 * rows * (size_t)pl->words : 0) + 256;
+    std::vector<uint32_t> offs;
+    const uint64_t tab = jacobi_table_layout(distinct_k(pl), &offs);
+    // row tables, per k (at most one per row) Jacobi constants, masks and
+    // table offsets, the tables, and the n-list bitmaps with dat
+    return 8 * rows + 56 * rows + 4 * tab + (dat ? 4 * rows * (size_t)pl->words : 0) + 256;
 }
 
 // Jacobi constants of a 32-bit a > 0: odd part n, 2-adic valuation z and
@@ -132,6 +164,28 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     HIPCHK(hipMemcpy(dj->d_rowinfo, rowinfo.data(), rowinfo.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dj->d_rowk, rowk.data(), rowk.size() * 4, hipMemcpyHostToDevice));
     J.kjac = dj->d_kvals;
+    {
+        std::vector<uint32_t> toffs;
+        const uint64_t words = jacobi_table_layout(kvals, &toffs);
+        HIPCHK(hipMalloc(&dj->d_kjt, toffs.size() * 4));
+        HIPCHK(hipMemcpy(dj->d_kjt, toffs.data(), toffs.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&dj->d_jtab, std::max<uint64_t>(words, 1) * 4));
+        hipStream_t bs = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+        for (size_t i = 0; i < kvals.size(); ++i) {
+            if (toffs[i] == 0xffffffffu) continue;
+            uint32_t n = kvals[i];
+            while ((n & 1) == 0) n >>= 1;
+            const uint32_t w = (n + 31) / 32;
+            hipLaunchKernelGGL(rs_jacobi_table, dim3((w + 255) / 256), dim3(256), 0, bs, n, w,
+                               dj->d_jtab + toffs[i]);
+        }
+        const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(bs);
+        (void)hipStreamDestroy(bs);
+        if (e1 != hipSuccess || e2 != hipSuccess) return RS_ERR_HIP;
+        J.kjt = dj->d_kjt;
+        J.jtab = dj->d_jtab;
+    }
     HIPCHK(hipMalloc(&dj->d_kw0, kw0.size() * 4));
     HIPCHK(hipMalloc(&dj->d_kmask, kmask.size() * 8));
     HIPCHK(hipMemcpy(dj->d_kw0, kw0.data(), kw0.size() * 4, hipMemcpyHostToDevice));

@@ -43,6 +43,10 @@ struct JobDev {
     const uint4* kjac;        // per distinct k (ascending): odd part n, 2-adic
                               // valuation z, floor((2^64-1)/n) as lo, hi [nk]
     uint4 bjac;               // the same for the base
+    const uint32_t* kjt;      // per distinct k: word offset of its quadratic-
+                              // character table in jtab, or ~0u (no table)
+    const uint32_t* jtab;     // bit x of a k's table: (x/n) = +1, x < n, n the
+                              // odd part of k
     const uint32_t* kw0;      // per distinct k: 64-row word holding its first row
     const uint64_t* kmask;    // per distinct k: its even-r rows in words w0 and
                               // w0 + 1, then its odd-r rows in the same words
@@ -73,6 +77,10 @@ struct HitDev {
     uint32_t job;
     uint32_t pad;
 };
+
+// Quadratic-character tables: k whose odd part n is at most JTAB_MAX_N get
+// one, in order of k, while the job's tables stay within JTAB_MAX_WORDS.
+enum : uint32_t { JTAB_MAX_N = 1u << 28, JTAB_MAX_WORDS = 1u << 26 };
 
 static inline int rec_words(int Q, int T, int nrows) {
     return REC_BH + Q + T + (nrows + 63) / 64;

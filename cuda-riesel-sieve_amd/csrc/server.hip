@@ -144,6 +144,13 @@ static size_t footprint(const rs_server* s, const rs_plan& pl, bool dat) {
     return job_device_bytes(&pl, dat) + (size_t)s->buf_cap * 4;
 }
 
+extern "C" rs_status rs_server_job_bytes(const rs_server* s, const rs_plan* pl, int dat,
+                                        int64_t* bytes) {
+    if (!s || !pl || !bytes) return RS_ERR_ARG;
+    *bytes = (int64_t)footprint(s, *pl, dat != 0);
+    return RS_OK;
+}
+
 extern "C" rs_status rs_server_submit(rs_server* s, const rs_plan* pl, int m, int tile, int dat,
                                       uint64_t p_lo, uint64_t p_hi, double ns_per_prime,
                                       int64_t* job_id) {

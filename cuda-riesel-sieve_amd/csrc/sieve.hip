@@ -327,7 +327,27 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
             cur = w0;
         }
         const uint4 kj = J.kjac[i];
-        const int jkv = jb == 0 ? 0 : jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
+        const u32 tabw = J.kjt[i];
+        int jkv;
+        if (jb == 0) {
+            jkv = 0;
+        } else if (tabw != 0xffffffffu && p > kj.x) {
+            // (k/p) = (2/p)^z (n/p), (n/p) = (p mod n / n) (-1)^((n-1)/2 (p-1)/2);
+            // p > n is prime, so p mod n is a unit and the table bit decides
+            const u32 p8 = lo32(p) & 7;
+            int t = 1;
+            if ((kj.y & 1) && (p8 == 3 || p8 == 5)) t = -t;
+            if ((kj.x & 3) == 3 && (p8 & 3) == 3) t = -t;
+            const u64 rn = ((u64)kj.w << 32) | kj.z;
+            const u64 q = umulhi64(p, rn);
+            u64 r = p - mad32(lo32(q), kj.x, (u64)(hi32(q) * kj.x) << 32);
+            if (r >= kj.x) r -= kj.x;
+            if (r >= kj.x) r -= kj.x;
+            const u32 x = (u32)r;
+            jkv = (J.jtab[tabw + (x >> 5)] >> (x & 31)) & 1 ? t : -t;
+        } else {
+            jkv = jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
+        }
         const u64* const km = J.kmask + 4 * i;
         if (jkv == want) lo |= km[0], hi |= km[1];
         if (jkv * jb == want) lo |= km[2], hi |= km[3];
@@ -548,6 +568,20 @@ extern "C" __global__ void __launch_bounds__(256) rs_copy_probe(const uint4* __r
 // acc += *c (prime counts of device-generated chunks, for timing reports).
 extern "C" __global__ void rs_accum_count(const uint32_t* c, unsigned long long* acc) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *acc += *c;
+}
+
+// Quadratic-character table of an odd n > 1: bit x of the table is set iff
+// the Jacobi symbol (x/n) = +1, for x < n; one thread per 32-bit word.
+extern "C" __global__ void __launch_bounds__(256) rs_jacobi_table(uint32_t n, uint32_t words,
+                                                                  uint32_t* __restrict__ out) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    uint32_t bits = 0;
+    for (uint32_t i = 0; i < 32; ++i) {
+        const uint32_t x = w * 32 + i;
+        if (x < n && jacobi_loop(x, n, 1) == 1) bits |= 1u << i;
+    }
+    out[w] = bits;
 }
 
 extern "C" __global__ void rs_zero_u32(uint32_t* x, int n) {

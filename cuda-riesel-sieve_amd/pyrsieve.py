@@ -88,6 +88,7 @@ _SIGS = {
     "rs_batch_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
     "rs_batch_destroy": (None, [vp]),
     "rs_server_create": (i32, [i32, i64, i32, u64, i32, i64, ctypes.c_double, P(vp)]),
+    "rs_server_job_bytes": (i32, [vp, vp, i32, P(i64)]),
     "rs_server_submit": (i32, [vp, vp, i32, i32, i32, u64, u64, ctypes.c_double, P(i64)]),
     "rs_server_step": (i32, [vp, P(i32), P(i32), P(i64)]),
     "rs_server_poll": (i32, [vp, i64, P(i32), P(u64), P(i64)]),
@@ -320,6 +321,13 @@ class Server:
                                                         chunk_width, lds_cap, hit_capacity,
                                                         quantum_ms, ctypes.byref(h)))
         self.h = h
+
+    def job_bytes(self, plan, dat=False):
+        """Device bytes admission charges for a job of this plan."""
+        v = i64()
+        _chk("rs_server_job_bytes", lib().rs_server_job_bytes(self.h, plan.h, int(dat),
+                                                              ctypes.byref(v)))
+        return v.value
 
     def submit(self, plan, p_lo, p_hi, m=0, tile=0, dat=False, ns_per_prime=0.0):
         jid = i64()

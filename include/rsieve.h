@@ -295,6 +295,11 @@ rs_status rs_server_create(int device, int64_t hbm_budget, int max_active,
 /* Queues a job: the plan is copied; m/tile as for rs_sieve_create; dat != 0
  * keeps only n in the job's lists; primes p in [p_lo, p_hi).  ns_per_prime
  * is the job's measured kernel cost (rs_autotune), 0 if unknown. */
+/* Device bytes a job of this plan holds while admitted (row tables, Jacobi
+ * constants and character tables, n-list bitmaps with dat, one prime
+ * window buffer): what admission charges against hbm_budget. */
+rs_status rs_server_job_bytes(const rs_server* s, const rs_plan* pl, int dat,
+                              int64_t* bytes);
 rs_status rs_server_submit(rs_server* s, const rs_plan* pl, int m, int tile,
                            int dat, uint64_t p_lo, uint64_t p_hi,
                            double ns_per_prime, int64_t* job_id);

@@ -504,7 +504,10 @@ def test_server_results_alone_and_in_the_mix(rs, orc, gpu, quantum_ms):
     ranges = [(10**9 + 1 + i * width, 10**9 + 1 + i * width + (2 + i % 3) * width)
               for i in range(len(plans))]
     ranges[1] = ranges[0]  # two jobs on the same windows share their primes
-    per_job = 4 * (2 * width / np.log(width) + 64)  # each job's prime buffer
+    # a budget for about two jobs at a time (the largest footprint x 2.5)
+    probe = rs.Server(hbm_budget=1 << 40, max_active=1, chunk_width=width, lds_cap=16384)
+    per_job = max(probe.job_bytes(pl, dat=(i == 3)) for i, pl in enumerate(plans))
+    probe.close()
     srv = rs.Server(hbm_budget=int(2.5 * per_job), max_active=3, chunk_width=width,
                     lds_cap=16384, quantum_ms=quantum_ms)
     ids, pending = [], list(range(len(plans)))
