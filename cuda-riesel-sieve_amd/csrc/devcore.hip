@@ -140,7 +140,7 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     if ((long long)std::min<int>(nrows, ROW_SLICE) * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;
     J.W = rec_words(J.Q, J.T, nrows);
     J.off_gt = REC_BH + J.Q;
-    J.off_j = J.off_gt + J.T;
+    J.off_j = J.off_gt + 2 * J.T;
     J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.T, nrows, J.W);
     // per k: the 64-row word of its first row and its even-r / odd-r rows
     // in that word and the next (a k has at most Q < 64 rows)

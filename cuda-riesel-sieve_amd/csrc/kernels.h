@@ -10,8 +10,9 @@
 //   [6 .. 14)   g^c for c = 0..7          (g = b^Q)
 //   [14 .. 22)  g^(8a) for a = 0..7
 //   [22 .. 22+Q)        BH[r] = b^r * H * 2^32, H = g^(x0 + m - 1)
-//   [22+Q .. 22+Q+T)    GT[t] = G^t, G = g^m
-//   [22+Q+T .. +NW)     row masks: bit i of word w is set iff row 64w+i passes
+//   [22+Q .. 22+Q+2T)   G^t mod p (plain) and its Shoup constant, t < T,
+//                       G = g^m, interleaved
+//   [22+Q+2T .. +NW)    row masks: bit i of word w is set iff row 64w+i passes
 //                       the Jacobi filter, (k/p) = +1 for even r and
 //                       (k*b/p) = +1 for odd r (NW = ceil(nrows / 64))
 enum { REC_P = 0, REC_MP = 1, REC_ONE = 2, REC_G64 = 3, REC_GW = 4, REC_GWS = 5, REC_GP = 6,
@@ -38,7 +39,7 @@ struct JobDev {
     int plus;       // 1: k*b^n+1 (Sierpinski), 0: k*b^n-1
     int W;        // record words
     int off_gt;   // REC_BH + Q
-    int off_j;    // off_gt + T: the row masks
+    int off_j;    // off_gt + 2 T: the row masks
     int lds_words;  // 32-bit LDS words per wave (table, counts, Y, rows, record)
     const uint4* kjac;        // per distinct k (ascending): odd part n, 2-adic
                               // valuation z, floor((2^64-1)/n) as lo, hi [nk]
@@ -83,18 +84,18 @@ struct HitDev {
 enum : uint32_t { JTAB_MAX_N = 1u << 28, JTAB_MAX_WORDS = 1u << 26 };
 
 static inline int rec_words(int Q, int T, int nrows) {
-    return REC_BH + Q + T + (nrows + 63) / 64;
+    return REC_BH + Q + 2 * T + (nrows + 63) / 64;
 }
 
 // LDS layout of one wavefront, in 32-bit words from its base:
 //   [0, 4 NB)                 table: NB buckets x 8 uint16 fingerprints (0 = empty)
-//   [4 NB, 4 NB + NB/2)       bucket fill counts, two uint16 per word
+//   [4 NB, 5 NB)              bucket fill counts, one uint32 per bucket
 //   then 2 Q T                Y products (uint64)
 //   then 2 min(nrows, ROW_SLICE)  active rows of one slice (k, 8 r T | row << 19)
 //   then 2 W                  the prime's record (uint64)
 static inline int lds_words_per_wave(int logNB, int Q, int T, int nrows, int W) {
     const int NB = 1 << logNB;
     const int list_rows = nrows < ROW_SLICE ? nrows : ROW_SLICE;  // one row slice
-    int w = 4 * NB + NB / 2 + 2 * Q * T + 2 * list_rows + 2 * W;
+    int w = 5 * NB + 2 * Q * T + 2 * list_rows + 2 * W;
     return (w + 3) & ~3;
 }

@@ -70,17 +70,17 @@ __device__ __forceinline__ u64 montmul(u64 a, u64 b, u64 p, u64 mp) {
 }
 
 // k*y*2^-32 mod p before the final subtraction, in [0, 2p), for 32-bit k
-// and y < p: one 32-bit Montgomery step.  With k*y = a1*2^32 + lo(a0) and
-// m*p = m*hi(p)*2^32 + m*lo(p), the quotient (k*y + m*p)/2^32 is
-// a1 + m*hi(p) + hi(m*lo(p) + lo(a0)) (the low word of that last sum is 0).
-// It is < 2p < 2^64, so the partial sum a1 + m*hi(p) cannot overflow and
-// takes a1 as the 64-bit addend of one multiply-add.
+// and y < p: one 32-bit Montgomery step.  With a0 = k*lo(y), m = lo(a0)*mp
+// and s0 = m*lo(p) + lo(a0) (whose low word is 0), the quotient
+// (k*y + m*p)/2^32 is k*hi(y) + m*hi(p) + hi(a0) + hi(s0).  It is < 2p <
+// 2^64, so no partial sum overflows and the last two terms enter as the
+// 64-bit addend of the two multiply-adds.
 __device__ __forceinline__ u64 halfmul_lazy(u32 k, u64 y, u64 p, u32 mp0) {
     const u64 a0 = mad32(k, lo32(y), 0);
-    const u64 a1 = mad32(k, hi32(y), hi32(a0));
     const u32 m = lo32(a0) * mp0;
     const u64 s0 = mad32(m, lo32(p), lo32(a0));     // low word is 0
-    return mad32(m, hi32(p), a1) + hi32(s0);
+    // k*hi(y) + m*hi(p) + hi(a0) + hi(s0), every partial sum below 2p
+    return mad32(k, hi32(y), mad32(m, hi32(p), (u64)hi32(a0) + hi32(s0)));
 }
 
 // k*y*2^-32 mod p for 32-bit k and y < p.
@@ -107,9 +107,17 @@ __device__ __forceinline__ u64 shoup_mul(u64 x, u64 w, u64 ws, u64 p) {
     return csub(x * w - q * p, p);
 }
 
+// a * 2^-64 mod p (Montgomery reduction of a 64-bit a): with m = a*mp mod
+// 2^64, a + m*p == 0 mod 2^64, so (a + m*p) / 2^64 = hi64(m*p) + (a != 0),
+// which is at most p.
+__device__ __forceinline__ u64 redc(u64 a, u64 p, u64 mp) {
+    const u64 r = umulhi64(a * mp, p) + (a != 0 ? 1 : 0);
+    return r == p ? 0 : r;
+}
+
 // The Shoup pair (w, ws) of a Montgomery form wm.
 __device__ __forceinline__ void shoup_pair(u64 wm, u64 p, u64 mp, u64* w, u64* ws) {
-    *w = montmul(wm, 1, p, mp);
+    *w = redc(wm, p, mp);
     *ws = wm * mp;
 }
 

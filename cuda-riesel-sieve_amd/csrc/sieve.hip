@@ -63,6 +63,28 @@ __device__ __forceinline__ int div_T(u32 idx, u32 invT) {
 // low word, the 16-bit fingerprint is its low 15 bits with the top bit set,
 // so it is never 0, the empty-slot value.
 __device__ __forceinline__ u32 bucket_of(u64 v, u32 nbm) { return (lo32(v) >> 15) & nbm; }
+// The same by bit-field extract of logNB bits (one instruction).
+__device__ __forceinline__ u32 bucket_bfe(u64 v, u32 logNB) {
+    return __builtin_amdgcn_ubfe(lo32(v), 15, logNB);
+}
+
+// Nonzero iff some 16-bit half of x is zero (exact as a boolean: the
+// lowest zero half takes no borrow from below).
+__device__ __forceinline__ u32 has_zero_half(u32 x) {
+    return (x - 0x10001u) & ~x & 0x80008000u;
+}
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Nonzero iff one of the 8 16-bit slots of e equals the fingerprint in
+// both halves of fp2: the packed minimum of the XORs has a zero half.
+__device__ __forceinline__ u32 bucket_match(uint4 e, u32 fp2) {
+    const u16x2 a = __builtin_bit_cast(u16x2, e.x ^ fp2), b = __builtin_bit_cast(u16x2, e.y ^ fp2);
+    const u16x2 c = __builtin_bit_cast(u16x2, e.z ^ fp2), d = __builtin_bit_cast(u16x2, e.w ^ fp2);
+    const u16x2 mn = __builtin_elementwise_min(__builtin_elementwise_min(a, b),
+                                               __builtin_elementwise_min(c, d));
+    return has_zero_half(__builtin_bit_cast(u32, mn));
+}
 __device__ __forceinline__ u32 fingerprint(u64 v) { return (lo32(v) & 0x7fffu) | 0x8000u; }
 
 // Active-row entries of the giant steps: .x = k, .y = 8*r*T (byte offset of
@@ -71,11 +93,7 @@ __device__ __forceinline__ u32 row_entry(u32 r, u32 T, u32 rowrel) {
     return (r * T * 8u) | (rowrel << 19);
 }
 
-// Nonzero iff some 16-bit half of x is zero (exact as a boolean: the
-// lowest zero half takes no borrow from below).
-__device__ __forceinline__ u32 has_zero_half(u32 x) {
-    return (x - 0x10001u) & ~x & 0x80008000u;
-}
+
 
 // Emits every baby step j with g^j == target (all 64 lanes take part:
 // lane l checks j = l + 64 v) as a hit of giant step tt of the row whose
@@ -156,7 +174,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             s[u] = LAZY ? halfmul_lazy(row.x, yp[u], p, mp0) : halfmul(row.x, yp[u], p, mp0);
-            bk[u] = bucket_of(s[u], nbm);
+            bk[u] = bucket_bfe(s[u], (u32)J.logNB);
             const u32 fp = fingerprint(s[u]);
             fp2[u] = fp | (fp << 16);
         }
@@ -166,8 +184,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
         u64 anys = 0;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            const u32 z = has_zero_half(e[u].x ^ fp2[u]) | has_zero_half(e[u].y ^ fp2[u]) |
-                          has_zero_half(e[u].z ^ fp2[u]) | has_zero_half(e[u].w ^ fp2[u]);
+            const u32 z = bucket_match(e[u], fp2[u]);
             mm[u] = lanes(z != 0) & okm;
             slow[u] = 0;
             if (LAZY) {
@@ -190,9 +207,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
                     const u32 f2 = fp | (fp << 16);
                     while (true) {
                         const uint4 f = tab[b];
-                        const u32 z = has_zero_half(f.x ^ f2) | has_zero_half(f.y ^ f2) |
-                                      has_zero_half(f.z ^ f2) | has_zero_half(f.w ^ f2);
-                        if (z) hit = true;
+                        if (bucket_match(f, f2)) hit = true;
                         if (f.w < 0x10000u) break;
                         b = (b + 1) & nbm;
                     }
@@ -295,9 +310,12 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
         bh = shoup_mul(bh, w, ws, p);
     }
     shoup_pair(G, p, mp, &w, &ws);
-    u64 gt = one;
+    u64 gt = one;  // Montgomery form of G^t
     for (int t = 0; t < J.T; ++t) {
-        R[J.off_gt + t] = gt;
+        u64 gp, gs;
+        shoup_pair(gt, p, mp, &gp, &gs);
+        R[J.off_gt + 2 * t] = gp;
+        R[J.off_gt + 2 * t + 1] = gs;
         gt = shoup_mul(gt, w, ws, p);
     }
     // Jacobi bits; a prime dividing b gets none, so no row is searched
@@ -374,7 +392,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     uint4* const tab = (uint4*)wbase;
     uint16_t* const tab16 = (uint16_t*)wbase;
     u32* const cnt = wbase + 4 * NB;
-    u64* const Y = (u64*)(cnt + NB / 2);
+    u64* const Y = (u64*)(cnt + NB);
     uint2* const list = (uint2*)(Y + Q * T);
     u64* const R = (u64*)(list + min(J.nrows, ROW_SLICE));
     const int L = m >> 6;
@@ -387,7 +405,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
 
     // clear table and counts
     for (u32 b = lane; b < NB; b += 64) tab[b] = make_uint4(0, 0, 0, 0);
-    for (u32 c = lane; c < NB / 2; c += 64) cnt[c] = 0;
+    for (u32 c = 4 * lane; c < NB; c += 256) *(uint4*)(cnt + c) = make_uint4(0, 0, 0, 0);
     wave_sync();
 
     // baby steps: lane l inserts g^j, j = l + 64 s, as a fingerprint in the
@@ -398,24 +416,24 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     for (int s = 0; s < L; ++s) {
         u32 b = bucket_of(v, nbm);
         const u32 fp = fingerprint(v);
-        while (true) {
-            const u32 sh = (b & 1) * 16;
-            const u32 c = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
-            if (c < 8) {
-                tab16[b * 8 + c] = (uint16_t)fp;
-                break;
-            }
+        u32 c = atomicAdd(&cnt[b], 1u);
+        if (c >= 8) {  // rare: walk to the first bucket with a free slot
             ovf = true;
-            b = (b + 1) & nbm;
+            do {
+                b = (b + 1) & nbm;
+                c = atomicAdd(&cnt[b], 1u);
+            } while (c >= 8);
         }
+        tab16[b * 8 + c] = (uint16_t)fp;
         if (s + 1 < L) v = shoup_mul(v, gw, gws, p);
     }
     const bool chain = __ballot(ovf) != 0;
 
-    // Y[r*T + t] = b^r H G^t 2^32 (Montgomery form)
+    // Y[r*T + t] = b^r H G^t 2^32 (Montgomery form): BH[r] times the plain
+    // G^t by Shoup's method
     for (int idx = lane; idx < Q * T; idx += 64) {
         const int r = div_T((u32)idx, J.invT), t = idx - r * T;
-        Y[idx] = montmul(R[REC_BH + r], R[J.off_gt + t], p, mp);
+        Y[idx] = shoup_mul(R[REC_BH + r], R[J.off_gt + 2 * t], R[J.off_gt + 2 * t + 1], p);
     }
 
     // rows in slices of at most ROW_SLICE: those passing the Jacobi filter
