@@ -226,20 +226,31 @@ rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_wor
     const size_t per_wave = (size_t)lds_words * 4, shared = (size_t)shared_words * 4;
     if (lds_words < 1 || shared_words < 0 || per_wave + shared > (size_t)max_lds)
         return RS_ERR_ARG;
-    // waves of a block are independent (one table each); up to 4 per block
-    L->wpb = (int)std::min<size_t>(kMaxWavesPerBlock, ((size_t)max_lds - shared) / per_wave);
+    // waves of a block are independent (one table each); 1 to 4 per block,
+    // whichever keeps the most waves resident per CU (ties: more per block,
+    // so the block-shared part is loaded fewer times)
+    // the attribute is per function: always the device limit, so launches
+    // of different LDS sizes never restrict one another
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(rs_sieve_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+    int best_w = 0, best_wpb = 1, best_cu = 1;
+    for (int wpb = 1; wpb <= kMaxWavesPerBlock; ++wpb) {
+        const size_t bytes = per_wave * wpb + shared;
+        if (bytes > (size_t)max_lds) break;
+        int per_cu = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(rs_sieve_kernel), 64 * wpb, bytes));
+        if (per_cu * wpb >= best_w) {
+            best_w = per_cu * wpb;
+            best_wpb = wpb;
+            best_cu = per_cu;
+        }
+    }
+    L->wpb = best_wpb;
     L->lds_words = lds_words;
     L->shared_words = shared_words;
     L->lds_bytes = per_wave * L->wpb + shared;
-    // the attribute is per function: always the device limit, so launches
-    // of different LDS sizes never restrict one another
-    if (L->lds_bytes > 65536)
-        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(rs_sieve_kernel),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(rs_sieve_kernel), 64 * L->wpb, L->lds_bytes));
-    L->grid = std::max(1, cus * std::max(1, per_cu));
+    L->grid = std::max(1, cus * std::max(1, best_cu));
     return RS_OK;
 }
 
