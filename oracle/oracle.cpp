@@ -1,8 +1,8 @@
 // oracle.cpp -- CPU restatement of the reference Riesel BSGS sieve.
 //
 // TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls
-// this file: it is used by tests/, by __graft_entry__.smoke() as the checker,
-// and by bench.py's cpu_baseline leg.  It restates the algorithm of
+// this file: it is used by tests/, by __graft_entry__.smoke() to compare its
+// results, and by bench.py's cpu_baseline leg.  It restates the algorithm of
 // kernel_v0.24.6.cu (rob147147/CUDA-Riesel-Sieve v0.24.6) in plain C++ with
 // exact 128-bit arithmetic, applying these decisions on top of it:
 //   * exact REDC instead of the PTX montmul whose limb-1 carry can be lost

@@ -1,7 +1,7 @@
 """ctypes wrapper for liboracle.so, the CPU restatement of the reference sieve.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg; never by the product path.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+compare results and by bench.py's cpu_baseline leg; never by the product path.
 Each function mirrors one in oracle.cpp, which cites kernel_v0.24.6.cu.
 """
 import ctypes

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/oracle.cpp) before its results are trusted.
 
 Each oracle function is checked against something it does not compute
 itself: Python big-integer arithmetic, Euler's criterion, exhaustive search,
