@@ -72,6 +72,8 @@ def main():
                     help="Poisson arrival rate as a fraction of the plain job rate")
     ap.add_argument("--serve-active", type=int, default=16)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
+    ap.add_argument("--serve-sweep", type=str, default="",
+                    help="comma-separated loads for a Poisson rate sweep (e.g. 0.3,0.5,0.7,0.9)")
     ap.add_argument("--serve-quantum-ms", type=float, default=40.0,
                     help="device time per job per server step (0: whole 2^30 windows)")
     ap.add_argument("--no-batch", action="store_true")
@@ -520,6 +522,22 @@ def bench_serving(args, rs, p0, chunk):
                       "max_active": act}
     log(f"serving: poisson rate {rate:.2f}/s p50 {np.percentile(lat, 50):.2f}s "
         f"p99 {np.percentile(lat, 99):.2f}s")
+    if args.serve_sweep:
+        sweep = []
+        for load in [float(x) for x in args.serve_sweep.split(",") if x]:
+            rate_l = load * len(plans) / plain_s
+            arr_l = workloads.poisson_arrivals(len(plans), rate_l, seed=7)
+            lat, res, span, primes, act = serve_run(rs, plans, ms, tiles, ns, p0, p_hi, chunk,
+                                                    arr_l, args)
+            if res != plain_hits:
+                raise RuntimeError("served hits differ from the plain runs")
+            sweep.append({"load": load, "rate_jobs_per_s": rate_l,
+                          "latency_p50_s": float(np.percentile(lat, 50)),
+                          "latency_p99_s": float(np.percentile(lat, 99)),
+                          "aggregate_primes_per_sec": primes / span, "max_active": act})
+            log(f"serving sweep: load {load} p50 {np.percentile(lat, 50):.2f}s "
+                f"p99 {np.percentile(lat, 99):.2f}s")
+        out["sweep"] = sweep
     out["hits_equal_plain"] = True
     out["factors"] = sum(len(h) for h in plain_hits)
     out["ms"], out["tiles"] = ms, tiles
