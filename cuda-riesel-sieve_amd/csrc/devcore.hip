@@ -134,6 +134,7 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.invT = magic_div(J.T);
     J.tw = tw;
     J.lazy = kvals.back() < (1u << 24);
+    J.kmax = kvals.back();
     J.plus = pl->c == 1;
     // flattened giant-step indices of a row slice (and their division by T)
     // stay below 2^16

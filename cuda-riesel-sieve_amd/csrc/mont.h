@@ -83,6 +83,16 @@ __device__ __forceinline__ u64 halfmul_lazy(u32 k, u64 y, u64 p, u32 mp0) {
     return mad32(k, hi32(y), mad32(m, hi32(p), (u64)hi32(a0) + hi32(s0)));
 }
 
+// The low 32 bits of halfmul_lazy(k, y, p, mp0): bits 32..63 of
+// k*y + m*p, which are the high word of (k*lo(y) + m*lo(p)) mod 2^64 plus
+// the low words of k*hi(y) and m*hi(p).
+__device__ __forceinline__ u32 halfmul_lazy_lo(u32 k, u64 y, u64 p, u32 mp0) {
+    const u64 a0 = mad32(k, lo32(y), 0);
+    const u32 m = lo32(a0) * mp0;
+    const u64 x = mad32(m, lo32(p), a0);  // mod 2^64: only its high word is used
+    return hi32(x) + k * hi32(y) + m * hi32(p);
+}
+
 // k*y*2^-32 mod p for 32-bit k and y < p.
 __device__ __forceinline__ u64 halfmul(u32 k, u64 y, u64 p, u32 mp0) {
     return csub(halfmul_lazy(k, y, p, mp0), p);

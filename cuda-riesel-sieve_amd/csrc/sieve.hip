@@ -142,12 +142,14 @@ __device__ __forceinline__ u64 lanes(bool c) { return __builtin_amdgcn_ballot_w6
 // per lane and 64 runs per round.  A lane reads its row entry once and its
 // TW Y products at fixed offsets; each item is one 32x64 Montgomery
 // product s = k*Y[r][t] and one bucket read, all TW issued before any is
-// tested.  With LAZY the product is left in [0, 2p) (the case s >= p has
-// probability about k/2^33) and such items are re-probed after reduction;
-// with CHAIN (some bucket of this prime overflowed) an item whose bucket
-// is full also follows the chain to the next buckets.  Both go through the
-// slow path below, as does the exact check of a fingerprint match
-// (rescan()).  The loop is wave-uniform because rescan() needs every lane.
+// tested.  With LAZY (every k < 2^24) only the low word of the product
+// left in [0, 2p) is formed: when it exceeds p the true residue is below
+// p*kmax/2^32, and the baby table holds such residues a second time, plus p
+// (bsgs_prime), so the unreduced value finds them.  With CHAIN (some bucket
+// of this prime overflowed) an item whose bucket is full also follows the
+// chain to the next buckets.  A fingerprint match is resolved by an exact
+// rescan() of the reduced product.  The loop is wave-uniform because
+// rescan() needs every lane.
 template <int TW, bool CHAIN, bool LAZY>
 __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int r0,
                                             const uint4* __restrict__ tab,
@@ -173,43 +175,41 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
         u32 fp2[TW], bk[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            s[u] = LAZY ? halfmul_lazy(row.x, yp[u], p, mp0) : halfmul(row.x, yp[u], p, mp0);
-            bk[u] = bucket_bfe(s[u], (u32)J.logNB);
-            const u32 fp = fingerprint(s[u]);
+            u32 lo;
+            if (LAZY) {
+                lo = halfmul_lazy_lo(row.x, yp[u], p, mp0);
+            } else {
+                s[u] = halfmul(row.x, yp[u], p, mp0);
+                lo = lo32(s[u]);
+            }
+            bk[u] = __builtin_amdgcn_ubfe(lo, 15, (u32)J.logNB);
+            const u32 fp = (lo & 0x7fffu) | 0x8000u;
             fp2[u] = fp | (fp << 16);
         }
 #pragma unroll
         for (int u = 0; u < TW; ++u) e[u] = tab[bk[u]];
-        u64 mm[TW], slow[TW];
-        u64 anys = 0;
+        u64 mm[TW], full[TW];
+        u64 anyf = 0;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            const u32 z = bucket_match(e[u], fp2[u]);
-            mm[u] = lanes(z != 0) & okm;
-            slow[u] = 0;
-            if (LAZY) {
-                const u64 big = lanes(s[u] >= p) & okm;  // probed with the wrong bits
-                mm[u] &= ~big;
-                slow[u] |= big;
+            mm[u] = lanes(bucket_match(e[u], fp2[u]) != 0) & okm;
+            if (CHAIN) {
+                full[u] = lanes(e[u].w >= 0x10000u) & okm;  // slot 7 taken
+                anyf |= full[u];
             }
-            if (CHAIN) slow[u] |= lanes(e[u].w >= 0x10000u) & okm;  // slot 7 taken
-            anys |= slow[u];
         }
-        // slow path: reduce, then probe the bucket and its overflow chain
-        if ((CHAIN || LAZY) && anys) {
+        // full buckets: follow the chain to the first bucket with slot 7 free
+        if (CHAIN && anyf) {
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
                 bool hit = false;
-                if ((slow[u] >> lane) & 1) {
-                    s[u] = csub(s[u], p);
-                    u32 b = bucket_of(s[u], nbm);
-                    const u32 fp = fingerprint(s[u]);
-                    const u32 f2 = fp | (fp << 16);
+                if ((full[u] >> lane) & 1) {
+                    u32 b = bk[u];
                     while (true) {
-                        const uint4 f = tab[b];
-                        if (bucket_match(f, f2)) hit = true;
-                        if (f.w < 0x10000u) break;
                         b = (b + 1) & nbm;
+                        const uint4 f = tab[b];
+                        if (bucket_match(f, fp2[u])) hit = true;
+                        if (f.w < 0x10000u) break;
                     }
                 }
                 mm[u] |= lanes(hit);
@@ -221,10 +221,18 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
             while (mb) {
                 const int src = __builtin_ctzll(mb);
                 mb &= mb - 1;
-                rescan(J, job, csub(readlane64(s[u], src), p), __builtin_amdgcn_readlane(row.x, src),
-                       __builtin_amdgcn_readlane(row.y, src), r0,
-                       __builtin_amdgcn_readlane(t0, src) + u, vstart, g64, p, mp, hits,
-                       hit_count, hit_cap);
+                const u32 k = __builtin_amdgcn_readlane(row.x, src);
+                const u32 yent = __builtin_amdgcn_readlane(row.y, src);
+                const int tt = __builtin_amdgcn_readlane(t0, src) + u;
+                u64 target;
+                if (LAZY) {  // the exact product of the matched item
+                    const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
+                    target = halfmul(k, yv, p, mp0);
+                } else {
+                    target = readlane64(s[u], src);
+                }
+                rescan(J, job, target, k, yent, r0, tt, vstart, g64, p, mp, hits, hit_count,
+                       hit_cap);
             }
         }
     }
@@ -413,9 +421,9 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     const u64 vstart = montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp);
     u64 v = vstart;
     bool ovf = false;  // some insert went past a full bucket
-    for (int s = 0; s < L; ++s) {
-        u32 b = bucket_of(v, nbm);
-        const u32 fp = fingerprint(v);
+    auto insert = [&](u64 val) {
+        u32 b = bucket_of(val, nbm);
+        const u32 fp = fingerprint(val);
         u32 c = atomicAdd(&cnt[b], 1u);
         if (c >= 8) {  // rare: walk to the first bucket with a free slot
             ovf = true;
@@ -425,6 +433,13 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
             } while (c >= 8);
         }
         tab16[b * 8 + c] = (uint16_t)fp;
+    };
+    // lazy giant steps leave products in [0, 2p): an unreduced one is s + p
+    // with s < p*kmax/2^32 <= bound, so such baby values go in twice
+    const u64 bound = J.lazy ? (p >> 32) * J.kmax + J.kmax + 1 : 0;
+    for (int s = 0; s < L; ++s) {
+        insert(v);
+        if (v < bound) insert(v + p);
         if (s + 1 < L) v = shoup_mul(v, gw, gws, p);
     }
     const bool chain = __ballot(ovf) != 0;
