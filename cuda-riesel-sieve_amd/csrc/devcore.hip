@@ -126,14 +126,15 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.maxN = pl->maxN;
     J.nrows = nrows;
     J.nk = (int)kvals.size();
-    // buckets: average load between 1.5 and 3 of the 8 slots
+    // buckets: average load between 1.5 and 3 of the 8 slots, counting the
+    // second entries of baby values below p*kmax/2^32 (a kmax/2^32 share)
+    const int entries = m + (int)(((uint64_t)m * kvals.back() >> 32) + 1);
     int logNB = 0;
-    while ((1 << logNB) * 3 < m) ++logNB;
+    while ((1 << logNB) * 3 < entries) ++logNB;
     J.logNB = std::max(logNB, 2);
     if (J.Q * J.T >= 65536 || J.T > 65535) return RS_ERR_ARG;
     J.invT = magic_div(J.T);
     J.tw = tw;
-    J.lazy = kvals.back() < (1u << 24);
     J.kmax = kvals.back();
     J.plus = pl->c == 1;
     // flattened giant-step indices of a row slice (and their division by T)

@@ -35,10 +35,8 @@ struct JobDev {
     uint32_t invT;  // ceil(2^32 / T) for T >= 2, else 0: idx / T for idx < 2^16
     int tw;         // giant-step tile: consecutive t of one row per lane, 1, 2 or 4
                     // (T is rounded up to a multiple of tw)
-    int lazy;       // every k < 2^24: giant steps probe with the low word of
-                    // the product left in [0, 2p) (see the baby table's
-                    // second entries)
-    uint32_t kmax;  // largest k
+    uint32_t kmax;  // largest k: baby values below about p*kmax/2^32 get a
+                    // second table entry (plus p) for the unreduced giant steps
     int plus;       // 1: k*b^n+1 (Sierpinski), 0: k*b^n-1
     int W;        // record words
     int off_gt;   // REC_BH + Q

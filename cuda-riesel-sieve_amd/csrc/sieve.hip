@@ -142,15 +142,15 @@ __device__ __forceinline__ u64 lanes(bool c) { return __builtin_amdgcn_ballot_w6
 // per lane and 64 runs per round.  A lane reads its row entry once and its
 // TW Y products at fixed offsets; each item is one 32x64 Montgomery
 // product s = k*Y[r][t] and one bucket read, all TW issued before any is
-// tested.  With LAZY (every k < 2^24) only the low word of the product
-// left in [0, 2p) is formed: when it exceeds p the true residue is below
-// p*kmax/2^32, and the baby table holds such residues a second time, plus p
-// (bsgs_prime), so the unreduced value finds them.  With CHAIN (some bucket
-// of this prime overflowed) an item whose bucket is full also follows the
-// chain to the next buckets.  A fingerprint match is resolved by an exact
-// rescan() of the reduced product.  The loop is wave-uniform because
-// rescan() needs every lane.
-template <int TW, bool CHAIN, bool LAZY>
+// tested.  Only the low word of the product left in [0, 2p) is formed:
+// when that product exceeds p the true residue is below p*kmax/2^32, and
+// the baby table holds such residues a second time, plus p (bsgs_prime), so
+// the unreduced value finds them.  With CHAIN (some bucket of this prime
+// overflowed) an item whose bucket is full also follows the chain to the
+// next buckets.  A fingerprint match is resolved by an exact rescan() of
+// the reduced product.  The loop is wave-uniform because rescan() needs
+// every lane.
+template <int TW, bool CHAIN>
 __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int r0,
                                             const uint4* __restrict__ tab,
                                             const char* __restrict__ Yb,
@@ -170,18 +170,11 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
         if (!ok) a = A - 1, t0 = 0;
         const uint2 row = list[a];
         const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * t0);
-        u64 s[TW];
         uint4 e[TW];
         u32 fp2[TW], bk[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            u32 lo;
-            if (LAZY) {
-                lo = halfmul_lazy_lo(row.x, yp[u], p, mp0);
-            } else {
-                s[u] = halfmul(row.x, yp[u], p, mp0);
-                lo = lo32(s[u]);
-            }
+            const u32 lo = halfmul_lazy_lo(row.x, yp[u], p, mp0);
             bk[u] = __builtin_amdgcn_ubfe(lo, 15, (u32)J.logNB);
             const u32 fp = (lo & 0x7fffu) | 0x8000u;
             fp2[u] = fp | (fp << 16);
@@ -224,13 +217,9 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
                 const u32 k = __builtin_amdgcn_readlane(row.x, src);
                 const u32 yent = __builtin_amdgcn_readlane(row.y, src);
                 const int tt = __builtin_amdgcn_readlane(t0, src) + u;
-                u64 target;
-                if (LAZY) {  // the exact product of the matched item
-                    const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
-                    target = halfmul(k, yv, p, mp0);
-                } else {
-                    target = readlane64(s[u], src);
-                }
+                // the exact product of the matched item
+                const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
+                const u64 target = halfmul(k, yv, p, mp0);
                 rescan(J, job, target, k, yent, r0, tt, vstart, g64, p, mp, hits, hit_count,
                        hit_cap);
             }
@@ -434,9 +423,9 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
         }
         tab16[b * 8 + c] = (uint16_t)fp;
     };
-    // lazy giant steps leave products in [0, 2p): an unreduced one is s + p
-    // with s < p*kmax/2^32 <= bound, so such baby values go in twice
-    const u64 bound = J.lazy ? (p >> 32) * J.kmax + J.kmax + 1 : 0;
+    // giant steps leave products in [0, 2p): an unreduced one is s + p with
+    // s < p*kmax/2^32 <= bound, so such baby values go in twice
+    const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;
     for (int s = 0; s < L; ++s) {
         insert(v);
         if (v < bound) insert(v + p);
@@ -459,20 +448,20 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
                                   : compact_rows<false>(J, rows_lds, R, list, r0, r1, T);
         wave_sync();
         const char* const Yb = (const char*)Y;
-#define RS_GIANT(TWV, CH, LZ)                                                                  \
-    giant_steps<TWV, CH, LZ>(J, job, A, r0, tab, Yb, list, vstart, g64, p, mp, hits, hit_count, \
-                             hit_cap)
-#define RS_GIANT_TW(CH, LZ)          \
-    if (J.tw >= 4)                   \
-        RS_GIANT(4, CH, LZ);         \
-    else if (J.tw == 2)              \
-        RS_GIANT(2, CH, LZ);         \
-    else                             \
-        RS_GIANT(1, CH, LZ);
-        if (J.lazy) {
-            if (chain) { RS_GIANT_TW(true, true) } else { RS_GIANT_TW(false, true) }
+#define RS_GIANT(TWV, CH)                                                                     \
+    giant_steps<TWV, CH>(J, job, A, r0, tab, Yb, list, vstart, g64, p, mp, hits, hit_count, \
+                         hit_cap)
+#define RS_GIANT_TW(CH)      \
+    if (J.tw >= 4)           \
+        RS_GIANT(4, CH);     \
+    else if (J.tw == 2)      \
+        RS_GIANT(2, CH);     \
+    else                     \
+        RS_GIANT(1, CH);
+        if (chain) {
+            RS_GIANT_TW(true)
         } else {
-            if (chain) { RS_GIANT_TW(true, false) } else { RS_GIANT_TW(false, false) }
+            RS_GIANT_TW(false)
         }
 #undef RS_GIANT_TW
 #undef RS_GIANT

@@ -324,6 +324,27 @@ def test_many_k_base2_near_2_62(rs, orc, gpu):
     assert got == want
 
 
+def test_k_near_2_31_second_baby_entries(rs, orc, gpu):
+    """k just below 2^31 (the largest k of the format): giant-step products
+    left in [0, 2p) exceed p for many items and half the baby values are in
+    the table twice; small and large p, two tiles, against the oracle."""
+    plan = rs.Plan(rs.Abcd.synth(6, 40, (1 << 31) - (1 << 24), (1 << 31) - 1, 30000, 90000,
+                                 3000, seed=17, odd_k=False))
+    assert max(plan.rows()[0]) > (1 << 31) - (1 << 24)
+    for low, width in ((10**7 + 1, 1 << 20), ((1 << 62) + 1, 1 << 16)):
+        offs = rs.PrimeGen(low + width, threads=16).range(low, low + width)
+        m = plan.auto_m()
+        want = run_oracle(orc, plan, m, low, offs)
+        for tile in (1, 4):
+            sv = rs.Sieve(plan, m=m, max_batch=1 << 15, tile=tile)
+            sv.submit(low, offs)
+            got = sv.collect()
+            sv.close()
+            assert got == want, (low, tile)
+        if low < 1 << 32:
+            assert len(want) > 0
+
+
 def test_many_hits_small_p_datless(rs, orc, gpu):
     """p near 1e6: many hits per prime, exercises the append buffer."""
     plan = rs.Plan(rs.Abcd.synth(5, 10, 3, 10**5, 2000, 60000, 200, seed=4, odd_k=False))
