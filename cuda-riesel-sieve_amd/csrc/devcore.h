@@ -11,7 +11,8 @@
 
 extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
                                            const ChunkDev* chunks, int nchunks,
-                                           long long ngroups, int lds_words, int shared_words,
+                                           long long ngroups, int lds_words, int tab_words,
+                                           int shared_words,
                                            int scratch_words, uint64_t* scratch,
                                            unsigned long long* work, HitDev* hits,
                                            uint32_t* hit_count, uint32_t hit_cap);
@@ -42,7 +43,9 @@ struct Launcher {
     int grid = 0;
     int wpb = 1;
     int lds_words = 0;       // per wavefront
-    int shared_words = 0;    // block-shared words before the wavefront regions
+    int tab_words = 0;       // of which the table slot (a power of two, at least
+                             // the table of every job launched)
+    int shared_words = 0;    // block-shared words after the wavefront regions
     size_t lds_bytes = 0;
     int scratch_words = 0;  // per wavefront: 64 records
     uint64_t* scratch = nullptr;
@@ -85,8 +88,8 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj);
 size_t job_device_bytes(const rs_plan* pl, bool dat);
 void free_launcher(Launcher& L);
 // shared_words: block-shared LDS (the row table of a single-job launch).
-rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity,
-                        int shared_words = 0);
+rs_status init_launcher(Launcher* L, int device, int lds_words, int tab_words, int W,
+                        int64_t hit_capacity, int shared_words = 0);
 // Grid shape for a per-wave LDS layout (no allocation).
 rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words = 0);
 rs_status init_multi(MultiLauncher* M, int device, int w_cap, int64_t hit_capacity);

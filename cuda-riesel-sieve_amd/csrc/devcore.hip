@@ -19,7 +19,8 @@
 
 extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
                                            const ChunkDev* chunks, int nchunks,
-                                           long long ngroups, int lds_words, int shared_words,
+                                           long long ngroups, int lds_words, int tab_words,
+                                           int shared_words,
                                            int scratch_words, uint64_t* scratch,
                                            unsigned long long* work, HitDev* hits,
                                            uint32_t* hit_count, uint32_t hit_cap);
@@ -144,6 +145,7 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + 2 * J.T;
     J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.T, nrows, J.W);
+    J.tab_words = 4 << J.logNB;
     // per k: the 64-row word of its first row and its even-r / odd-r rows
     // in that word and the next (a k has at most Q < 64 rows)
     std::vector<uint32_t> kw0(kvals.size(), 0xffffffffu);
@@ -231,6 +233,11 @@ rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_wor
     // waves of a block are independent (one table each); 1 to 4 per block,
     // whichever keeps the most waves resident per CU (ties: more per block,
     // so the block-shared part is loaded fewer times)
+    // the kernel addresses its tables from LDS address 0: it must have no
+    // static LDS before the dynamic region
+    hipFuncAttributes fa{};
+    HIPCHK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(rs_sieve_kernel)));
+    if (fa.sharedSizeBytes != 0) return RS_ERR_HIP;
     // the attribute is per function: always the device limit, so launches
     // of different LDS sizes never restrict one another
     HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(rs_sieve_kernel),
@@ -267,10 +274,11 @@ static rs_status alloc_buffers(Launcher* L, size_t scratch_words_total, int64_t 
     return RS_OK;
 }
 
-rs_status init_launcher(Launcher* L, int device, int lds_words, int W, int64_t hit_capacity,
-                        int shared_words) {
+rs_status init_launcher(Launcher* L, int device, int lds_words, int tab_words, int W,
+                        int64_t hit_capacity, int shared_words) {
     rs_status st = launch_geometry(device, lds_words, L, shared_words);
     if (st != RS_OK) return st;
+    L->tab_words = tab_words;
     L->scratch_words = 64 * W;
     return alloc_buffers(L, (size_t)L->grid * L->wpb * L->scratch_words, hit_capacity);
 }
@@ -303,7 +311,8 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
         key[j] = {(long long)it->second.grid * kMaxWavesPerBlock + it->second.wpb, 0};
     }
     // classes in order of decreasing occupancy; a class takes the largest
-    // layout and record of its members (same grid shape by construction)
+    // table, the largest rest of the layout and the largest record of its
+    // members (their sum may give a grid of fewer waves than the members')
     std::vector<long long> keys;
     for (auto& k : key) keys.push_back(k.first);
     std::sort(keys.begin(), keys.end(), std::greater<long long>());
@@ -312,12 +321,14 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
     sorted.reserve(chunks->size());
     for (long long k : keys) {
         ClassRange cr{};
-        int lds = 0, W = 0;
+        int tabw = 0, rest = 0, W = 0;
         for (size_t j = 0; j < jobs.size(); ++j)
             if (key[j].first == k) {
-                lds = std::max(lds, jobs[j].lds_words);
+                tabw = std::max(tabw, jobs[j].tab_words);
+                rest = std::max(rest, jobs[j].lds_words - jobs[j].tab_words);
                 W = std::max(W, jobs[j].W);
             }
+        const int lds = tabw + rest;
         cr.chunk0 = (int)sorted.size();
         long long g = 0;
         for (const ChunkDev& c : *chunks) {
@@ -330,7 +341,15 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
         cr.nchunks = (int)sorted.size() - cr.chunk0;
         cr.ngroups = g;
         if (cr.nchunks == 0) continue;
-        cr.L = M->geo.at(lds);
+        auto it = M->geo.find(lds);
+        if (it == M->geo.end()) {
+            Launcher g;
+            rs_status st = launch_geometry(M->base.device, lds, &g);
+            if (st != RS_OK) return st;
+            it = M->geo.emplace(lds, g).first;
+        }
+        cr.L = it->second;
+        cr.L.tab_words = tabw;
         cr.L.scratch_words = 64 * W;
         cr.L.scratch = M->base.scratch;
         cr.L.work = M->base.work;
@@ -369,7 +388,7 @@ rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const 
     const int grid = (int)std::min<long long>(L.grid, (ngroups + L.wpb - 1) / L.wpb);
     hipLaunchKernelGGL(rs_sieve_kernel, dim3(std::max(1, grid)), dim3(64 * L.wpb), L.lds_bytes,
                        stream, j0, c0, jobs, chunks, nchunks, ngroups, L.lds_words,
-                       jobs ? 0 : L.shared_words,
+                       L.tab_words, jobs ? 0 : L.shared_words,
                        L.scratch_words, L.scratch, L.work, L.hits, L.hit_count, L.hit_cap);
     HIPCHK(hipGetLastError());
     return RS_OK;

@@ -108,7 +108,8 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int t
     const int nrows = s->job.J.nrows;
     const int shared = nrows <= ROW_SLICE ? (2 * nrows + 3) & ~3 : 0;
     if (st == RS_OK)
-        st = init_launcher(&s->L, device, s->job.J.lds_words, s->job.J.W, hit_capacity, shared);
+        st = init_launcher(&s->L, device, s->job.J.lds_words, s->job.J.tab_words, s->job.J.W,
+                           hit_capacity, shared);
     if (st == RS_OK &&
         (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
          hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess))

@@ -42,6 +42,7 @@ struct JobDev {
     int off_gt;   // REC_BH + Q
     int off_j;    // off_gt + 2 T: the row masks
     int lds_words;  // 32-bit LDS words per wave (table, counts, Y, rows, record)
+    int tab_words;  // of which the table: 4 NB, a power of two
     const uint4* kjac;        // per distinct k (ascending): odd part n, 2-adic
                               // valuation z, floor((2^64-1)/n) as lo, hi [nk]
     uint4 bjac;               // the same for the base
@@ -88,9 +89,11 @@ static inline int rec_words(int Q, int T, int nrows) {
     return REC_BH + Q + 2 * T + (nrows + 63) / 64;
 }
 
-// LDS layout of one wavefront, in 32-bit words from its base:
-//   [0, 4 NB)                 table: NB buckets x 8 uint16 fingerprints (0 = empty)
-//   [4 NB, 5 NB)              bucket fill counts, one uint32 per bucket
+// LDS words of one wavefront: its table, placed apart from the rest so
+// that it is aligned to its size,
+//   4 NB                      NB buckets x 8 uint16 fingerprints (0 = empty)
+// and, from the base of the rest,
+//   NB                        bucket fill counts, one uint32 per bucket
 //   then 2 Q T                Y products (uint64)
 //   then 2 min(nrows, ROW_SLICE)  active rows of one slice (k, 8 r T | row << 19)
 //   then 2 W                  the prime's record (uint64)

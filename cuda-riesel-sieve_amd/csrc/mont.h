@@ -98,6 +98,35 @@ __device__ __forceinline__ u64 halfmul(u32 k, u64 y, u64 p, u32 mp0) {
     return csub(halfmul_lazy(k, y, p, mp0), p);
 }
 
+// Shoup entry of a residue given as z = y 2^32 mod p: yq:lo(y) with the
+// plain y = z 2^-32 mod p and yq = floor(y 2^32 / p).  One 32-bit
+// Montgomery step gives both: its multiplier m = lo(z)*mp0 satisfies
+// y 2^32 = z + m p with 0 <= z < p, so m is yq and y < p exactly.
+__device__ __forceinline__ u64 shoup32_entry(u64 z, u64 p, u32 mp0) {
+    const u32 yq = lo32(z) * mp0;
+    const u64 s0 = mad32(yq, lo32(p), lo32(z));  // low word is 0
+    const u32 ylo = yq * hi32(p) + hi32(z) + hi32(s0);
+    return ((u64)yq << 32) | ylo;
+}
+
+// Low word of Shoup's product k*y - floor(k*yq/2^32)*p for a 32-bit k and
+// the entry yq:lo(y): the product lies in [0, 2p) and exceeds p only when
+// the reduced product is below p*k/2^32.  negp0 = -lo(p) mod 2^32.
+__device__ __forceinline__ u32 shoup32_lo(u32 k, u32 ylo, u32 yq, u32 negp0) {
+    return lo32(mad32(__umulhi(k, yq), negp0, (u64)(k * ylo)));
+}
+
+// k*y mod p from the entry yq:lo(y).  y is recovered from it: since
+// y 2^32 = yq p + z with z < p, y - floor(yq p / 2^32) lies in
+// [0, p/2^32 + 1), below 2^32, so the low word of y fixes it.
+__device__ __forceinline__ u64 shoup32_full(u32 k, u64 ent, u64 p) {
+    const u32 ylo = lo32(ent), yq = hi32(ent);
+    const u64 a = mad32(yq, hi32(p), hi32(mad32(yq, lo32(p), 0)));
+    const u64 y = a + (u32)(ylo - lo32(a));
+    const u32 q = __umulhi(k, yq);
+    return csub((u64)k * y - (u64)q * p, p);
+}
+
 // High 64 bits of x*y.
 __device__ __forceinline__ u64 umulhi64(u64 x, u64 y) {
     const u64 t0 = mad32(lo32(x), lo32(y), 0);

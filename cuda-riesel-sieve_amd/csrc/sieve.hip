@@ -59,33 +59,42 @@ __device__ __forceinline__ int div_T(u32 idx, u32 invT) {
     return invT ? (int)__umulhi(idx, invT) : (int)idx;
 }
 
-// Table hash of a residue: the bucket is taken from bits 15 and up of its
-// low word, the 16-bit fingerprint is its low 15 bits with the top bit set,
-// so it is never 0, the empty-slot value.
-__device__ __forceinline__ u32 bucket_of(u64 v, u32 nbm) { return (lo32(v) >> 15) & nbm; }
-// The same by bit-field extract of logNB bits (one instruction).
-__device__ __forceinline__ u32 bucket_bfe(u64 v, u32 logNB) {
-    return __builtin_amdgcn_ubfe(lo32(v), 15, logNB);
-}
-
-// Nonzero iff some 16-bit half of x is zero (exact as a boolean: the
-// lowest zero half takes no borrow from below).
-__device__ __forceinline__ u32 has_zero_half(u32 x) {
-    return (x - 0x10001u) & ~x & 0x80008000u;
+// Table hash of a residue: the bucket is taken from bits 4 and up of its
+// low word and the 16-bit fingerprint is its high half.  Empty slots hold
+// 0, so a probe whose fingerprint is 0 matches them and is resolved by the
+// exact rescan (a 2^-16 chance per probe); slot occupancy is read from the
+// fill counts, never from the slots.  The bucket's byte offset in the table
+// is lo & (nbm << 4), which a table aligned to its size adds by OR.  (Above
+// 4096 buckets the bucket bits reach into the fingerprint, whose distinct
+// bits within a bucket drop below 16: only the rescan rate grows.)
+__device__ __forceinline__ u32 bucket_of(u64 v, u32 nbm) { return (lo32(v) >> 4) & nbm; }
+__device__ __forceinline__ u32 fingerprint(u64 v) { return lo32(v) >> 16; }
+// The fingerprint of a low word in both halves (one byte permute).
+__device__ __forceinline__ u32 fingerprint2(u32 lo) {
+    return __builtin_amdgcn_perm(lo, lo, 0x03020302u);
 }
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const u32x4 lds_u32x4;
 
-// Nonzero iff one of the 8 16-bit slots of e equals the fingerprint in
-// both halves of fp2: the packed minimum of the XORs has a zero half.
-__device__ __forceinline__ u32 bucket_match(uint4 e, u32 fp2) {
+// The bucket at LDS byte address a.
+__device__ __forceinline__ uint4 lds_bucket(u32 a) {
+    const u32x4 v = *(lds_u32x4*)(size_t)a;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// True iff one of the 8 16-bit slots of e equals the fingerprint in both
+// halves of fp2: the packed minimum of the XORs, folded with its halves
+// swapped, has a zero low half.
+__device__ __forceinline__ bool bucket_match(uint4 e, u32 fp2) {
     const u16x2 a = __builtin_bit_cast(u16x2, e.x ^ fp2), b = __builtin_bit_cast(u16x2, e.y ^ fp2);
     const u16x2 c = __builtin_bit_cast(u16x2, e.z ^ fp2), d = __builtin_bit_cast(u16x2, e.w ^ fp2);
     const u16x2 mn = __builtin_elementwise_min(__builtin_elementwise_min(a, b),
                                                __builtin_elementwise_min(c, d));
-    return has_zero_half(__builtin_bit_cast(u32, mn));
+    const u16x2 f = __builtin_elementwise_min(mn, __builtin_shufflevector(mn, mn, 1, 0));
+    return f.x == 0;
 }
-__device__ __forceinline__ u32 fingerprint(u64 v) { return (lo32(v) & 0x7fffu) | 0x8000u; }
 
 // Active-row entries of the giant steps: .x = k, .y = 8*r*T (byte offset of
 // the row's Y run, < 2^19) | (row - slice start) << 19.
@@ -134,34 +143,52 @@ __device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32
     }
 }
 
+// A copy of the struct at a wave-uniform address, held in scalar registers
+// (read once by every lane, then taken from the first lane): job and chunk
+// constants then steer loops and offsets without vector instructions.
+template <typename S>
+__device__ __forceinline__ S uniform_load(const S* p) {
+    static_assert(sizeof(S) % 4 == 0, "struct of 32-bit words");
+    S out;
+    const u32* src = (const u32*)p;
+    u32* dst = (u32*)&out;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(S) / 4); ++i)
+        dst[i] = (u32)__builtin_amdgcn_readfirstlane(src[i]);
+    return out;
+}
+
 // Lane mask of a per-lane condition (straight to a scalar register pair).
 __device__ __forceinline__ u64 lanes(bool c) { return __builtin_amdgcn_ballot_w64(c); }
 
 // Giant steps of one prime: the A*T grid of (active row a, step t) is cut
 // into runs of TW consecutive t of one row (T is a multiple of TW), one run
 // per lane and 64 runs per round.  A lane reads its row entry once and its
-// TW Y products at fixed offsets; each item is one 32x64 Montgomery
-// product s = k*Y[r][t] and one bucket read, all TW issued before any is
-// tested.  Only the low word of the product left in [0, 2p) is formed:
-// when that product exceeds p the true residue is below p*kmax/2^32, and
-// the baby table holds such residues a second time, plus p (bsgs_prime), so
-// the unreduced value finds them.  With CHAIN (some bucket of this prime
+// TW Y entries at fixed offsets; each item is the low word of one Shoup
+// product s = k*Y[r][t] in [0, 2p) (shoup32_lo: three multiplies) and one
+// bucket read, all TW issued before any is tested.  When s exceeds p the
+// true residue is below p*kmax/2^32, and the baby table holds such
+// residues a second time, plus p (bsgs_prime), so the unreduced value
+// finds them.  With CHAIN (some bucket of this prime
 // overflowed) an item whose bucket is full also follows the chain to the
-// next buckets.  A fingerprint match is resolved by an exact rescan() of
+// next buckets (fullness is read from the fill counts `cnt`).  A
+// fingerprint match is resolved by an exact rescan() of
 // the reduced product.  The loop is wave-uniform because rescan() needs
 // every lane.
 template <int TW, bool CHAIN>
 __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int r0,
-                                            const uint4* __restrict__ tab,
+                                            u32 tab_addr,
+                                            const u32* __restrict__ cnt,
                                             const char* __restrict__ Yb,
                                             const uint2* __restrict__ list, u64 vstart, u64 g64,
                                             u64 p, u64 mp, HitDev* __restrict__ hits,
                                             u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
     const u32 nbm = (1u << J.logNB) - 1;
+    const u32 bmask = nbm << 4;
     const int T = J.T;
     const int total = A * T;
-    const u32 mp0 = lo32(mp);
+    const u32 negp0 = 0u - lo32(p);
     for (int base = 0; base < total; base += 64 * TW) {
         const int i0 = base + TW * lane;
         const bool ok = i0 < total;
@@ -171,38 +198,37 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
         const uint2 row = list[a];
         const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * t0);
         uint4 e[TW];
-        u32 fp2[TW], bk[TW];
+        u32 fp2[TW], ba[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            const u32 lo = halfmul_lazy_lo(row.x, yp[u], p, mp0);
-            bk[u] = __builtin_amdgcn_ubfe(lo, 15, (u32)J.logNB);
-            const u32 fp = (lo & 0x7fffu) | 0x8000u;
-            fp2[u] = fp | (fp << 16);
+            const u32 lo = shoup32_lo(row.x, lo32(yp[u]), hi32(yp[u]), negp0);
+            ba[u] = (lo & bmask) | tab_addr;  // the bucket's LDS byte address
+            fp2[u] = fingerprint2(lo);
         }
 #pragma unroll
-        for (int u = 0; u < TW; ++u) e[u] = tab[bk[u]];
+        for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
         u64 mm[TW], full[TW];
         u64 anyf = 0;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            mm[u] = lanes(bucket_match(e[u], fp2[u]) != 0) & okm;
+            mm[u] = lanes(bucket_match(e[u], fp2[u])) & okm;
             if (CHAIN) {
-                full[u] = lanes(e[u].w >= 0x10000u) & okm;  // slot 7 taken
+                full[u] = lanes(cnt[(ba[u] - tab_addr) >> 4] >= 8) & okm;
                 anyf |= full[u];
             }
         }
-        // full buckets: follow the chain to the first bucket with slot 7 free
+        // full buckets: follow the chain to the first bucket that is not full
         if (CHAIN && anyf) {
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
                 bool hit = false;
                 if ((full[u] >> lane) & 1) {
-                    u32 b = bk[u];
+                    u32 b = (ba[u] - tab_addr) >> 4;
                     while (true) {
                         b = (b + 1) & nbm;
-                        const uint4 f = tab[b];
+                        const uint4 f = lds_bucket(tab_addr + 16 * b);
                         if (bucket_match(f, fp2[u])) hit = true;
-                        if (f.w < 0x10000u) break;
+                        if (cnt[b] < 8) break;
                     }
                 }
                 mm[u] |= lanes(hit);
@@ -219,7 +245,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
                 const int tt = __builtin_amdgcn_readlane(t0, src) + u;
                 // the exact product of the matched item
                 const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
-                const u64 target = halfmul(k, yv, p, mp0);
+                const u64 target = shoup32_full(k, yv, p);
                 rescan(J, job, target, k, yent, r0, tt, vstart, g64, p, mp, hits, hit_count,
                        hit_cap);
             }
@@ -376,9 +402,12 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
 }
 
 // BSGS of one prime by the whole wavefront.  `pre` holds record words
-// 0..63 (lane l has word l), the rest are read from Rg; LDS at wbase.
+// 0..63 (lane l has word l), the rest are read from Rg.  The table is at
+// tab, LDS byte address tab_addr, a multiple of its size; the rest of the
+// wavefront's LDS (counts, Y, row list, record) at wbase.
 __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
-                                           const u64* __restrict__ Rg, u32* __restrict__ wbase,
+                                           const u64* __restrict__ Rg, uint4* __restrict__ tab,
+                                           u32 tab_addr, u32* __restrict__ wbase,
                                            const uint2* __restrict__ rows_lds,
                                            bool rows_in_lds, HitDev* __restrict__ hits,
                                            u32* __restrict__ hit_count, u32 hit_cap) {
@@ -386,9 +415,8 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     const u32 NB = 1u << J.logNB;
     const u32 nbm = NB - 1;
     const int Q = J.Q, T = J.T, m = J.m, W = J.W;
-    uint4* const tab = (uint4*)wbase;
-    uint16_t* const tab16 = (uint16_t*)wbase;
-    u32* const cnt = wbase + 4 * NB;
+    uint16_t* const tab16 = (uint16_t*)tab;
+    u32* const cnt = wbase;
     u64* const Y = (u64*)(cnt + NB);
     uint2* const list = (uint2*)(Y + Q * T);
     u64* const R = (u64*)(list + min(J.nrows, ROW_SLICE));
@@ -433,11 +461,13 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     }
     const bool chain = __ballot(ovf) != 0;
 
-    // Y[r*T + t] = b^r H G^t 2^32 (Montgomery form): BH[r] times the plain
-    // G^t by Shoup's method
+    // Y[r*T + t]: the Shoup entry of b^r H G^t, made from its form times
+    // 2^32, which is BH[r] times the plain G^t by Shoup's method
+    const u32 mp0 = lo32(mp);
     for (int idx = lane; idx < Q * T; idx += 64) {
         const int r = div_T((u32)idx, J.invT), t = idx - r * T;
-        Y[idx] = shoup_mul(R[REC_BH + r], R[J.off_gt + 2 * t], R[J.off_gt + 2 * t + 1], p);
+        const u64 z = shoup_mul(R[REC_BH + r], R[J.off_gt + 2 * t], R[J.off_gt + 2 * t + 1], p);
+        Y[idx] = shoup32_entry(z, p, mp0);
     }
 
     // rows in slices of at most ROW_SLICE: those passing the Jacobi filter
@@ -449,8 +479,8 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
         wave_sync();
         const char* const Yb = (const char*)Y;
 #define RS_GIANT(TWV, CH)                                                                     \
-    giant_steps<TWV, CH>(J, job, A, r0, tab, Yb, list, vstart, g64, p, mp, hits, hit_count, \
-                         hit_cap)
+    giant_steps<TWV, CH>(J, job, A, r0, tab_addr, cnt, Yb, list, vstart, g64, p, mp, hits, \
+                         hit_count, hit_cap)
 #define RS_GIANT_TW(CH)      \
     if (J.tw >= 4)           \
         RS_GIANT(4, CH);     \
@@ -475,23 +505,31 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
                 const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
-                int lds_words, int shared_words, int scratch_words,
+                int lds_words, int tab_words, int shared_words, int scratch_words,
                 uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
                 HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
     extern __shared__ uint32_t lds[];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wpb = blockDim.x >> 6;
-    // single-job launches keep the job's row table (k, kidx | r << 16) in
-    // block-shared LDS (shared_words >= 2 * nrows), read by every prime
-    uint2* const rows_lds = (uint2*)lds;
+    // block LDS: the wavefronts' tables (tab_words each, a power of two at
+    // least the table of every job of the launch, so each is aligned to its
+    // size), then the rest of each wavefront's words (lds_words - tab_words),
+    // then the block-shared part: single-job launches keep the job's row
+    // table (k, kidx | r << 16) there (shared_words >= 2 * nrows), read by
+    // every prime
+    uint2* const rows_lds = (uint2*)(lds + wpb * lds_words);
     const bool rows_in_lds = shared_words > 0 && !jobs;
     if (rows_in_lds) {
         for (int i = threadIdx.x; i < j0.nrows; i += blockDim.x)
             rows_lds[i] = make_uint2(j0.rowk[i], j0.rowinfo[i]);
         __syncthreads();
     }
-    u32* const wbase = lds + shared_words + wv * lds_words;
+    // the kernel has no static LDS (checked by the host), so the dynamic
+    // region starts at LDS address 0 and each table is aligned to its slot
+    uint4* const tab = (uint4*)(lds + wv * tab_words);
+    const u32 tab_addr = (u32)(wv * tab_words) * 4u;
+    u32* const wbase = lds + wpb * tab_words + wv * (lds_words - tab_words);
     u64* const S = scratch + ((long long)blockIdx.x * wpb + wv) * scratch_words;
 
     while (true) {
@@ -510,10 +548,11 @@ rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
                 else hi = mid - 1;
             }
         }
-        const ChunkDev C = chunks ? chunks[ci] : c0;
-        const JobDev J = jobs ? jobs[C.job] : j0;
+        const ChunkDev C = chunks ? uniform_load(chunks + ci) : c0;
+        const JobDev J = jobs ? uniform_load(jobs + C.job) : j0;
         const long long q0 = ((long long)grp - C.group0) * 64;
-        const long long cnt = C.count_dev ? (long long)*C.count_dev : C.count;
+        const long long cnt =
+            C.count_dev ? (long long)(u32)__builtin_amdgcn_readfirstlane(*C.count_dev) : C.count;
         const int n = (int)min(64ll, cnt - q0);
         if (n <= 0) break;  // device-counted chunk: groups past the end are empty
 
@@ -527,8 +566,8 @@ rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
         for (int i = 0; i < n; ++i) {
             const u64 cur = pre;
             if (i + 1 < n) pre = lane < J.W ? S[(i + 1) * J.W + lane] : 0;
-            bsgs_prime(J, C.job, cur, S + i * J.W, wbase, rows_lds, rows_in_lds, hits,
-                       hit_count, hit_cap);
+            bsgs_prime(J, C.job, cur, S + i * J.W, tab, tab_addr, wbase, rows_lds, rows_in_lds,
+                       hits, hit_count, hit_cap);
         }
         wave_sync_global();  // records are rewritten by the next group
     }
