@@ -133,7 +133,8 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     int logNB = 0;
     while ((1 << logNB) * 3 < entries) ++logNB;
     J.logNB = std::max(logNB, 2);
-    if (J.Q * J.T >= 65536 || J.T > 65535) return RS_ERR_ARG;
+    J.ys = J.T + (J.T % 2 == 0 ? 1 : 0);
+    if (J.Q * J.ys >= 65536 || J.T > 65535) return RS_ERR_ARG;
     J.invT = magic_div(J.T);
     J.tw = tw;
     J.kmax = kvals.back();
@@ -144,7 +145,7 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.W = rec_words(J.Q, J.T, nrows);
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + 2 * J.T;
-    J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.T, nrows, J.W);
+    J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.ys, nrows, J.W);
     J.tab_words = 4 << J.logNB;
     // per k: the 64-row word of its first row and its even-r / odd-r rows
     // in that word and the next (a k has at most Q < 64 rows)

@@ -43,6 +43,8 @@ struct JobDev {
     int off_j;    // off_gt + 2 T: the row masks
     int lds_words;  // 32-bit LDS words per wave (table, counts, Y, rows, record)
     int tab_words;  // of which the table: 4 NB, a power of two
+    int ys;         // Y row stride in entries: T, or T + 1 when T is even, so
+                    // that rows of different r start on different LDS banks
     const uint4* kjac;        // per distinct k (ascending): odd part n, 2-adic
                               // valuation z, floor((2^64-1)/n) as lo, hi [nk]
     uint4 bjac;               // the same for the base
@@ -94,12 +96,12 @@ static inline int rec_words(int Q, int T, int nrows) {
 //   4 NB                      NB buckets x 8 uint16 fingerprints (0 = empty)
 // and, from the base of the rest,
 //   NB                        bucket fill counts, one uint32 per bucket
-//   then 2 Q T                Y products (uint64)
+//   then 2 Q ys               Y entries (uint64), rows ys apart
 //   then 2 min(nrows, ROW_SLICE)  active rows of one slice (k, 8 r T | row << 19)
 //   then 2 W                  the prime's record (uint64)
-static inline int lds_words_per_wave(int logNB, int Q, int T, int nrows, int W) {
+static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W) {
     const int NB = 1 << logNB;
     const int list_rows = nrows < ROW_SLICE ? nrows : ROW_SLICE;  // one row slice
-    int w = 5 * NB + 2 * Q * T + 2 * list_rows + 2 * W;
+    int w = 5 * NB + 2 * Q * ys + 2 * list_rows + 2 * W;
     return (w + 3) & ~3;
 }

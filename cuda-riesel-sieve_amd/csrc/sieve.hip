@@ -96,10 +96,10 @@ __device__ __forceinline__ bool bucket_match(uint4 e, u32 fp2) {
     return f.x == 0;
 }
 
-// Active-row entries of the giant steps: .x = k, .y = 8*r*T (byte offset of
-// the row's Y run, < 2^19) | (row - slice start) << 19.
-__device__ __forceinline__ u32 row_entry(u32 r, u32 T, u32 rowrel) {
-    return (r * T * 8u) | (rowrel << 19);
+// Active-row entries of the giant steps: .x = k, .y = 8*r*ys (byte offset
+// of the row's Y run, ys the Y row stride; < 2^19) | (row - slice start) << 19.
+__device__ __forceinline__ u32 row_entry(u32 r, u32 ys, u32 rowrel) {
+    return (r * ys * 8u) | (rowrel << 19);
 }
 
 
@@ -113,7 +113,7 @@ __device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32
                                        HitDev* __restrict__ hits, u32* __restrict__ hit_count,
                                        u32 hit_cap) {
     const int lane = threadIdx.x & 63;
-    const u32 r = (u32)div_T((yent & 0x7ffffu) >> 3, J.invT);
+    const u32 r = ((yent & 0x7ffffu) >> 3) / (u32)J.ys;
     const u32 row = (u32)r0 + (yent >> 19);
     u64 w = vstart;
     for (int v = 0; v < (J.m >> 6); ++v) {
@@ -271,18 +271,14 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
         const u64 mask = ((u64)mhi << 32) | mlo;
         if ((mask >> lane) & 1) {
             const int row = c + lane;
-            u32 k, info;
-            if (FROM_LDS) {
-                const uint2 rw = rows_lds[row];
-                k = rw.x;
-                info = rw.y;
-            } else {
-                info = J.rowinfo[row];
-                k = J.rowk[row];
-            }
+            // the block-shared table holds finished entries (one slice: r0 = 0)
+            const uint2 ent = FROM_LDS ? rows_lds[row]
+                                       : make_uint2(J.rowk[row], row_entry(J.rowinfo[row] >> 16,
+                                                                           (u32)J.ys,
+                                                                           (u32)(row - r0)));
             const u32 below = __builtin_amdgcn_mbcnt_hi(hi32(mask),
                                                         __builtin_amdgcn_mbcnt_lo(lo32(mask), 0));
-            list[A + below] = make_uint2(k, row_entry(info >> 16, (u32)T, (u32)(row - r0)));
+            list[A + below] = ent;
         }
         A += __popcll(mask);
     }
@@ -418,7 +414,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     uint16_t* const tab16 = (uint16_t*)tab;
     u32* const cnt = wbase;
     u64* const Y = (u64*)(cnt + NB);
-    uint2* const list = (uint2*)(Y + Q * T);
+    uint2* const list = (uint2*)(Y + Q * J.ys);
     u64* const R = (u64*)(list + min(J.nrows, ROW_SLICE));
     const int L = m >> 6;
 
@@ -461,13 +457,13 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     }
     const bool chain = __ballot(ovf) != 0;
 
-    // Y[r*T + t]: the Shoup entry of b^r H G^t, made from its form times
+    // Y[r*ys + t]: the Shoup entry of b^r H G^t, made from its form times
     // 2^32, which is BH[r] times the plain G^t by Shoup's method
     const u32 mp0 = lo32(mp);
     for (int idx = lane; idx < Q * T; idx += 64) {
         const int r = div_T((u32)idx, J.invT), t = idx - r * T;
         const u64 z = shoup_mul(R[REC_BH + r], R[J.off_gt + 2 * t], R[J.off_gt + 2 * t + 1], p);
-        Y[idx] = shoup32_entry(z, p, mp0);
+        Y[r * J.ys + t] = shoup32_entry(z, p, mp0);
     }
 
     // rows in slices of at most ROW_SLICE: those passing the Jacobi filter
@@ -515,14 +511,14 @@ rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
     // block LDS: the wavefronts' tables (tab_words each, a power of two at
     // least the table of every job of the launch, so each is aligned to its
     // size), then the rest of each wavefront's words (lds_words - tab_words),
-    // then the block-shared part: single-job launches keep the job's row
-    // table (k, kidx | r << 16) there (shared_words >= 2 * nrows), read by
-    // every prime
+    // then the block-shared part: single-job launches whose rows fit one
+    // slice keep the job's active-row entries (k, row_entry) there
+    // (shared_words >= 2 * nrows), read by every prime
     uint2* const rows_lds = (uint2*)(lds + wpb * lds_words);
     const bool rows_in_lds = shared_words > 0 && !jobs;
     if (rows_in_lds) {
         for (int i = threadIdx.x; i < j0.nrows; i += blockDim.x)
-            rows_lds[i] = make_uint2(j0.rowk[i], j0.rowinfo[i]);
+            rows_lds[i] = make_uint2(j0.rowk[i], row_entry(j0.rowinfo[i] >> 16, (u32)j0.ys, i));
         __syncthreads();
     }
     // the kernel has no static LDS (checked by the host), so the dynamic
