@@ -127,6 +127,29 @@ __device__ __forceinline__ u64 shoup32_full(u32 k, u64 ent, u64 p) {
     return csub((u64)k * y - (u64)q * p, p);
 }
 
+// Products mod p < 2^50 in double precision, on residues kept as exact
+// integers in (-p, p).  With P = a*b (|a|, |b| < p), h = fl(a*b) and
+// l = P - h exactly (the FMA error term).  t = fl(h/p + 1.5*2^52) rounds
+// h/p to an integer q (the constant fixes the rounding position, |h/p| <
+// 2^50); the rounding of 1/p and of the product move h/p from P/p by less
+// than 0.375, so |P - q p| < 0.875 p.  Then fma(-q, p, h) = h - q p, an
+// integer below 2^53 in magnitude, is exact, and adding l is exact too.
+constexpr double kFpRound = 6755399441055744.0;  // 1.5 * 2^52
+constexpr double kFpLow = 4503599627370496.0;    // 2^52
+constexpr uint64_t kFpMaxP = 1ull << 50;
+
+__device__ __forceinline__ double fp_mulmod(double a, double b, double p, double pinv) {
+#pragma clang fp contract(off)
+    const double h = a * b;
+    const double l = __fma_rn(a, b, -h);
+    const double q = __fma_rn(h, pinv, kFpRound) - kFpRound;
+    return __fma_rn(-q, p, h) + l;
+}
+
+// The low 64 bits of the residue r + 2^52 for an integer 0 <= r < 2^52:
+// bits 0..51 are r itself.
+__device__ __forceinline__ u64 fp_bits(double r) { return (u64)__double_as_longlong(r + kFpLow); }
+
 // High 64 bits of x*y.
 __device__ __forceinline__ u64 umulhi64(u64 x, u64 y) {
     const u64 t0 = mad32(lo32(x), lo32(y), 0);

@@ -450,10 +450,25 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     // giant steps leave products in [0, 2p): an unreduced one is s + p with
     // s < p*kmax/2^32 <= bound, so such baby values go in twice
     const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;
-    for (int s = 0; s < L; ++s) {
-        insert(v);
-        if (v < bound) insert(v + p);
-        if (s + 1 < L) v = shoup_mul(v, gw, gws, p);
+    if (p < kFpMaxP) {
+        // the chain in double precision (fp_mulmod: about a third of the
+        // integer Shoup step's instructions); each value is made canonical
+        // and its low word read from the bits of value + 2^52
+        const double pd = (double)p, pinv = 1.0 / pd, wd = (double)gw;
+        const double bd = (double)bound;  // below 2^50
+        double vd = (double)v;
+        for (int s = 0; s < L; ++s) {
+            const double c = vd < 0.0 ? vd + pd : vd;
+            insert(fp_bits(c));
+            if (c < bd) insert(fp_bits(c + pd));
+            if (s + 1 < L) vd = fp_mulmod(vd, wd, pd, pinv);
+        }
+    } else {
+        for (int s = 0; s < L; ++s) {
+            insert(v);
+            if (v < bound) insert(v + p);
+            if (s + 1 < L) v = shoup_mul(v, gw, gws, p);
+        }
     }
     const bool chain = __ballot(ovf) != 0;
 

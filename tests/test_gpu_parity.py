@@ -293,6 +293,47 @@ def test_large_p_near_2_62_and_2_63(rs, orc, gpu):
         assert got == want
 
 
+def planted_pairs(b, p, nmin, nmax):
+    """(k, n) with 1 < k < 2^31 and p | k*b^n - 1, n in [nmin, nmax]: k is
+    b^-n mod p whenever that is small (about one n in p/2^31)."""
+    inv = pow(b, -1, p)
+    v = pow(inv, nmin, p)
+    out = []
+    for n in range(nmin, nmax + 1):
+        if 1 < v < 1 << 31:
+            out.append((v, n))
+        v = v * inv % p
+    return out
+
+
+def test_p_around_2_50_both_baby_chains(rs, orc, gpu):
+    """Primes on both sides of 2^50, where the baby-step chain switches from
+    double precision to 64-bit integers: factors planted for the nearest
+    primes below and above 2^50 must be found, and the whole window's hits
+    equal the oracle's."""
+    b = 5
+    lo_p = max(q for q in range((1 << 50) - 400, 1 << 50) if is_prime(q))
+    hi_p = min(q for q in range((1 << 50) + 1, (1 << 50) + 400) if is_prime(q))
+    kn = {}
+    planted = []
+    for q in (lo_p, hi_p):
+        pairs = planted_pairs(b, q, 1 << 20, (1 << 20) + (1 << 21))
+        assert pairs, q
+        for k, n in pairs[:3]:
+            kn.setdefault(k, set()).update(range(n - 30, n + 31))
+            planted.append((q, k, n))
+    plan = rs.Plan(rs.Abcd.from_lists(b, {k: sorted(v) for k, v in kn.items()}))
+    low = lo_p - (1 << 12)
+    offs = rs.PrimeGen(hi_p + (1 << 12), threads=16).range(low, hi_p + (1 << 12))
+    m = plan.auto_m()
+    got = run_device(rs, plan, low, offs, m=m)
+    want = run_oracle(orc, plan, m, low, offs)
+    assert got == want
+    for h in planted:
+        assert h in got, h
+    assert all(rs.verify_factor(p, k, b, n) for p, k, n in got)
+
+
 def test_many_rows_sliced_all_tiles(rs, orc, gpu):
     """200 k of base 1030 give ~1000+ active (k, r) rows, more than one LDS
     row slice (512): every slice and every giant-step tile width must agree
