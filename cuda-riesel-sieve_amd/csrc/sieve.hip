@@ -264,10 +264,12 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
                                             int r0, int r1, int T) {
     const int lane = threadIdx.x & 63;
     int A = 0;
-    for (int c = r0; c < r1; c += 64) {
-        const u64 wm = R[J.off_j + (c >> 6)];
-        const u32 mlo = (u32)__builtin_amdgcn_readfirstlane(lo32(wm));
-        const u32 mhi = (u32)__builtin_amdgcn_readfirstlane(hi32(wm));
+    // the slice's mask words (at most 8), lane w holding word w
+    const int nw = (r1 - r0 + 63) >> 6;
+    const u64 wm = lane < nw ? R[J.off_j + (r0 >> 6) + lane] : 0;
+    for (int c = r0, w = 0; c < r1; c += 64, ++w) {
+        const u32 mlo = (u32)__builtin_amdgcn_readlane(lo32(wm), w);
+        const u32 mhi = (u32)__builtin_amdgcn_readlane(hi32(wm), w);
         const u64 mask = ((u64)mhi << 32) | mlo;
         if ((mask >> lane) & 1) {
             const int row = c + lane;
@@ -323,9 +325,12 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     // k*b^n+1: solve (-k) b^r g^x == 1, i.e. negate the giant-step starts
     u64 bh = montmul(H, c32, p, mp);
     if (J.plus && bh != 0) bh = p - bh;
+    // below 2^50 the Y products are made in double precision: BH and the
+    // plain G^t are stored as doubles (exact, < 2^50) and no Shoup word
+    const bool fpy = p < kFpMaxP;
     shoup_pair(B, p, mp, &w, &ws);
     for (int r = 0; r < J.Q; ++r) {
-        R[REC_BH + r] = bh;
+        R[REC_BH + r] = fpy ? (u64)__double_as_longlong((double)bh) : bh;
         bh = shoup_mul(bh, w, ws, p);
     }
     shoup_pair(G, p, mp, &w, &ws);
@@ -333,7 +338,7 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     for (int t = 0; t < J.T; ++t) {
         u64 gp, gs;
         shoup_pair(gt, p, mp, &gp, &gs);
-        R[J.off_gt + 2 * t] = gp;
+        R[J.off_gt + 2 * t] = fpy ? (u64)__double_as_longlong((double)gp) : gp;
         R[J.off_gt + 2 * t + 1] = gs;
         gt = shoup_mul(gt, w, ws, p);
     }
@@ -450,11 +455,13 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     // giant steps leave products in [0, 2p): an unreduced one is s + p with
     // s < p*kmax/2^32 <= bound, so such baby values go in twice
     const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;
-    if (p < kFpMaxP) {
-        // the chain in double precision (fp_mulmod: about a third of the
-        // integer Shoup step's instructions); each value is made canonical
-        // and its low word read from the bits of value + 2^52
-        const double pd = (double)p, pinv = 1.0 / pd, wd = (double)gw;
+    // p < 2^50: products in double precision (fp_mulmod: about a third of
+    // the integer Shoup step's instructions); values are made canonical and
+    // their low words read from the bits of value + 2^52
+    const bool fpm = p < kFpMaxP;
+    const double pd = (double)p, pinv = fpm ? 1.0 / pd : 0.0;
+    if (fpm) {
+        const double wd = (double)gw;
         const double bd = (double)bound;  // below 2^50
         double vd = (double)v;
         for (int s = 0; s < L; ++s) {
@@ -475,10 +482,22 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     // Y[r*ys + t]: the Shoup entry of b^r H G^t, made from its form times
     // 2^32, which is BH[r] times the plain G^t by Shoup's method
     const u32 mp0 = lo32(mp);
-    for (int idx = lane; idx < Q * T; idx += 64) {
-        const int r = div_T((u32)idx, J.invT), t = idx - r * T;
-        const u64 z = shoup_mul(R[REC_BH + r], R[J.off_gt + 2 * t], R[J.off_gt + 2 * t + 1], p);
-        Y[r * J.ys + t] = shoup32_entry(z, p, mp0);
+    if (fpm) {  // BH and G^t are doubles in the record
+        for (int idx = lane; idx < Q * T; idx += 64) {
+            const int r = div_T((u32)idx, J.invT), t = idx - r * T;
+            double zd = fp_mulmod(__longlong_as_double((long long)R[REC_BH + r]),
+                                  __longlong_as_double((long long)R[J.off_gt + 2 * t]), pd, pinv);
+            zd = zd < 0.0 ? zd + pd : zd;
+            const u64 z = fp_bits(zd) & ((1ull << 52) - 1);
+            Y[r * J.ys + t] = shoup32_entry(z, p, mp0);
+        }
+    } else {
+        for (int idx = lane; idx < Q * T; idx += 64) {
+            const int r = div_T((u32)idx, J.invT), t = idx - r * T;
+            const u64 z =
+                shoup_mul(R[REC_BH + r], R[J.off_gt + 2 * t], R[J.off_gt + 2 * t + 1], p);
+            Y[r * J.ys + t] = shoup32_entry(z, p, mp0);
+        }
     }
 
     // rows in slices of at most ROW_SLICE: those passing the Jacobi filter
