@@ -9,7 +9,7 @@
 #include "../host/host.h"
 #include "kernels.h"
 
-extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
+extern "C" __global__ void rs_sieve_kernel(ChunkDev c0, const JobDev* jobs,
                                            const ChunkDev* chunks, int nchunks,
                                            long long ngroups, int lds_words, int tab_words,
                                            int shared_words,
@@ -34,6 +34,7 @@ struct DevJob {
     uint32_t* d_rowinfo = nullptr;
     uint32_t* d_rowk = nullptr;
     uint32_t* d_bitmap = nullptr;
+    JobDev* d_self = nullptr;    // J itself in device memory
 };
 
 // Resources of the persistent kernel on one device: grid shape, per-wave
@@ -102,8 +103,10 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
 rs_status launch_classes(const std::vector<ClassRange>& cls, hipStream_t stream,
                          const JobDev* jobs, const ChunkDev* d_chunks);
 void free_multi(MultiLauncher& M);
-rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const ChunkDev& c0,
-                 const JobDev* jobs, const ChunkDev* chunks, int nchunks, long long ngroups);
+// jobs: device job constants, indexed by the chunks' .job (c0 when chunks
+// is null); a launcher with shared_words is for a single job.
+rs_status launch(const Launcher& L, hipStream_t stream, const JobDev* jobs, const ChunkDev& c0,
+                 const ChunkDev* chunks, int nchunks, long long ngroups);
 rs_status drain_hits(const Launcher& L, hipStream_t stream, std::vector<HitDev>* out,
                      int64_t* lost);
 bool hit_less(const HitDev& a, const HitDev& b);

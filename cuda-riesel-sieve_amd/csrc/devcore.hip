@@ -17,7 +17,7 @@
         if ((x) != hipSuccess) return RS_ERR_HIP; \
     } while (0)
 
-extern "C" __global__ void rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* jobs,
+extern "C" __global__ void rs_sieve_kernel(ChunkDev c0, const JobDev* jobs,
                                            const ChunkDev* chunks, int nchunks,
                                            long long ngroups, int lds_words, int tab_words,
                                            int shared_words,
@@ -43,6 +43,8 @@ void free_job(DevJob& j) {
     (void)hipFree(j.d_bitmap);
     j.d_kvals = nullptr;
     j.d_rowinfo = j.d_rowk = j.d_bitmap = nullptr;
+    (void)hipFree(j.d_self);
+    j.d_self = nullptr;
 }
 
 // Word offsets of the quadratic-character tables of the distinct k (~0u
@@ -76,7 +78,8 @@ size_t job_device_bytes(const rs_plan* pl, bool dat) {
     const uint64_t tab = jacobi_table_layout(distinct_k(pl), &offs);
     // row tables, per k (at most one per row) Jacobi constants, masks and
     // table offsets, the tables, and the n-list bitmaps with dat
-    return 8 * rows + 56 * rows + 4 * tab + (dat ? 4 * rows * (size_t)pl->words : 0) + 256;
+    return 8 * rows + 56 * rows + 4 * tab + (dat ? 4 * rows * (size_t)pl->words : 0) + 256 +
+           sizeof(JobDev);
 }
 
 // Jacobi constants of a 32-bit a > 0: odd part n, 2-adic valuation z and
@@ -207,6 +210,11 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
         HIPCHK(hipMemcpy(dj->d_bitmap, pl->bitmaps.data(), bytes, hipMemcpyHostToDevice));
         J.bitmap = dj->d_bitmap;
     }
+    // the constants themselves in device memory: single-job launches read
+    // them from there like batched ones (not as a kernel argument, whose
+    // words would stay in scalar registers for the whole kernel)
+    HIPCHK(hipMalloc(&dj->d_self, sizeof(JobDev)));
+    HIPCHK(hipMemcpy(dj->d_self, &J, sizeof(JobDev), hipMemcpyHostToDevice));
     return RS_OK;
 }
 
@@ -368,9 +376,8 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
 rs_status launch_classes(const std::vector<ClassRange>& cls, hipStream_t stream,
                          const JobDev* jobs, const ChunkDev* d_chunks) {
     for (const ClassRange& c : cls) {
-        JobDev j0{};
         ChunkDev c0{};
-        rs_status st = launch(c.L, stream, j0, c0, jobs, d_chunks + c.chunk0, c.nchunks,
+        rs_status st = launch(c.L, stream, jobs, c0, d_chunks + c.chunk0, c.nchunks,
                               c.ngroups);
         if (st != RS_OK) return st;
     }
@@ -382,14 +389,14 @@ void free_multi(MultiLauncher& M) {
     M.geo.clear();
 }
 
-rs_status launch(const Launcher& L, hipStream_t stream, const JobDev& j0, const ChunkDev& c0,
-                 const JobDev* jobs, const ChunkDev* chunks, int nchunks, long long ngroups) {
+rs_status launch(const Launcher& L, hipStream_t stream, const JobDev* jobs, const ChunkDev& c0,
+                 const ChunkDev* chunks, int nchunks, long long ngroups) {
     hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)L.work, 2);
     HIPCHK(hipGetLastError());
     const int grid = (int)std::min<long long>(L.grid, (ngroups + L.wpb - 1) / L.wpb);
     hipLaunchKernelGGL(rs_sieve_kernel, dim3(std::max(1, grid)), dim3(64 * L.wpb), L.lds_bytes,
-                       stream, j0, c0, jobs, chunks, nchunks, ngroups, L.lds_words,
-                       L.tab_words, jobs ? 0 : L.shared_words,
+                       stream, c0, jobs, chunks, nchunks, ngroups, L.lds_words,
+                       L.tab_words, L.shared_words,
                        L.scratch_words, L.scratch, L.work, L.hits, L.hit_count, L.hit_cap);
     HIPCHK(hipGetLastError());
     return RS_OK;

@@ -150,7 +150,7 @@ static rs_status sieve_launch(rs_sieve* s, uint64_t low, const uint32_t* d_offs,
     c0.count = count;
     c0.group0 = 0;
     c0.job = 0;
-    rs_status st = launch(s->L, s->stream, s->job.J, c0, nullptr, nullptr, 1, (count + 63) / 64);
+    rs_status st = launch(s->L, s->stream, s->job.d_self, c0, nullptr, 1, (count + 63) / 64);
     if (st != RS_OK) return st;
     HIPCHK(hipEventRecord(t.b, s->stream));
     s->timings.push_back(t);
@@ -243,7 +243,7 @@ extern "C" rs_status rs_sieve_submit_range(rs_sieve* s, uint64_t lo, uint64_t hi
         c0.offs = s->d_range;
         c0.count = s->range_cap;
         c0.count_dev = cnt_dev;
-        st = launch(s->L, s->stream, s->job.J, c0, nullptr, nullptr, 1, (s->range_cap + 63) / 64);
+        st = launch(s->L, s->stream, s->job.d_self, c0, nullptr, 1, (s->range_cap + 63) / 64);
         if (st != RS_OK) return st;
         HIPCHK(hipEventRecord(t.b, s->stream));
         hipLaunchKernelGGL(rs_accum_count, dim3(1), dim3(64), 0, s->stream, cnt_dev,

@@ -26,8 +26,9 @@
 // means x = x0 + t*m + (m - 1 - j), so t in [0, T) covers x in
 // [x0, x0 + T*m - 1] which contains [floor(minN/Q), floor(maxN/Q)].
 //
-// Chunks and jobs come either by value (c0, j0: one job, one chunk) or from
-// device tables (batched launches of many jobs).
+// Job constants are always read from a device table (one entry for a
+// single-job launch); chunks come by value (c0: one chunk) or from a device
+// table (batched launches of many jobs).
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -533,7 +534,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
 // At least 4 wavefronts per SIMD (128 VGPRs): the LDS layouts of typical
 // jobs allow 4, and fewer leave the VALU idle on LDS waits.
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
+rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
                 const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
                 int lds_words, int tab_words, int shared_words, int scratch_words,
                 uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
@@ -549,10 +550,11 @@ rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
     // slice keep the job's active-row entries (k, row_entry) there
     // (shared_words >= 2 * nrows), read by every prime
     uint2* const rows_lds = (uint2*)(lds + wpb * lds_words);
-    const bool rows_in_lds = shared_words > 0 && !jobs;
+    const bool rows_in_lds = shared_words > 0;  // single-job launch: job 0
     if (rows_in_lds) {
-        for (int i = threadIdx.x; i < j0.nrows; i += blockDim.x)
-            rows_lds[i] = make_uint2(j0.rowk[i], row_entry(j0.rowinfo[i] >> 16, (u32)j0.ys, i));
+        const JobDev J0 = uniform_load(jobs);
+        for (int i = threadIdx.x; i < J0.nrows; i += blockDim.x)
+            rows_lds[i] = make_uint2(J0.rowk[i], row_entry(J0.rowinfo[i] >> 16, (u32)J0.ys, i));
         __syncthreads();
     }
     // the kernel has no static LDS (checked by the host), so the dynamic
@@ -579,7 +581,7 @@ rs_sieve_kernel(JobDev j0, ChunkDev c0, const JobDev* __restrict__ jobs,
             }
         }
         const ChunkDev C = chunks ? uniform_load(chunks + ci) : c0;
-        const JobDev J = jobs ? uniform_load(jobs + C.job) : j0;
+        const JobDev J = uniform_load(jobs + C.job);
         const long long q0 = ((long long)grp - C.group0) * 64;
         const long long cnt =
             C.count_dev ? (long long)(u32)__builtin_amdgcn_readfirstlane(*C.count_dev) : C.count;
