@@ -176,7 +176,7 @@ __device__ __forceinline__ u64 lanes(bool c) { return __builtin_amdgcn_ballot_w6
 // fingerprint match is resolved by an exact rescan() of
 // the reduced product.  The loop is wave-uniform because rescan() needs
 // every lane.
-template <int TW, bool CHAIN>
+template <int TW, bool CHAIN, bool EXACT>
 __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int r0,
                                             u32 tab_addr,
                                             const u32* __restrict__ cnt,
@@ -190,13 +190,17 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
     const int T = J.T;
     const int total = A * T;
     const u32 negp0 = 0u - lo32(p);
+    // each lane's run (row a, first step t0) is carried from round to
+    // round: a round of 64*TW items moves it by rstep rows and tstep steps,
+    // tstep = 0 when EXACT (t0 < T always, so only a is clamped for lanes
+    // past the end)
+    int a = div_T((u32)(TW * lane), J.invT);
+    int t0 = TW * lane - a * T;
+    const int rstep = div_T(64u * TW, J.invT), tstep = 64 * TW - rstep * T;
     for (int base = 0; base < total; base += 64 * TW) {
-        const int i0 = base + TW * lane;
-        const bool ok = i0 < total;
+        const bool ok = a < A;
         const u64 okm = lanes(ok);
-        int a = div_T((u32)i0, J.invT), t0 = i0 - a * T;
-        if (!ok) a = A - 1, t0 = 0;
-        const uint2 row = list[a];
+        const uint2 row = list[min(a, A - 1)];
         const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * t0);
         uint4 e[TW];
         u32 fp2[TW], ba[TW];
@@ -250,6 +254,11 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
                 rescan(J, job, target, k, yent, r0, tt, vstart, g64, p, mp, hits, hit_count,
                        hit_cap);
             }
+        }
+        a += rstep;
+        if (!EXACT) {
+            t0 += tstep;
+            if (t0 >= T) t0 -= T, a += 1;
         }
     }
 }
@@ -509,20 +518,24 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
                                   : compact_rows<false>(J, rows_lds, R, list, r0, r1, T);
         wave_sync();
         const char* const Yb = (const char*)Y;
-#define RS_GIANT(TWV, CH)                                                                     \
-    giant_steps<TWV, CH>(J, job, A, r0, tab_addr, cnt, Yb, list, vstart, g64, p, mp, hits, \
-                         hit_count, hit_cap)
-#define RS_GIANT_TW(CH)      \
-    if (J.tw >= 4)           \
-        RS_GIANT(4, CH);     \
-    else if (J.tw == 2)      \
-        RS_GIANT(2, CH);     \
-    else                     \
-        RS_GIANT(1, CH);
+#define RS_GIANT(TWV, CH, EX)                                                                \
+    giant_steps<TWV, CH, EX>(J, job, A, r0, tab_addr, cnt, Yb, list, vstart, g64, p, mp, hits, \
+                             hit_count, hit_cap)
+#define RS_GIANT_TW(CH, EX)      \
+    if (J.tw >= 4)               \
+        RS_GIANT(4, CH, EX);     \
+    else if (J.tw == 2)          \
+        RS_GIANT(2, CH, EX);     \
+    else                         \
+        RS_GIANT(1, CH, EX);
+        // rounds of 64*TW items cover whole rows when T/TW divides 64
+        // (runs then keep their step t0); chain mode is rare and general
         if (chain) {
-            RS_GIANT_TW(true)
+            RS_GIANT_TW(true, false)
+        } else if ((64 * J.tw) % T == 0) {
+            RS_GIANT_TW(false, true)
         } else {
-            RS_GIANT_TW(false)
+            RS_GIANT_TW(false, false)
         }
 #undef RS_GIANT_TW
 #undef RS_GIANT
