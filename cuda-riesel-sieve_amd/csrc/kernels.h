@@ -4,19 +4,24 @@
 #include <stdint.h>
 
 // Record of one prime written by the set-up phase and read by the BSGS
-// phase (64-bit words).  All residues are Montgomery forms (R = 2^64).
-//   [0] p   [1] mp = -p^-1 mod 2^64   [2] one = R mod p   [3] g^64
-//   [4] g^64 mod p (plain)   [5] its Shoup constant floor(g^64 2^64 / p)
-//   [6 .. 14)   g^c for c = 0..7          (g = b^Q)
-//   [14 .. 22)  g^(8a) for a = 0..7
-//   [22 .. 22+Q)        BH[r] = b^r * H * 2^32, H = g^(x0 + m - 1)
-//   [22+Q .. 22+Q+2T)   G^t mod p (plain) and its Shoup constant, t < T,
-//                       G = g^m, interleaved
-//   [22+Q+2T .. +NW)    row masks: bit i of word w is set iff row 64w+i passes
+// phase (64-bit words).  Residues are Montgomery forms (R = 2^64) unless
+// marked plain.  For p < 2^50 ("fp": the BSGS phase multiplies in double
+// precision) some words hold doubles instead, as noted.
+//   [0] p   [1] mp = -p^-1 mod 2^64   [2] fp: 1/p, else unused   [3] g^64
+//   [4] g^64 mod p (plain; fp: as a double)
+//   [5] its Shoup constant floor(g^64 2^64 / p) (fp: the double of the
+//       second-entry bound (p >> 32) kmax + kmax + 1)
+//   [6 .. 14)   g^c for c = 0..7          (g = b^Q; fp: doubles)
+//   [14 .. 22)  g^(8a) for a = 0..7       (fp: plain, doubles)
+//   [22]        fp: p as a double
+//   [23 .. 23+Q)        BH[r] = b^r * H * 2^32, H = g^(x0 + m - 1) (fp: doubles)
+//   [23+Q .. 23+Q+2T)   G^t mod p (plain; fp: doubles) and its Shoup
+//                       constant, t < T, G = g^m, interleaved
+//   [23+Q+2T .. +NW)    row masks: bit i of word w is set iff row 64w+i passes
 //                       the Jacobi filter, (k/p) = +1 for even r and
 //                       (k*b/p) = +1 for odd r (NW = ceil(nrows / 64))
-enum { REC_P = 0, REC_MP = 1, REC_ONE = 2, REC_G64 = 3, REC_GW = 4, REC_GWS = 5, REC_GP = 6,
-       REC_G8P = 14, REC_BH = 22 };
+enum { REC_P = 0, REC_MP = 1, REC_PINV = 2, REC_G64 = 3, REC_GW = 4, REC_GWS = 5, REC_GP = 6,
+       REC_G8P = 14, REC_PD = 22, REC_BH = 23 };
 
 // Rows handled per compaction pass of the BSGS phase: bounds the LDS list
 // of active rows whatever the job's row count.

@@ -309,12 +309,16 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     const u64 g = mont_pow(B, (u32)J.Q, one, p, mp);
     // baby-step start powers: lane l of the BSGS wave starts at
     // g^(l & 7) * g^(8 (l >> 3))
+    // below 2^50 the BSGS phase multiplies in double precision: its
+    // constants are stored as doubles (exact integers below 2^50)
+    const bool fpy = p < kFpMaxP;
+    auto dbits = [](u64 v) { return (u64)__double_as_longlong((double)v); };
     u64 w, ws;
     shoup_pair(g, p, mp, &w, &ws);
     u64 x = one;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        R[REC_GP + c] = x;
+        R[REC_GP + c] = fpy ? dbits(x) : x;
         x = shoup_mul(x, w, ws, p);
     }
     const u64 g8 = x;
@@ -322,25 +326,25 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     x = one;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        R[REC_G8P + a] = x;
+        R[REC_G8P + a] = fpy ? dbits(redc(x, p, mp)) : x;  // fp: plain
         x = shoup_mul(x, w, ws, p);
     }
     const u64 g64 = x;
     shoup_pair(g64, p, mp, &w, &ws);
-    R[REC_GW] = w;
-    R[REC_GWS] = ws;
+    const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;  // second baby entries
+    R[REC_GW] = fpy ? dbits(w) : w;
+    R[REC_GWS] = fpy ? dbits(bound) : ws;
+    R[REC_PD] = dbits(p);
+    R[REC_PINV] = (u64)__double_as_longlong(fpy ? 1.0 / (double)p : 0.0);
     const u64 G = mont_pow(g64, (u32)(J.m >> 6), one, p, mp);
     const u64 H = mont_pow(g, (u32)(J.x0 + J.m - 1), one, p, mp);
     const u64 c32 = montmul(1ull << 32, r2, p, mp);  // Mont(2^32)
     // k*b^n+1: solve (-k) b^r g^x == 1, i.e. negate the giant-step starts
     u64 bh = montmul(H, c32, p, mp);
     if (J.plus && bh != 0) bh = p - bh;
-    // below 2^50 the Y products are made in double precision: BH and the
-    // plain G^t are stored as doubles (exact, < 2^50) and no Shoup word
-    const bool fpy = p < kFpMaxP;
     shoup_pair(B, p, mp, &w, &ws);
     for (int r = 0; r < J.Q; ++r) {
-        R[REC_BH + r] = fpy ? (u64)__double_as_longlong((double)bh) : bh;
+        R[REC_BH + r] = fpy ? dbits(bh) : bh;
         bh = shoup_mul(bh, w, ws, p);
     }
     shoup_pair(G, p, mp, &w, &ws);
@@ -348,7 +352,7 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     for (int t = 0; t < J.T; ++t) {
         u64 gp, gs;
         shoup_pair(gt, p, mp, &gp, &gs);
-        R[J.off_gt + 2 * t] = fpy ? (u64)__double_as_longlong((double)gp) : gp;
+        R[J.off_gt + 2 * t] = fpy ? dbits(gp) : gp;
         R[J.off_gt + 2 * t + 1] = gs;
         gt = shoup_mul(gt, w, ws, p);
     }
@@ -408,7 +412,6 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     if (cur + 1 < nw) RM[cur + 1] = hi;
     R[REC_P] = p;
     R[REC_MP] = mp;
-    R[REC_ONE] = one;
     R[REC_G64] = g64;
 }
 
@@ -444,9 +447,27 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     for (u32 c = 4 * lane; c < NB; c += 256) *(uint4*)(cnt + c) = make_uint4(0, 0, 0, 0);
     wave_sync();
 
+    // p < 2^50: products in double precision (fp_mulmod: about a third of
+    // the integer Shoup step's instructions); values are made canonical and
+    // their low words read from the bits of value + 2^52.  The record holds
+    // the constants as doubles.
+    const bool fpm = p < kFpMaxP;
+    auto rec_d = [&](int w) { return __longlong_as_double((long long)readlane64(pre, w)); };
+    auto lds_d = [](u64 v) { return __longlong_as_double((long long)v); };
+    const double pd = fpm ? rec_d(REC_PD) : 0.0, pinv = fpm ? rec_d(REC_PINV) : 0.0;
+
     // baby steps: lane l inserts g^j, j = l + 64 s, as a fingerprint in the
-    // first free slot of its bucket (or of the next non-full bucket)
-    const u64 vstart = montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp);
+    // first free slot of its bucket (or of the next non-full bucket); it
+    // starts from g^(l & 7) g^(8 (l >> 3))
+    u64 vstart;
+    double vd = 0.0;
+    if (fpm) {
+        vd = fp_mulmod(lds_d(R[REC_GP + (lane & 7)]), lds_d(R[REC_G8P + (lane >> 3)]), pd, pinv);
+        vd = vd < 0.0 ? vd + pd : vd;
+        vstart = fp_bits(vd) & ((1ull << 52) - 1);
+    } else {
+        vstart = montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp);
+    }
     u64 v = vstart;
     bool ovf = false;  // some insert went past a full bucket
     auto insert = [&](u64 val) {
@@ -465,15 +486,8 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     // giant steps leave products in [0, 2p): an unreduced one is s + p with
     // s < p*kmax/2^32 <= bound, so such baby values go in twice
     const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;
-    // p < 2^50: products in double precision (fp_mulmod: about a third of
-    // the integer Shoup step's instructions); values are made canonical and
-    // their low words read from the bits of value + 2^52
-    const bool fpm = p < kFpMaxP;
-    const double pd = (double)p, pinv = fpm ? 1.0 / pd : 0.0;
     if (fpm) {
-        const double wd = (double)gw;
-        const double bd = (double)bound;  // below 2^50
-        double vd = (double)v;
+        const double wd = rec_d(REC_GW), bd = rec_d(REC_GWS);  // g^64 and the bound
         for (int s = 0; s < L; ++s) {
             const double c = vd < 0.0 ? vd + pd : vd;
             insert(fp_bits(c));
