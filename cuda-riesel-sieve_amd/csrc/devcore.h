@@ -31,8 +31,7 @@ struct DevJob {
     uint32_t* d_jtab = nullptr;  // the tables
     uint32_t* d_kw0 = nullptr;   // row-mask words of each k
     uint64_t* d_kmask = nullptr;
-    uint32_t* d_rowinfo = nullptr;
-    uint32_t* d_rowk = nullptr;
+    uint2* d_rowent = nullptr;
     uint32_t* d_bitmap = nullptr;
     JobDev* d_self = nullptr;    // J itself in device memory
 };

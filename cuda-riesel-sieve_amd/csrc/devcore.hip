@@ -38,11 +38,11 @@ void free_job(DevJob& j) {
     (void)hipFree(j.d_kmask);
     j.d_kw0 = nullptr;
     j.d_kmask = nullptr;
-    (void)hipFree(j.d_rowinfo);
-    (void)hipFree(j.d_rowk);
+    (void)hipFree(j.d_rowent);
     (void)hipFree(j.d_bitmap);
     j.d_kvals = nullptr;
-    j.d_rowinfo = j.d_rowk = j.d_bitmap = nullptr;
+    j.d_rowent = nullptr;
+    j.d_bitmap = nullptr;
     (void)hipFree(j.d_self);
     j.d_self = nullptr;
 }
@@ -165,12 +165,17 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     for (size_t i = 0; i < kvals.size(); ++i) kjac[i] = jacobi_consts(kvals[i]);
     J.bjac = jacobi_consts(pl->base);
     HIPCHK(hipMalloc(&dj->d_kvals, kjac.size() * sizeof(uint4)));
-    HIPCHK(hipMalloc(&dj->d_rowinfo, rowinfo.size() * 4));
-    HIPCHK(hipMalloc(&dj->d_rowk, rowk.size() * 4));
+    // giant-step entries of the rows (the Y offset of r and the row within
+    // its slice)
+    std::vector<uint2> rowent(nrows);
+    for (int i = 0; i < nrows; ++i)
+        rowent[i] = make_uint2(rowk[i], ((rowinfo[i] >> 16) * (uint32_t)J.ys * 8u) |
+                                            ((uint32_t)(i % ROW_SLICE) << 19));
+    HIPCHK(hipMalloc(&dj->d_rowent, rowent.size() * sizeof(uint2)));
     HIPCHK(hipMemcpy(dj->d_kvals, kjac.data(), kjac.size() * sizeof(uint4),
                      hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dj->d_rowinfo, rowinfo.data(), rowinfo.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dj->d_rowk, rowk.data(), rowk.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dj->d_rowent, rowent.data(), rowent.size() * sizeof(uint2),
+                     hipMemcpyHostToDevice));
     J.kjac = dj->d_kvals;
     {
         std::vector<uint32_t> toffs;
@@ -200,8 +205,7 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     HIPCHK(hipMemcpy(dj->d_kmask, kmask.data(), kmask.size() * 8, hipMemcpyHostToDevice));
     J.kw0 = dj->d_kw0;
     J.kmask = dj->d_kmask;
-    J.rowinfo = dj->d_rowinfo;
-    J.rowk = dj->d_rowk;
+    J.rowent = dj->d_rowent;
     J.words = pl->words;
     J.bitmap = nullptr;
     if (dat) {
@@ -317,7 +321,11 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
             it = M->geo.emplace(jobs[j].lds_words, g).first;
         }
         if (jobs[j].W > M->w_cap) return RS_ERR_ARG;
-        key[j] = {(long long)it->second.grid * kMaxWavesPerBlock + it->second.wpb, 0};
+        // occupancy, then table size: a class then takes its members' table
+        // and largest rest, whose sum is at most its largest member's layout
+        key[j] = {((long long)it->second.grid * kMaxWavesPerBlock + it->second.wpb) << 24 |
+                      jobs[j].tab_words,
+                  0};
     }
     // classes in order of decreasing occupancy; a class takes the largest
     // table, the largest rest of the layout and the largest record of its

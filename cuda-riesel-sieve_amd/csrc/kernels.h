@@ -60,8 +60,8 @@ struct JobDev {
     const uint32_t* kw0;      // per distinct k: 64-row word holding its first row
     const uint64_t* kmask;    // per distinct k: its even-r rows in words w0 and
                               // w0 + 1, then its odd-r rows in the same words
-    const uint32_t* rowinfo;  // per row: kidx | r << 16
-    const uint32_t* rowk;     // per row: k
+    const uint2* rowent;      // per row: its giant-step entry (k, 8 r ys |
+                              // (row mod ROW_SLICE) << 19)
     const uint32_t* bitmap;   // optional n-list bitmaps (rows x words, MSB first)
     int words;                // 32-bit words per row bitmap
 };

@@ -97,11 +97,9 @@ __device__ __forceinline__ bool bucket_match(uint4 e, u32 fp2) {
     return f.x == 0;
 }
 
-// Active-row entries of the giant steps: .x = k, .y = 8*r*ys (byte offset
-// of the row's Y run, ys the Y row stride; < 2^19) | (row - slice start) << 19.
-__device__ __forceinline__ u32 row_entry(u32 r, u32 ys, u32 rowrel) {
-    return (r * ys * 8u) | (rowrel << 19);
-}
+// Active-row entries of the giant steps (built on the host, JobDev.rowent):
+// .x = k, .y = 8*r*ys (byte offset of the row's Y run, ys the Y row stride;
+// < 2^19) | (row - slice start) << 19.
 
 
 
@@ -277,22 +275,34 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
     // the slice's mask words (at most 8), lane w holding word w
     const int nw = (r1 - r0 + 63) >> 6;
     const u64 wm = lane < nw ? R[J.off_j + (r0 >> 6) + lane] : 0;
-    for (int c = r0, w = 0; c < r1; c += 64, ++w) {
+    // passing lanes (rows below r1 only) write their entry at their rank
+    auto put = [&](int w, auto entry) {
         const u32 mlo = (u32)__builtin_amdgcn_readlane(lo32(wm), w);
         const u32 mhi = (u32)__builtin_amdgcn_readlane(hi32(wm), w);
         const u64 mask = ((u64)mhi << 32) | mlo;
         if ((mask >> lane) & 1) {
-            const int row = c + lane;
-            // the block-shared table holds finished entries (one slice: r0 = 0)
-            const uint2 ent = FROM_LDS ? rows_lds[row]
-                                       : make_uint2(J.rowk[row], row_entry(J.rowinfo[row] >> 16,
-                                                                           (u32)J.ys,
-                                                                           (u32)(row - r0)));
             const u32 below = __builtin_amdgcn_mbcnt_hi(hi32(mask),
                                                         __builtin_amdgcn_mbcnt_lo(lo32(mask), 0));
-            list[A + below] = ent;
+            list[A + below] = entry();
         }
         A += __popcll(mask);
+    };
+    if (FROM_LDS) {
+        // finished entries in block-shared LDS
+        for (int c = r0, w = 0; c < r1; c += 64, ++w)
+            put(w, [&] { return rows_lds[c + lane]; });
+    } else {
+        // from the job's table in global memory: every load of the slice
+        // issued before the first is used (ROW_SLICE = 8 x 64 rows)
+        uint2 ent[ROW_SLICE / 64];
+#pragma unroll
+        for (int w = 0; w < ROW_SLICE / 64; ++w) {
+            const int row = r0 + 64 * w + lane;
+            ent[w] = row < r1 ? J.rowent[row] : make_uint2(0, 0);
+        }
+#pragma unroll
+        for (int w = 0; w < ROW_SLICE / 64; ++w)
+            if (w < nw) put(w, [&] { return ent[w]; });
     }
     return A;
 }
@@ -574,14 +584,14 @@ rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
     // least the table of every job of the launch, so each is aligned to its
     // size), then the rest of each wavefront's words (lds_words - tab_words),
     // then the block-shared part: single-job launches whose rows fit one
-    // slice keep the job's active-row entries (k, row_entry) there
+    // slice keep the job's active-row entries (JobDev.rowent) there
     // (shared_words >= 2 * nrows), read by every prime
     uint2* const rows_lds = (uint2*)(lds + wpb * lds_words);
-    const bool rows_in_lds = shared_words > 0;  // single-job launch: job 0
+    // (shared_words > 0 only for single-chunk launches, whose job is c0.job)
+    const bool rows_in_lds = shared_words > 0;
     if (rows_in_lds) {
-        const JobDev J0 = uniform_load(jobs);
-        for (int i = threadIdx.x; i < J0.nrows; i += blockDim.x)
-            rows_lds[i] = make_uint2(J0.rowk[i], row_entry(J0.rowinfo[i] >> 16, (u32)J0.ys, i));
+        const JobDev J0 = uniform_load(jobs + c0.job);
+        for (int i = threadIdx.x; i < J0.nrows; i += blockDim.x) rows_lds[i] = J0.rowent[i];
         __syncthreads();
     }
     // the kernel has no static LDS (checked by the host), so the dynamic
@@ -591,21 +601,32 @@ rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
     u32* const wbase = lds + wpb * tab_words + wv * (lds_words - tab_words);
     u64* const S = scratch + ((long long)blockIdx.x * wpb + wv) * scratch_words;
 
+    int ci = 0;                       // chunk of the previous group
+    long long c_lo = 0, c_hi = -1;    // its group range [c_lo, c_hi)
     while (true) {
         unsigned long long grp = 0;
         if (lane == 0) grp = atomicAdd(work, 1ull);
         grp = ((unsigned long long)(u32)__builtin_amdgcn_readfirstlane((u32)(grp >> 32)) << 32) |
               (u32)__builtin_amdgcn_readfirstlane((u32)grp);
         if ((long long)grp >= ngroups) break;
-        // the chunk holding this group (chunks sorted by group0)
-        int ci = 0;
-        if (chunks) {
+        // the chunk holding this group (chunks sorted by group0); a wave's
+        // next group is almost always in the same chunk, so the last one
+        // found and its group range are kept
+        if (chunks && ((long long)grp < c_lo || (long long)grp >= c_hi)) {
             int hi = nchunks - 1;
+            ci = 0;
             while (ci < hi) {
                 const int mid = (ci + hi + 1) >> 1;
                 if (chunks[mid].group0 <= (long long)grp) ci = mid;
                 else hi = mid - 1;
             }
+            ci = __builtin_amdgcn_readfirstlane(ci);
+            c_lo = chunks[ci].group0;
+            c_hi = ci + 1 < nchunks ? chunks[ci + 1].group0 : ngroups;
+            c_lo = ((long long)__builtin_amdgcn_readfirstlane((int)(c_lo >> 32)) << 32) |
+                   (u32)__builtin_amdgcn_readfirstlane((u32)c_lo);
+            c_hi = ((long long)__builtin_amdgcn_readfirstlane((int)(c_hi >> 32)) << 32) |
+                   (u32)__builtin_amdgcn_readfirstlane((u32)c_hi);
         }
         const ChunkDev C = chunks ? uniform_load(chunks + ci) : c0;
         const JobDev J = uniform_load(jobs + C.job);

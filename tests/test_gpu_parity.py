@@ -508,6 +508,34 @@ def test_batched_jobs_equal_alone_and_oracle(rs, orc, gpu):
     bat.close()
 
 
+def test_batch_large_chunk_single_launch_equals_plain(rs, gpu):
+    """A chunk large enough for its own single-job launch inside the batch
+    graph (row table in block-shared LDS), next to small chunks of other jobs
+    in class launches: every job's hits equal its plain run, on replay too."""
+    big = rs.Plan(rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2))
+    small = rs.Plan(rs.Abcd.synth(5, 6, 3, 10**6, 20000, 60000, 1000, seed=8, odd_k=False))
+    low = 10**12 + 1
+    gpg = rs.GpuPrimeGen(low + (1 << 28))
+    buf = gpg.device_range(low, low + (1 << 28))
+    assert buf.count > 9_000_000
+    sbuf = gpg.device_range(low, low + (1 << 21))
+    bat = rs.Batch([big, small], [512, 256])
+    bat.add_chunk(0, low, buf)
+    bat.add_chunk(1, low, sbuf)
+    bat.run()
+    first = bat.collect()
+    bat.run()
+    assert bat.collect() == first
+    bat.close()
+    for j, (pl, m, b) in enumerate(((big, 512, buf), (small, 256, sbuf))):
+        sv = rs.Sieve(pl, m=m)
+        sv.submit_device(low, b)
+        want = sv.collect()
+        sv.close()
+        assert [(p, k, n) for (jj, p, k, n) in first if jj == j] == want, j
+    assert any(jj == 0 for (jj, _, _, _) in first)
+
+
 def test_gpu_primes_equal_cpu_generator(rs, gpu):
     """GPU segmented sieve (+ Miller-Rabin above the sieving bound) vs the
     host generator, including tiny ranges, 1e12 and ranges near 2^62/2^63."""
