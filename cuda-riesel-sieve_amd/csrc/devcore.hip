@@ -381,6 +381,53 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
     return RS_OK;
 }
 
+rs_status plan_launches(MultiLauncher* M, const std::vector<JobDev>& jobs,
+                        std::vector<ChunkDev>* chunks, std::vector<SingleLaunch>* singles,
+                        std::vector<ClassRange>* cls) {
+    singles->clear();
+    cls->clear();
+    std::vector<ChunkDev> rest;
+    for (const ChunkDev& c : *chunks) {
+        if (c.count == 0) continue;
+        const JobDev& J = jobs[c.job];
+        const int shared = J.nrows <= ROW_SLICE ? (2 * J.nrows + 3) & ~3 : 0;
+        SingleLaunch s;
+        rs_status st = launch_geometry(M->base.device, J.lds_words, &s.L, shared);
+        if (st != RS_OK) return st;
+        const long long groups = (c.count + 63) / 64;
+        if (groups < (long long)kSingleGroupsPerWave * s.L.grid * s.L.wpb ||
+            J.W > M->w_cap) {
+            rest.push_back(c);
+            continue;
+        }
+        s.L.tab_words = J.tab_words;
+        s.L.scratch_words = 64 * J.W;
+        s.L.scratch = M->base.scratch;
+        s.L.work = M->base.work;
+        s.L.hits = M->base.hits;
+        s.L.hit_count = M->base.hit_count;
+        s.L.hit_cap = M->base.hit_cap;
+        if ((size_t)s.L.grid * s.L.wpb * s.L.scratch_words > M->max_waves * 64 * (size_t)M->w_cap) {
+            rest.push_back(c);
+            continue;
+        }
+        s.c0 = c;
+        s.c0.group0 = 0;
+        singles->push_back(s);
+    }
+    *chunks = std::move(rest);
+    return chunks->empty() ? RS_OK : plan_classes(M, jobs, chunks, cls);
+}
+
+rs_status launch_singles(const std::vector<SingleLaunch>& singles, hipStream_t stream,
+                         const JobDev* jobs) {
+    for (const SingleLaunch& s : singles) {
+        rs_status st = launch(s.L, stream, jobs, s.c0, nullptr, 1, (s.c0.count + 63) / 64);
+        if (st != RS_OK) return st;
+    }
+    return RS_OK;
+}
+
 rs_status launch_classes(const std::vector<ClassRange>& cls, hipStream_t stream,
                          const JobDev* jobs, const ChunkDev* d_chunks) {
     for (const ClassRange& c : cls) {

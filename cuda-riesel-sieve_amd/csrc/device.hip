@@ -450,7 +450,7 @@ struct rs_batch {
     JobDev* d_jobs = nullptr;
     MultiLauncher M;
     std::vector<ClassRange> classes;
-    std::vector<std::pair<Launcher, ChunkDev>> singles;  // large chunks, own launch
+    std::vector<SingleLaunch> singles;  // large chunks, own launch
     hipStream_t stream = nullptr;
     std::vector<ChunkDev> chunks;      // as added; ordered by class at capture
     ChunkDev* d_chunks = nullptr;
@@ -550,46 +550,14 @@ extern "C" rs_status rs_batch_clear(rs_batch* b) {
     return RS_OK;
 }
 
-// A chunk of at least this many 64-prime groups per resident wave of its
-// job's own grid gets a single-job launch: launch tails are then negligible,
-// and the job's row table can sit in block-shared LDS, which a launch
-// mixing jobs cannot do (about 2-8% faster on 2^30-wide chunks).
-static const int kSingleGroupsPerWave = 32;
-
 // Gives large chunks single-job launches and orders the rest by occupancy
 // class, uploads the chunk table and captures one zero + kernel pair per
 // launch into a graph.
 static rs_status batch_capture(rs_batch* b) {
     std::vector<JobDev> host(b->jobs.size());
     for (size_t i = 0; i < b->jobs.size(); ++i) host[i] = b->jobs[i].J;
-    std::vector<ChunkDev> ordered;
-    b->singles.clear();
-    for (const ChunkDev& c : b->chunks) {
-        const JobDev& J = host[c.job];
-        const int shared = J.nrows <= ROW_SLICE ? (2 * J.nrows + 3) & ~3 : 0;
-        Launcher L;
-        rs_status st = launch_geometry(b->device, J.lds_words, &L, shared);
-        if (st != RS_OK) return st;
-        const long long groups = (c.count + 63) / 64;
-        if (groups < (long long)kSingleGroupsPerWave * L.grid * L.wpb) {
-            ordered.push_back(c);
-            continue;
-        }
-        L.tab_words = J.tab_words;
-        L.scratch_words = 64 * J.W;
-        L.scratch = b->M.base.scratch;
-        L.work = b->M.base.work;
-        L.hits = b->M.base.hits;
-        L.hit_count = b->M.base.hit_count;
-        L.hit_cap = b->M.base.hit_cap;
-        if ((size_t)L.grid * L.wpb * L.scratch_words > b->M.max_waves * 64 * (size_t)b->M.w_cap)
-            return RS_ERR_ARG;
-        ChunkDev c0 = c;
-        c0.group0 = 0;
-        b->singles.emplace_back(L, c0);
-    }
-    b->classes.clear();
-    rs_status st = ordered.empty() ? RS_OK : plan_classes(&b->M, host, &ordered, &b->classes);
+    std::vector<ChunkDev> ordered = b->chunks;
+    rs_status st = plan_launches(&b->M, host, &ordered, &b->singles, &b->classes);
     if (st != RS_OK) return st;
     if (ordered.size() > b->d_chunks_cap) {
         (void)hipFree(b->d_chunks);
@@ -610,11 +578,7 @@ static rs_status batch_capture(rs_batch* b) {
         b->graph = nullptr;
     }
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (const auto& sc : b->singles) {
-        st = launch(sc.first, b->stream, b->d_jobs, sc.second, nullptr, 1,
-                    (sc.second.count + 63) / 64);
-        if (st != RS_OK) break;
-    }
+    st = launch_singles(b->singles, b->stream, b->d_jobs);
     if (st == RS_OK) st = launch_classes(b->classes, b->stream, b->d_jobs, b->d_chunks);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &graph);

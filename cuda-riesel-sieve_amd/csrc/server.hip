@@ -295,8 +295,8 @@ extern "C" rs_status rs_server_step(rs_server* s, int* active, int* queued,
         wins[k].count = s->h_counts[k];
         if ((int64_t)wins[k].count > s->buf_cap) return RS_ERR_OVERFLOW;
     }
-    // 3. every active job's window: one persistent launch per occupancy
-    //    class of the active jobs' LDS layouts
+    // 3. every active job's window: large windows in single-job launches,
+    //    the rest in one persistent launch per occupancy class
     std::vector<ChunkDev> chunks;
     std::vector<JobDev> jhost;
     long long step_primes = 0;
@@ -312,14 +312,18 @@ extern "C" rs_status rs_server_step(rs_server* s, int* active, int* queued,
         step_primes += c.count;
         chunks.push_back(c);
     }
-    if (!chunks.empty()) {
+    const bool launched = !chunks.empty();
+    if (launched) {
         std::vector<ClassRange> cls;
-        rs_status st = plan_classes(&s->M, jhost, &chunks, &cls);
+        std::vector<SingleLaunch> singles;
+        rs_status st = plan_launches(&s->M, jhost, &chunks, &singles, &cls);
         if (st != RS_OK) return st;
-        HIPCHK(hipMemcpyAsync(s->d_chunks, chunks.data(), sizeof(ChunkDev) * chunks.size(),
-                              hipMemcpyHostToDevice, s->stream));
+        if (!chunks.empty())
+            HIPCHK(hipMemcpyAsync(s->d_chunks, chunks.data(), sizeof(ChunkDev) * chunks.size(),
+                                  hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipEventRecord(s->ev_a, s->stream));
-        st = launch_classes(cls, s->stream, s->d_jobs, s->d_chunks);
+        st = launch_singles(singles, s->stream, s->d_jobs);
+        if (st == RS_OK) st = launch_classes(cls, s->stream, s->d_jobs, s->d_chunks);
         if (st != RS_OK) return st;
         HIPCHK(hipEventRecord(s->ev_b, s->stream));
     }
@@ -329,7 +333,7 @@ extern "C" rs_status rs_server_step(rs_server* s, int* active, int* queued,
     rs_status st = drain_hits(s->M.base, s->stream, &hits, &lost);
     if (st != RS_OK) return st;
     if (lost) return RS_ERR_OVERFLOW;
-    if (!chunks.empty()) {
+    if (launched) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, s->ev_a, s->ev_b));
         s->kernel_ms += ms;
