@@ -68,8 +68,10 @@ __device__ __forceinline__ int div_T(u32 idx, u32 invT) {
 // is lo & (nbm << 4), which a table aligned to its size adds by OR.  (Above
 // 4096 buckets the bucket bits reach into the fingerprint, whose distinct
 // bits within a bucket drop below 16: only the rescan rate grows.)
-__device__ __forceinline__ u32 bucket_of(u64 v, u32 nbm) { return (lo32(v) >> 4) & nbm; }
-__device__ __forceinline__ u32 fingerprint(u64 v) { return lo32(v) >> 16; }
+__device__ __forceinline__ u32 bucket_of(u32 lo, u32 logNB) {
+    return __builtin_amdgcn_ubfe(lo, 4, logNB);
+}
+__device__ __forceinline__ unsigned short fingerprint(u32 lo) { return (unsigned short)(lo >> 16); }
 // The fingerprint of a low word in both halves (one byte permute).
 __device__ __forceinline__ u32 fingerprint2(u32 lo) {
     return __builtin_amdgcn_perm(lo, lo, 0x03020302u);
@@ -439,7 +441,6 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     const u32 NB = 1u << J.logNB;
     const u32 nbm = NB - 1;
     const int Q = J.Q, T = J.T, m = J.m, W = J.W;
-    uint16_t* const tab16 = (uint16_t*)tab;
     u32* const cnt = wbase;
     u64* const Y = (u64*)(cnt + NB);
     uint2* const list = (uint2*)(Y + Q * J.ys);
@@ -481,8 +482,12 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     u64 v = vstart;
     bool ovf = false;  // some insert went past a full bucket
     auto insert = [&](u64 val) {
-        u32 b = bucket_of(val, nbm);
-        const u32 fp = fingerprint(val);
+        const u32 lo = lo32(val);
+        u32 b = bucket_of(lo, (u32)J.logNB);
+        // the bucket's LDS byte address, kept as one value (the empty asm
+        // stops the slot offset being folded back into b, one more add)
+        u32 ta = tab_addr + 16 * b;
+        asm volatile("" : "+v"(ta));
         u32 c = atomicAdd(&cnt[b], 1u);
         if (c >= 8) {  // rare: walk to the first bucket with a free slot
             ovf = true;
@@ -490,8 +495,9 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
                 b = (b + 1) & nbm;
                 c = atomicAdd(&cnt[b], 1u);
             } while (c >= 8);
+            ta = tab_addr + 16 * b;
         }
-        tab16[b * 8 + c] = (uint16_t)fp;
+        *(__attribute__((address_space(3))) unsigned short*)(size_t)(ta + 2 * c) = fingerprint(lo);
     };
     // giant steps leave products in [0, 2p): an unreduced one is s + p with
     // s < p*kmax/2^32 <= bound, so such baby values go in twice
