@@ -112,8 +112,16 @@ __device__ __forceinline__ u64 shoup32_entry(u64 z, u64 p, u32 mp0) {
 // Low word of Shoup's product k*y - floor(k*yq/2^32)*p for a 32-bit k and
 // the entry yq:lo(y): the product lies in [0, 2p) and exceeds p only when
 // the reduced product is below p*k/2^32.  negp0 = -lo(p) mod 2^32.
+// (Three instructions: k*ylo as a 64-bit product, the high product, and
+// one multiply-add whose low word is the result.  Written in C, the
+// compiler turns the last step into a multiply and a subtract because only
+// the low word is used; the one-instruction asm keeps the multiply-add.)
 __device__ __forceinline__ u32 shoup32_lo(u32 k, u32 ylo, u32 yq, u32 negp0) {
-    return lo32(mad32(__umulhi(k, yq), negp0, (u64)(k * ylo)));
+    const u64 ky = mad32(k, ylo, 0);
+    const u32 q = __umulhi(k, yq);
+    u64 r, carry;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(q), "v"(negp0), "v"(ky));
+    return lo32(r);
 }
 
 // k*y mod p from the entry yq:lo(y).  y is recovered from it: since

@@ -186,7 +186,8 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
                                             u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
     const u32 nbm = (1u << J.logNB) - 1;
-    const u32 bmask = nbm << 4;
+    u32 bmask = nbm << 4;
+    asm volatile("" : "+v"(bmask));  // in a vector register: one and-or per address
     const int T = J.T;
     const int total = A * T;
     const u32 negp0 = 0u - lo32(p);
