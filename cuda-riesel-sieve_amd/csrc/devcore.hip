@@ -17,13 +17,7 @@
         if ((x) != hipSuccess) return RS_ERR_HIP; \
     } while (0)
 
-extern "C" __global__ void rs_sieve_kernel(ChunkDev c0, const JobDev* jobs,
-                                           const ChunkDev* chunks, int nchunks,
-                                           long long ngroups, int lds_words, int tab_words,
-                                           int shared_words,
-                                           int scratch_words, uint64_t* scratch,
-                                           unsigned long long* work, HitDev* hits,
-                                           uint32_t* hit_count, uint32_t hit_cap);
+
 extern "C" __global__ void rs_zero_u32(uint32_t*, int);
 extern "C" __global__ void rs_jacobi_table(uint32_t n, uint32_t words, uint32_t* out);
 
@@ -101,6 +95,8 @@ uint32_t magic_div(int d) {
 rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     if (!pl) return RS_ERR_ARG;
     if (m <= 0) m = rs_plan_auto_m(pl);
+    const bool row_index = (tw & RS_TILE_ROW_INDEX) != 0;
+    tw &= ~RS_TILE_ROW_INDEX;
     if (tw <= 0) tw = 2;
     if (tw > 4) return RS_ERR_ARG;
     if (m < 64 || m % 64 != 0 || m > (1 << 16)) return RS_ERR_ARG;
@@ -148,7 +144,9 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.W = rec_words(J.Q, J.T, nrows);
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + 2 * J.T;
-    J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.ys, nrows, J.W);
+    J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.ys, nrows, J.W, false);
+    J.lds_words_idx = lds_words_per_wave(J.logNB, J.Q, J.ys, nrows, J.W, true);
+    J.list_idx = row_index && nrows <= ROW_SLICE ? 1 : 0;
     J.tab_words = 4 << J.logNB;
     // per k: the 64-row word of its first row and its even-r / odd-r rows
     // in that word and the next (a k has at most Q < 64 rows)
@@ -235,7 +233,12 @@ void free_launcher(Launcher& L) {
 
 // Sizes the grid for lds_words per wave and record size W (both maxima over
 // the jobs of the launch) and allocates scratch, counter and hits.
-rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words) {
+static const void* kernel_of(bool rix) {
+    return rix ? reinterpret_cast<const void*>(rs_sieve_kernel_rix)
+               : reinterpret_cast<const void*>(rs_sieve_kernel);
+}
+
+rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words, bool rix) {
     L->device = device;
     int max_lds = 0, cus = 0;
     HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -249,11 +252,11 @@ rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_wor
     // the kernel addresses its tables from LDS address 0: it must have no
     // static LDS before the dynamic region
     hipFuncAttributes fa{};
-    HIPCHK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(rs_sieve_kernel)));
+    HIPCHK(hipFuncGetAttributes(&fa, kernel_of(rix)));
     if (fa.sharedSizeBytes != 0) return RS_ERR_HIP;
     // the attribute is per function: always the device limit, so launches
     // of different LDS sizes never restrict one another
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(rs_sieve_kernel),
+    HIPCHK(hipFuncSetAttribute(kernel_of(rix),
                                hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
     int best_w = 0, best_wpb = 1, best_cu = 1;
     for (int wpb = 1; wpb <= kMaxWavesPerBlock; ++wpb) {
@@ -261,7 +264,7 @@ rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_wor
         if (bytes > (size_t)max_lds) break;
         int per_cu = 0;
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(rs_sieve_kernel), 64 * wpb, bytes));
+            &per_cu, kernel_of(rix), 64 * wpb, bytes));
         if (per_cu * wpb >= best_w) {
             best_w = per_cu * wpb;
             best_wpb = wpb;
@@ -269,6 +272,7 @@ rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_wor
         }
     }
     L->wpb = best_wpb;
+    L->rix = rix;
     L->lds_words = lds_words;
     L->shared_words = shared_words;
     L->lds_bytes = per_wave * L->wpb + shared;
@@ -288,8 +292,8 @@ static rs_status alloc_buffers(Launcher* L, size_t scratch_words_total, int64_t 
 }
 
 rs_status init_launcher(Launcher* L, int device, int lds_words, int tab_words, int W,
-                        int64_t hit_capacity, int shared_words) {
-    rs_status st = launch_geometry(device, lds_words, L, shared_words);
+                        int64_t hit_capacity, int shared_words, bool rix) {
+    rs_status st = launch_geometry(device, lds_words, L, shared_words, rix);
     if (st != RS_OK) return st;
     L->tab_words = tab_words;
     L->scratch_words = 64 * W;
@@ -298,12 +302,16 @@ rs_status init_launcher(Launcher* L, int device, int lds_words, int tab_words, i
 
 rs_status init_multi(MultiLauncher* M, int device, int w_cap, int64_t hit_capacity) {
     M->w_cap = w_cap;
-    // the smallest layout gives the most resident waves: size scratch for it
-    Launcher g;
-    rs_status st = launch_geometry(device, 64, &g);
-    if (st != RS_OK) return st;
+    // the smallest layout gives the most resident waves (of either kernel):
+    // size scratch for it
+    M->max_waves = 0;
+    for (bool rix : {false, true}) {
+        Launcher g;
+        rs_status st = launch_geometry(device, 64, &g, 0, rix);
+        if (st != RS_OK) return st;
+        M->max_waves = std::max(M->max_waves, (size_t)g.grid * g.wpb);
+    }
     M->base.device = device;
-    M->max_waves = (size_t)g.grid * g.wpb;
     return alloc_buffers(&M->base, M->max_waves * 64 * (size_t)w_cap, hit_capacity);
 }
 
@@ -392,7 +400,9 @@ rs_status plan_launches(MultiLauncher* M, const std::vector<JobDev>& jobs,
         const JobDev& J = jobs[c.job];
         const int shared = J.nrows <= ROW_SLICE ? (2 * J.nrows + 3) & ~3 : 0;
         SingleLaunch s;
-        rs_status st = launch_geometry(M->base.device, J.lds_words, &s.L, shared);
+        const bool rix = shared && J.list_idx;
+        rs_status st = launch_geometry(M->base.device, rix ? J.lds_words_idx : J.lds_words,
+                                       &s.L, shared, rix);
         if (st != RS_OK) return st;
         const long long groups = (c.count + 63) / 64;
         if (groups < (long long)kSingleGroupsPerWave * s.L.grid * s.L.wpb ||
@@ -449,9 +459,9 @@ rs_status launch(const Launcher& L, hipStream_t stream, const JobDev* jobs, cons
     hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)L.work, 2);
     HIPCHK(hipGetLastError());
     const int grid = (int)std::min<long long>(L.grid, (ngroups + L.wpb - 1) / L.wpb);
-    hipLaunchKernelGGL(rs_sieve_kernel, dim3(std::max(1, grid)), dim3(64 * L.wpb), L.lds_bytes,
-                       stream, c0, jobs, chunks, nchunks, ngroups, L.lds_words,
-                       L.tab_words, L.shared_words,
+    hipLaunchKernelGGL(L.rix ? rs_sieve_kernel_rix : rs_sieve_kernel, dim3(std::max(1, grid)),
+                       dim3(64 * L.wpb), L.lds_bytes, stream, c0, jobs, chunks, nchunks, ngroups,
+                       L.lds_words, L.tab_words, L.shared_words,
                        L.scratch_words, L.scratch, L.work, L.hits, L.hit_count, L.hit_cap);
     HIPCHK(hipGetLastError());
     return RS_OK;

@@ -108,8 +108,11 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int t
     const int nrows = s->job.J.nrows;
     const int shared = nrows <= ROW_SLICE ? (2 * nrows + 3) & ~3 : 0;
     if (st == RS_OK)
-        st = init_launcher(&s->L, device, s->job.J.lds_words, s->job.J.tab_words, s->job.J.W,
-                           hit_capacity, shared);
+        st = init_launcher(&s->L, device,
+                           shared && s->job.J.list_idx ? s->job.J.lds_words_idx
+                                                       : s->job.J.lds_words,
+                           s->job.J.tab_words, s->job.J.W, hit_capacity, shared,
+                           shared && s->job.J.list_idx);
     if (st == RS_OK &&
         (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
          hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess))
@@ -129,11 +132,11 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int t
 extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
                                    int* lds_bytes_per_wave, int* tile) {
     if (!s) return RS_ERR_ARG;
-    if (tile) *tile = s->job.J.tw;
+    if (tile) *tile = s->job.J.tw | (s->job.J.list_idx ? RS_TILE_ROW_INDEX : 0);
     if (m) *m = s->job.J.m;
     if (T) *T = s->job.J.T;
     if (table_slots) *table_slots = 8 << s->job.J.logNB;
-    if (lds_bytes_per_wave) *lds_bytes_per_wave = s->job.J.lds_words * 4;
+    if (lds_bytes_per_wave) *lds_bytes_per_wave = s->L.lds_words * 4;
     return RS_OK;
 }
 
@@ -397,9 +400,11 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
             const int m = round64(m0 * num / 8);
             if (std::find(ms.begin(), ms.end(), m) == ms.end()) ms.push_back(m);
         }
+        const int rix = pl->rows_k.size() <= (size_t)ROW_SLICE ? RS_TILE_ROW_INDEX : 0;
         for (int m : ms) {
             bool fits = true;
-            for (int tile : {1, 2, 4}) {
+            for (int tile : {1, 2, 4, 1 | rix, 2 | rix, 4 | rix}) {
+                if (tile > 4 && !rix) continue;
                 rs_status st = tu.time(m, tile, &ns);
                 if (st == RS_ERR_ARG) {  // larger m will not fit either
                     fits = false;
@@ -434,6 +439,11 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
                 prev = ns;
             }
             if (tu.best_m != m) break;  // improved going up: skip going down
+        }
+        // the row-index layout at the best pair
+        if (pl->rows_k.size() <= (size_t)ROW_SLICE) {
+            st = tu.time(tu.best_m, tu.best_tile | RS_TILE_ROW_INDEX, &ns);
+            if (st != RS_OK && st != RS_ERR_ARG) return st;
         }
     }
     if (ncand) *ncand = tu.n;
@@ -520,7 +530,7 @@ extern "C" rs_status rs_batch_create(int device, int njobs, const rs_plan* const
 extern "C" rs_status rs_batch_job_info(const rs_batch* b, int job, int* m, int* T,
                                        int* lds_bytes, int* tile) {
     if (!b || job < 0 || job >= (int)b->jobs.size()) return RS_ERR_ARG;
-    if (tile) *tile = b->jobs[job].J.tw;
+    if (tile) *tile = b->jobs[job].J.tw | (b->jobs[job].J.list_idx ? RS_TILE_ROW_INDEX : 0);
     if (m) *m = b->jobs[job].J.m;
     if (T) *T = b->jobs[job].J.T;
     if (lds_bytes) *lds_bytes = b->jobs[job].J.lds_words * 4;

@@ -1,6 +1,7 @@
 // kernels.h -- job layout shared by the sieve kernels and their host side.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 // Record of one prime written by the set-up phase and read by the BSGS
@@ -47,6 +48,9 @@ struct JobDev {
     int off_gt;   // REC_BH + Q
     int off_j;    // off_gt + 2 T: the row masks
     int lds_words;  // 32-bit LDS words per wave (table, counts, Y, rows, record)
+    int lds_words_idx;  // the same with the row list as uint16 row numbers
+    int list_idx;   // 1: single-job launches list active rows by row number
+                    // (RS_TILE_ROW_INDEX; at most ROW_SLICE rows)
     int tab_words;  // of which the table: 4 NB, a power of two
     int ys;         // Y row stride in entries: T, or T + 1 when T is even, so
                     // that rows of different r start on different LDS banks
@@ -102,11 +106,17 @@ static inline int rec_words(int Q, int T, int nrows) {
 // and, from the base of the rest,
 //   NB                        bucket fill counts, one uint32 per bucket
 //   then 2 Q ys               Y entries (uint64), rows ys apart
-//   then 2 min(nrows, ROW_SLICE)  active rows of one slice (k, 8 r T | row << 19)
+//   then list_words           active rows of one slice: their entries (k,
+//                             8 r ys | row << 19), or with idx (single-job
+//                             launches, entries in block-shared LDS) their
+//                             uint16 row numbers
 //   then 2 W                  the prime's record (uint64)
-static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W) {
+__host__ __device__ inline int list_words(int nrows, bool idx) {
+    const int n = nrows < ROW_SLICE ? nrows : ROW_SLICE;  // one row slice
+    return idx ? (n + 3) / 4 * 2 : 2 * n;                 // even: the record is 8-aligned
+}
+static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W, bool idx) {
     const int NB = 1 << logNB;
-    const int list_rows = nrows < ROW_SLICE ? nrows : ROW_SLICE;  // one row slice
-    int w = 5 * NB + 2 * Q * ys + 2 * list_rows + 2 * W;
+    int w = 5 * NB + 2 * Q * ys + list_words(nrows, idx) + 2 * W;
     return (w + 3) & ~3;
 }

@@ -181,11 +181,14 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
                                             u32 tab_addr,
                                             const u32* __restrict__ cnt,
                                             const char* __restrict__ Yb,
-                                            const uint2* __restrict__ list, u64 vstart, u64 g64,
-                                            u64 p, u64 mp, HitDev* __restrict__ hits,
-                                            u32* __restrict__ hit_count, u32 hit_cap) {
+                                            const uint2* __restrict__ list,
+                                            const uint2* __restrict__ rows_lds, bool idx,
+                                            u64 vstart, u64 g64, u64 p, u64 mp,
+                                            HitDev* __restrict__ hits, u32* __restrict__ hit_count,
+                                            u32 hit_cap) {
     const int lane = threadIdx.x & 63;
     const u32 nbm = (1u << J.logNB) - 1;
+    const uint16_t* const list16 = (const uint16_t*)list;
     u32 bmask = nbm << 4;
     asm volatile("" : "+v"(bmask));  // in a vector register: one and-or per address
     const int T = J.T;
@@ -201,7 +204,9 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
     for (int base = 0; base < total; base += 64 * TW) {
         const bool ok = a < A;
         const u64 okm = lanes(ok);
-        const uint2 row = list[min(a, A - 1)];
+        // the row's entry: in the list, or (idx) behind its row number
+        const int ac = min(a, A - 1);
+        const uint2 row = idx ? rows_lds[list16[ac]] : list[ac];
         const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * t0);
         uint4 e[TW];
         u32 fp2[TW], ba[TW];
@@ -266,11 +271,12 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
 
 // Rows [r0, r1) that pass the Jacobi filter of this prime (row masks at
 // R + off_j, r0 a multiple of 64), compacted in order into `list` as
-// giant-step entries; returns their number.  The row table (k, kidx |
-// r << 16) is read from block-shared LDS at rows_lds (FROM_LDS) or from the
-// job's global arrays, by the passing lanes only.
+// giant-step entries from the job's global table or (FROM_LDS) from the
+// block-shared table rows_lds, which then holds the whole job (one slice,
+// r0 = 0); with idx (FROM_LDS only) as uint16 row numbers into rows_lds.
+// Returns their number.
 template <bool FROM_LDS>
-__device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_lds,
+__device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_lds, bool idx,
                                             const u64* __restrict__ R, uint2* __restrict__ list,
                                             int r0, int r1, int T) {
     const int lane = threadIdx.x & 63;
@@ -286,14 +292,20 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
         if ((mask >> lane) & 1) {
             const u32 below = __builtin_amdgcn_mbcnt_hi(hi32(mask),
                                                         __builtin_amdgcn_mbcnt_lo(lo32(mask), 0));
-            list[A + below] = entry();
+            entry(A + below);
         }
         A += __popcll(mask);
     };
     if (FROM_LDS) {
         // finished entries in block-shared LDS
-        for (int c = r0, w = 0; c < r1; c += 64, ++w)
-            put(w, [&] { return rows_lds[c + lane]; });
+        uint16_t* const list16 = (uint16_t*)list;
+        if (idx) {
+            for (int c = r0, w = 0; c < r1; c += 64, ++w)
+                put(w, [&](int at) { list16[at] = (uint16_t)(c + lane); });
+        } else {
+            for (int c = r0, w = 0; c < r1; c += 64, ++w)
+                put(w, [&](int at) { list[at] = rows_lds[c + lane]; });
+        }
     } else {
         // from the job's table in global memory: every load of the slice
         // issued before the first is used (ROW_SLICE = 8 x 64 rows)
@@ -305,7 +317,7 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
         }
 #pragma unroll
         for (int w = 0; w < ROW_SLICE / 64; ++w)
-            if (w < nw) put(w, [&] { return ent[w]; });
+            if (w < nw) put(w, [&](int at) { list[at] = ent[w]; });
     }
     return A;
 }
@@ -431,12 +443,14 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
 // BSGS of one prime by the whole wavefront.  `pre` holds record words
 // 0..63 (lane l has word l), the rest are read from Rg.  The table is at
 // tab, LDS byte address tab_addr, a multiple of its size; the rest of the
-// wavefront's LDS (counts, Y, row list, record) at wbase.
+// wavefront's LDS (counts, Y, row list, record) at wbase.  With idx (only
+// with rows_in_lds) the row list holds row numbers into rows_lds.
 __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
                                            const u64* __restrict__ Rg, uint4* __restrict__ tab,
                                            u32 tab_addr, u32* __restrict__ wbase,
                                            const uint2* __restrict__ rows_lds,
-                                           bool rows_in_lds, HitDev* __restrict__ hits,
+                                           bool rows_in_lds, bool idx,
+                                           HitDev* __restrict__ hits,
                                            u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
     const u32 NB = 1u << J.logNB;
@@ -445,7 +459,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     u32* const cnt = wbase;
     u64* const Y = (u64*)(cnt + NB);
     uint2* const list = (uint2*)(Y + Q * J.ys);
-    u64* const R = (u64*)(list + min(J.nrows, ROW_SLICE));
+    u64* const R = (u64*)((u32*)list + list_words(J.nrows, idx));
     const int L = m >> 6;
 
     if (lane < W) R[lane] = pre;
@@ -545,13 +559,13 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     // are compacted in order as (k, row << 8 | r), then their giant steps run
     for (int r0 = 0; r0 < J.nrows; r0 += ROW_SLICE) {
         const int r1 = min(J.nrows, r0 + ROW_SLICE);
-        const int A = rows_in_lds ? compact_rows<true>(J, rows_lds, R, list, r0, r1, T)
-                                  : compact_rows<false>(J, rows_lds, R, list, r0, r1, T);
+        const int A = rows_in_lds ? compact_rows<true>(J, rows_lds, idx, R, list, r0, r1, T)
+                                  : compact_rows<false>(J, rows_lds, false, R, list, r0, r1, T);
         wave_sync();
         const char* const Yb = (const char*)Y;
 #define RS_GIANT(TWV, CH, EX)                                                                \
-    giant_steps<TWV, CH, EX>(J, job, A, r0, tab_addr, cnt, Yb, list, vstart, g64, p, mp, hits, \
-                             hit_count, hit_cap)
+    giant_steps<TWV, CH, EX>(J, job, A, r0, tab_addr, cnt, Yb, list, rows_lds, idx,           \
+                             vstart, g64, p, mp, hits, hit_count, hit_cap)
 #define RS_GIANT_TW(CH, EX)      \
     if (J.tw >= 4)               \
         RS_GIANT(4, CH, EX);     \
@@ -575,14 +589,14 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     wave_sync();  // the next prime reuses the table
 }
 
-// At least 4 wavefronts per SIMD (128 VGPRs): the LDS layouts of typical
-// jobs allow 4, and fewer leave the VALU idle on LDS waits.
-extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
-                const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
-                int lds_words, int tab_words, int shared_words, int scratch_words,
-                uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
-                HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
+// The persistent kernel.  IDX: single-job launches whose wavefronts list
+// their active rows by row number (RS_TILE_ROW_INDEX), a smaller LDS layout.
+template <bool IDX>
+__device__ __forceinline__ void sieve_body(
+    ChunkDev c0, const JobDev* __restrict__ jobs, const ChunkDev* __restrict__ chunks,
+    int nchunks, long long ngroups, int lds_words, int tab_words, int shared_words,
+    int scratch_words, uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
+    HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
     extern __shared__ uint32_t lds[];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -654,10 +668,34 @@ rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
             const u64 cur = pre;
             if (i + 1 < n) pre = lane < J.W ? S[(i + 1) * J.W + lane] : 0;
             bsgs_prime(J, C.job, cur, S + i * J.W, tab, tab_addr, wbase, rows_lds, rows_in_lds,
-                       hits, hit_count, hit_cap);
+                       IDX, hits, hit_count, hit_cap);
         }
         wave_sync_global();  // records are rewritten by the next group
     }
+}
+
+// Full row entries: at least 4 wavefronts per SIMD (128 VGPRs), what the
+// LDS layouts of typical jobs allow; fewer leave the VALU idle on LDS waits.
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
+                const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
+                int lds_words, int tab_words, int shared_words, int scratch_words,
+                uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
+                HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
+    sieve_body<false>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
+                      scratch_words, scratch, work, hits, hit_count, hit_cap);
+}
+
+// Row numbers (single-job launches only): the smaller layout of typical
+// jobs fits 5 wavefronts per SIMD, so the code is held to 96 VGPRs.
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+rs_sieve_kernel_rix(ChunkDev c0, const JobDev* __restrict__ jobs,
+                    const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
+                    int lds_words, int tab_words, int shared_words, int scratch_words,
+                    uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
+                    HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
+    sieve_body<true>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
+                     scratch_words, scratch, work, hits, hit_count, hit_cap);
 }
 
 // ------------------------------------------------------------ test kernels

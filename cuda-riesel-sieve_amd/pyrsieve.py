@@ -111,6 +111,10 @@ _SIGS = {
 _lib = None
 
 
+
+# tile flag: active rows listed by row number (rsieve.h RS_TILE_ROW_INDEX)
+TILE_ROW_INDEX = 16
+
 class RsError(RuntimeError):
     def __init__(self, what, status):
         self.status = status

@@ -173,12 +173,18 @@ int rs_format_factor(const rs_factor* f, uint32_t b, int c, char* buf, int cap);
 /* ------------------------------------------------------------ device sieve */
 /* One sieve job on one GPU: base, Q, table size m (multiple of 64, or 0 for
  * the cost model's pick), rows and n window from a plan.  tile (1..4, 0 for
- * 2) is the number of giant steps of one row a lane takes per item.
+ * 2) is the number of giant steps of one row a lane takes per item,
+ * optionally | RS_TILE_ROW_INDEX: for a job of at most 512 rows, each
+ * wavefront lists its active rows as row numbers into the workgroup's
+ * shared row table instead of copies of the rows, a smaller LDS layout
+ * (more resident wavefronts) for one more LDS read per giant-step round;
+ * which is faster depends on the job (the autotuner times both).
  * Replaces the set-up at :1367-1512.  max_batch bounds the primes copied
  * from host memory per launch (the reference's arraySize, :851);
  * hit_capacity bounds the device append buffer (the reference keeps one
  * slot per prime, :615-621). */
 typedef struct rs_sieve rs_sieve;
+#define RS_TILE_ROW_INDEX 16
 
 rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int tile,
                           int64_t max_batch, int64_t hit_capacity,
@@ -219,8 +225,10 @@ void rs_sieve_destroy(rs_sieve* s);
  * p = low + offsets_dev[i] (device memory) and returns the fastest.
  * effort 0 times the full grid: multiples of 64 from 1/2 to 2 times the
  * cost model's m that fit one wavefront's LDS limit, each with tiles 1, 2
- * and 4.  effort 1 is a coordinate descent (tile at the model's m, then m
- * in steps of m/4 while the time improves), typically 5-8 timings.  cand_m,
+ * and 4, each with and without RS_TILE_ROW_INDEX (plans of at most 512
+ * rows).  effort 1 is a coordinate descent (tile at the model's m, then m
+ * in steps of m/4 while the time improves, then the row-index layout at
+ * the best pair), typically 6-9 timings.  cand_m,
  * cand_tile, ns_per_prime (optional, capacity cap) receive the measured
  * candidates, *ncand their number. */
 rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,

@@ -334,6 +334,38 @@ def test_p_around_2_50_both_baby_chains(rs, orc, gpu):
     assert all(rs.verify_factor(p, k, b, n) for p, k, n in got)
 
 
+def test_row_index_layout_all_tiles(rs, orc, gpu):
+    """Active rows listed by row number into the shared row table
+    (TILE_ROW_INDEX): every tile with and without it equals the oracle on a
+    window, and the exhaustive search on tiny primes."""
+    plan = rs.Plan(rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2))
+    assert plan.nrows <= 512
+    low = 10**9 + 1
+    offs = rs.PrimeGen(low + 2**20).range(low, low + 2**20)
+    want = run_oracle(orc, plan, 512, low, offs)
+    assert len(want) > 0
+    for tile in (1, 2, 4):
+        for flag in (0, rs.TILE_ROW_INDEX):
+            sv = rs.Sieve(plan, m=512, tile=tile | flag)
+            assert sv.tile == tile | flag
+            sv.submit(low, offs)
+            assert sv.collect() == want, (tile, flag)
+            sv.close()
+    tiny = tiny_job(rs, 6, {5: list(range(100, 700, 3)), 11: list(range(101, 650, 2)),
+                            12: [100, 150, 200, 640], 35: list(range(300, 700))})
+    primes = [q for q in range(5, 30000, 2) if is_prime(q)]
+    toffs = np.array([q - 3 for q in primes], np.uint32)
+    rk, rr, _ = tiny.rows()
+    job = orc.Job(tiny.base, tiny.Q, 128, tiny.minN, tiny.maxN, rk, rr, tiny.c)
+    brute = []
+    for q in primes:
+        brute += orc.brute_prime(job, q)
+    sv = rs.Sieve(tiny, m=128, tile=4 | rs.TILE_ROW_INDEX)
+    sv.submit(3, toffs)
+    assert sv.collect() == brute
+    sv.close()
+
+
 def test_many_rows_sliced_all_tiles(rs, orc, gpu):
     """200 k of base 1030 give ~1000+ active (k, r) rows, more than one LDS
     row slice (512): every slice and every giant-step tile width must agree
