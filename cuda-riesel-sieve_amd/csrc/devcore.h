@@ -13,9 +13,11 @@
     ChunkDev c0, const JobDev *jobs, const ChunkDev *chunks, int nchunks, long long ngroups, \
         int lds_words, int tab_words, int shared_words, int scratch_words, uint64_t *scratch, \
         unsigned long long *work, HitDev *hits, uint32_t *hit_count, uint32_t hit_cap
-// full row entries / row numbers (RS_TILE_ROW_INDEX, single-job launches)
+// 4 and 5 wavefronts per SIMD with full row entries, and 5 with row-number
+// lists (RS_TILE_ROW_INDEX, single-job launches)
 extern "C" __global__ void rs_sieve_kernel(RS_SIEVE_KERNEL_ARGS);
-extern "C" __global__ void rs_sieve_kernel_rix(RS_SIEVE_KERNEL_ARGS);
+extern "C" __global__ void rs_sieve_kernel5(RS_SIEVE_KERNEL_ARGS);
+extern "C" __global__ void rs_sieve_kernel5_rix(RS_SIEVE_KERNEL_ARGS);
 extern "C" __global__ void rs_zero_u32(uint32_t*, int);
 extern "C" __global__ void rs_jacobi_table(uint32_t n, uint32_t words, uint32_t* out);
 
@@ -46,7 +48,8 @@ struct Launcher {
     int tab_words = 0;       // of which the table slot (a power of two, at least
                              // the table of every job launched)
     int shared_words = 0;    // block-shared words after the wavefront regions
-    bool rix = false;        // rs_sieve_kernel_rix (row-number lists)
+    bool w5 = false;         // rs_sieve_kernel5 (row_index: rs_sieve_kernel5_rix)
+    bool row_index = false;
     size_t lds_bytes = 0;
     int scratch_words = 0;  // per wavefront: 64 records
     uint64_t* scratch = nullptr;
@@ -89,11 +92,14 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj);
 size_t job_device_bytes(const rs_plan* pl, bool dat);
 void free_launcher(Launcher& L);
 // shared_words: block-shared LDS (the row table of a single-job launch).
+// row_index: the layout lists rows by number (needs rs_sieve_kernel5).
 rs_status init_launcher(Launcher* L, int device, int lds_words, int tab_words, int W,
-                        int64_t hit_capacity, int shared_words = 0, bool rix = false);
-// Grid shape for a per-wave LDS layout (no allocation).
+                        int64_t hit_capacity, int shared_words = 0, bool row_index = false);
+// Grid shape for a per-wave LDS layout (no allocation): of the two kernels
+// the one keeping more wavefronts resident (rs_sieve_kernel on a tie), or
+// rs_sieve_kernel5 when row_index.
 rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words = 0,
-                          bool rix = false);
+                          bool row_index = false);
 rs_status init_multi(MultiLauncher* M, int device, int w_cap, int64_t hit_capacity);
 // Orders `chunks` (each with .job indexing `jobs`) by occupancy class,
 // numbering groups from 0 within each class, and returns one range per

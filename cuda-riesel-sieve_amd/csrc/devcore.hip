@@ -233,12 +233,14 @@ void free_launcher(Launcher& L) {
 
 // Sizes the grid for lds_words per wave and record size W (both maxima over
 // the jobs of the launch) and allocates scratch, counter and hits.
-static const void* kernel_of(bool rix) {
-    return rix ? reinterpret_cast<const void*>(rs_sieve_kernel_rix)
-               : reinterpret_cast<const void*>(rs_sieve_kernel);
+static const void* kernel_of(bool w5, bool row_index) {
+    return row_index ? reinterpret_cast<const void*>(rs_sieve_kernel5_rix)
+           : w5      ? reinterpret_cast<const void*>(rs_sieve_kernel5)
+                     : reinterpret_cast<const void*>(rs_sieve_kernel);
 }
 
-rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words, bool rix) {
+rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words,
+                          bool row_index) {
     L->device = device;
     int max_lds = 0, cus = 0;
     HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -248,31 +250,40 @@ rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_wor
         return RS_ERR_ARG;
     // waves of a block are independent (one table each); 1 to 4 per block,
     // whichever keeps the most waves resident per CU (ties: more per block,
-    // so the block-shared part is loaded fewer times)
-    // the kernel addresses its tables from LDS address 0: it must have no
-    // static LDS before the dynamic region
-    hipFuncAttributes fa{};
-    HIPCHK(hipFuncGetAttributes(&fa, kernel_of(rix)));
-    if (fa.sharedSizeBytes != 0) return RS_ERR_HIP;
-    // the attribute is per function: always the device limit, so launches
-    // of different LDS sizes never restrict one another
-    HIPCHK(hipFuncSetAttribute(kernel_of(rix),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-    int best_w = 0, best_wpb = 1, best_cu = 1;
-    for (int wpb = 1; wpb <= kMaxWavesPerBlock; ++wpb) {
-        const size_t bytes = per_wave * wpb + shared;
-        if (bytes > (size_t)max_lds) break;
-        int per_cu = 0;
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, kernel_of(rix), 64 * wpb, bytes));
-        if (per_cu * wpb >= best_w) {
-            best_w = per_cu * wpb;
-            best_wpb = wpb;
-            best_cu = per_cu;
+    // so the block-shared part is loaded fewer times).  rs_sieve_kernel5 only
+    // when it keeps a whole wavefront per SIMD more resident (4 per CU):
+    // its tighter register budget costs more than 1-3 extra wavefronts gain
+    // (measured on sr_745.abcd: 18 against 16 resident was 7% slower)
+    int w_of[2] = {0, 0}, wpb_of[2] = {1, 1}, cu_of[2] = {1, 1};
+    for (bool w5 : {false, true}) {
+        if (row_index && !w5) continue;
+        // the kernel addresses its tables from LDS address 0: it must have
+        // no static LDS before the dynamic region
+        hipFuncAttributes fa{};
+        HIPCHK(hipFuncGetAttributes(&fa, kernel_of(w5, row_index)));
+        if (fa.sharedSizeBytes != 0) return RS_ERR_HIP;
+        // the attribute is per function: always the device limit, so
+        // launches of different LDS sizes never restrict one another
+        HIPCHK(hipFuncSetAttribute(kernel_of(w5, row_index),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+        for (int wpb = 1; wpb <= kMaxWavesPerBlock; ++wpb) {
+            const size_t bytes = per_wave * wpb + shared;
+            if (bytes > (size_t)max_lds) break;
+            int per_cu = 0;
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, kernel_of(w5, row_index), 64 * wpb, bytes));
+            if (per_cu * wpb >= w_of[w5]) {
+                w_of[w5] = per_cu * wpb;
+                wpb_of[w5] = wpb;
+                cu_of[w5] = per_cu;
+            }
         }
     }
+    const bool best_w5 = row_index || w_of[1] >= w_of[0] + 4;
+    const int best_wpb = wpb_of[best_w5], best_cu = cu_of[best_w5];
     L->wpb = best_wpb;
-    L->rix = rix;
+    L->w5 = best_w5;
+    L->row_index = row_index;
     L->lds_words = lds_words;
     L->shared_words = shared_words;
     L->lds_bytes = per_wave * L->wpb + shared;
@@ -292,8 +303,8 @@ static rs_status alloc_buffers(Launcher* L, size_t scratch_words_total, int64_t 
 }
 
 rs_status init_launcher(Launcher* L, int device, int lds_words, int tab_words, int W,
-                        int64_t hit_capacity, int shared_words, bool rix) {
-    rs_status st = launch_geometry(device, lds_words, L, shared_words, rix);
+                        int64_t hit_capacity, int shared_words, bool row_index) {
+    rs_status st = launch_geometry(device, lds_words, L, shared_words, row_index);
     if (st != RS_OK) return st;
     L->tab_words = tab_words;
     L->scratch_words = 64 * W;
@@ -305,9 +316,9 @@ rs_status init_multi(MultiLauncher* M, int device, int w_cap, int64_t hit_capaci
     // the smallest layout gives the most resident waves (of either kernel):
     // size scratch for it
     M->max_waves = 0;
-    for (bool rix : {false, true}) {
+    for (bool row_index : {false, true}) {
         Launcher g;
-        rs_status st = launch_geometry(device, 64, &g, 0, rix);
+        rs_status st = launch_geometry(device, 64, &g, 0, row_index);
         if (st != RS_OK) return st;
         M->max_waves = std::max(M->max_waves, (size_t)g.grid * g.wpb);
     }
@@ -459,7 +470,10 @@ rs_status launch(const Launcher& L, hipStream_t stream, const JobDev* jobs, cons
     hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)L.work, 2);
     HIPCHK(hipGetLastError());
     const int grid = (int)std::min<long long>(L.grid, (ngroups + L.wpb - 1) / L.wpb);
-    hipLaunchKernelGGL(L.rix ? rs_sieve_kernel_rix : rs_sieve_kernel, dim3(std::max(1, grid)),
+    hipLaunchKernelGGL(L.row_index ? rs_sieve_kernel5_rix
+                       : L.w5      ? rs_sieve_kernel5
+                                   : rs_sieve_kernel,
+                       dim3(std::max(1, grid)),
                        dim3(64 * L.wpb), L.lds_bytes, stream, c0, jobs, chunks, nchunks, ngroups,
                        L.lds_words, L.tab_words, L.shared_words,
                        L.scratch_words, L.scratch, L.work, L.hits, L.hit_count, L.hit_cap);

@@ -590,7 +590,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
 }
 
 // The persistent kernel.  IDX: single-job launches whose wavefronts list
-// their active rows by row number (RS_TILE_ROW_INDEX), a smaller LDS layout.
+// active rows by row number (RS_TILE_ROW_INDEX), a smaller LDS layout.
 template <bool IDX>
 __device__ __forceinline__ void sieve_body(
     ChunkDev c0, const JobDev* __restrict__ jobs, const ChunkDev* __restrict__ chunks,
@@ -674,8 +674,10 @@ __device__ __forceinline__ void sieve_body(
     }
 }
 
-// Full row entries: at least 4 wavefronts per SIMD (128 VGPRs), what the
-// LDS layouts of typical jobs allow; fewer leave the VALU idle on LDS waits.
+// At least 4 wavefronts per SIMD (128 VGPRs): full row entries; the LDS
+// layouts of typical jobs allow 4, and fewer leave the VALU idle on LDS
+// waits.  The host picks this or rs_sieve_kernel5, whichever keeps more
+// wavefronts resident for the launch's layout.
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
                 const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
@@ -686,14 +688,26 @@ rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
                       scratch_words, scratch, work, hits, hit_count, hit_cap);
 }
 
-// Row numbers (single-job launches only): the smaller layout of typical
-// jobs fits 5 wavefronts per SIMD, so the code is held to 96 VGPRs.
+// At least 5 wavefronts per SIMD (96 VGPRs), for layouts small enough to
+// keep 5 resident: full row entries, and row-number lists (single-job
+// launches with RS_TILE_ROW_INDEX only).
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
-rs_sieve_kernel_rix(ChunkDev c0, const JobDev* __restrict__ jobs,
-                    const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
-                    int lds_words, int tab_words, int shared_words, int scratch_words,
-                    uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
-                    HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
+rs_sieve_kernel5(ChunkDev c0, const JobDev* __restrict__ jobs,
+                 const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
+                 int lds_words, int tab_words, int shared_words, int scratch_words,
+                 uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
+                 HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
+    sieve_body<false>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
+                      scratch_words, scratch, work, hits, hit_count, hit_cap);
+}
+
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+rs_sieve_kernel5_rix(ChunkDev c0, const JobDev* __restrict__ jobs,
+                     const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
+                     int lds_words, int tab_words, int shared_words, int scratch_words,
+                     uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
+                     HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count,
+                     uint32_t hit_cap) {
     sieve_body<true>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
                      scratch_words, scratch, work, hits, hit_count, hit_cap);
 }
