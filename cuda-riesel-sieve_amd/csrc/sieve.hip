@@ -445,6 +445,7 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
 // tab, LDS byte address tab_addr, a multiple of its size; the rest of the
 // wavefront's LDS (counts, Y, row list, record) at wbase.  With idx (only
 // with rows_in_lds) the row list holds row numbers into rows_lds.
+template <int BB>
 __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
                                            const u64* __restrict__ Rg, uint4* __restrict__ tab,
                                            u32 tab_addr, u32* __restrict__ wbase,
@@ -496,15 +497,11 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     }
     u64 v = vstart;
     bool ovf = false;  // some insert went past a full bucket
-    auto insert = [&](u64 val) {
-        const u32 lo = lo32(val);
-        u32 b = bucket_of(lo, (u32)J.logNB);
-        // the bucket's LDS byte address, kept as one value (the empty asm
-        // stops the slot offset being folded back into b, one more add)
-        u32 ta = tab_addr + 16 * b;
-        asm volatile("" : "+v"(ta));
-        u32 c = atomicAdd(&cnt[b], 1u);
-        if (c >= 8) {  // rare: walk to the first bucket with a free slot
+    // the first free slot at or after bucket b, given the slot count c
+    // that b's fill counter returned: rare, walk to the first bucket with a
+    // free slot; returns the slot's LDS byte address
+    auto place = [&](u32 b, u32 ta, u32 c) {
+        if (c >= 8) {
             ovf = true;
             do {
                 b = (b + 1) & nbm;
@@ -512,25 +509,69 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
             } while (c >= 8);
             ta = tab_addr + 16 * b;
         }
-        *(__attribute__((address_space(3))) unsigned short*)(size_t)(ta + 2 * c) = fingerprint(lo);
+        return ta + 2 * c;
+    };
+    auto store_fp = [](u32 at, u32 lo) {
+        *(__attribute__((address_space(3))) unsigned short*)(size_t)at = fingerprint(lo);
+    };
+    auto insert = [&](u32 lo) {
+        const u32 b = bucket_of(lo, (u32)J.logNB);
+        store_fp(place(b, tab_addr + 16 * b, atomicAdd(&cnt[b], 1u)), lo);
     };
     // giant steps leave products in [0, 2p): an unreduced one is s + p with
-    // s < p*kmax/2^32 <= bound, so such baby values go in twice
+    // s < p*kmax/2^32 <= bound, so such baby values go in twice (the second
+    // entry's low word is lo(value) + lo(p))
     const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;
+    const u32 plo = lo32(p);
+    // steps in blocks of BB: the block's values, then its fill-counter
+    // atomics (all in flight together), then its fingerprints; step(lo, dup,
+    // more) yields the next value's low word and whether it is below the
+    // bound, and advances the chain if more follow
+    auto baby = [&](auto step) {
+        int s0 = 0;
+        for (; s0 + BB <= L; s0 += BB) {
+            u32 lo[BB], ta[BB], c[BB], bk[BB];
+            bool dup[BB];
+#pragma unroll
+            for (int u = 0; u < BB; ++u) step(lo[u], dup[u], s0 + u + 1 < L);
+#pragma unroll
+            for (int u = 0; u < BB; ++u) {
+                bk[u] = bucket_of(lo[u], (u32)J.logNB);
+                // the bucket's LDS byte address, kept as one value (the empty
+                // asm stops the slot offset being folded back into the bucket
+                // index, one more add)
+                ta[u] = tab_addr + 16 * bk[u];
+                asm volatile("" : "+v"(ta[u]));
+                c[u] = atomicAdd(&cnt[bk[u]], 1u);
+            }
+#pragma unroll
+            for (int u = 0; u < BB; ++u) store_fp(place(bk[u], ta[u], c[u]), lo[u]);
+#pragma unroll
+            for (int u = 0; u < BB; ++u)
+                if (dup[u]) insert(lo[u] + plo);
+        }
+        for (; s0 < L; ++s0) {
+            u32 lo;
+            bool dup;
+            step(lo, dup, s0 + 1 < L);
+            insert(lo);
+            if (dup) insert(lo + plo);
+        }
+    };
     if (fpm) {
         const double wd = rec_d(REC_GW), bd = rec_d(REC_GWS);  // g^64 and the bound
-        for (int s = 0; s < L; ++s) {
+        baby([&](u32& lo, bool& dup, bool more) {
             const double c = vd < 0.0 ? vd + pd : vd;
-            insert(fp_bits(c));
-            if (c < bd) insert(fp_bits(c + pd));
-            if (s + 1 < L) vd = fp_mulmod(vd, wd, pd, pinv);
-        }
+            lo = lo32(fp_bits(c));
+            dup = c < bd;
+            if (more) vd = fp_mulmod(vd, wd, pd, pinv);
+        });
     } else {
-        for (int s = 0; s < L; ++s) {
-            insert(v);
-            if (v < bound) insert(v + p);
-            if (s + 1 < L) v = shoup_mul(v, gw, gws, p);
-        }
+        baby([&](u32& lo, bool& dup, bool more) {
+            lo = lo32(v);
+            dup = v < bound;
+            if (more) v = shoup_mul(v, gw, gws, p);
+        });
     }
     const bool chain = __ballot(ovf) != 0;
 
@@ -590,8 +631,9 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
 }
 
 // The persistent kernel.  IDX: single-job launches whose wavefronts list
-// active rows by row number (RS_TILE_ROW_INDEX), a smaller LDS layout.
-template <bool IDX>
+// active rows by row number (RS_TILE_ROW_INDEX), a smaller LDS layout.  BB:
+// baby steps per block of fill-counter atomics in flight.
+template <bool IDX, int BB>
 __device__ __forceinline__ void sieve_body(
     ChunkDev c0, const JobDev* __restrict__ jobs, const ChunkDev* __restrict__ chunks,
     int nchunks, long long ngroups, int lds_words, int tab_words, int shared_words,
@@ -667,12 +709,21 @@ __device__ __forceinline__ void sieve_body(
         for (int i = 0; i < n; ++i) {
             const u64 cur = pre;
             if (i + 1 < n) pre = lane < J.W ? S[(i + 1) * J.W + lane] : 0;
-            bsgs_prime(J, C.job, cur, S + i * J.W, tab, tab_addr, wbase, rows_lds, rows_in_lds,
+            bsgs_prime<BB>(J, C.job, cur, S + i * J.W, tab, tab_addr, wbase, rows_lds, rows_in_lds,
                        IDX, hits, hit_count, hit_cap);
         }
         wave_sync_global();  // records are rewritten by the next group
     }
 }
+
+#ifndef RS_BB4
+#define RS_BB4 4
+#endif
+#ifndef RS_BB5
+#define RS_BB5 2
+#endif
+// baby-step blocks of the 128- and the 96-VGPR builds
+constexpr int BB4 = RS_BB4, BB5 = RS_BB5;
 
 // At least 4 wavefronts per SIMD (128 VGPRs): full row entries; the LDS
 // layouts of typical jobs allow 4, and fewer leave the VALU idle on LDS
@@ -684,9 +735,10 @@ rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
                 int lds_words, int tab_words, int shared_words, int scratch_words,
                 uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
                 HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
-    sieve_body<false>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
+    sieve_body<false, BB4>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
                       scratch_words, scratch, work, hits, hit_count, hit_cap);
 }
+
 
 // At least 5 wavefronts per SIMD (96 VGPRs), for layouts small enough to
 // keep 5 resident: full row entries, and row-number lists (single-job
@@ -697,7 +749,7 @@ rs_sieve_kernel5(ChunkDev c0, const JobDev* __restrict__ jobs,
                  int lds_words, int tab_words, int shared_words, int scratch_words,
                  uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
                  HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
-    sieve_body<false>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
+    sieve_body<false, BB5>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
                       scratch_words, scratch, work, hits, hit_count, hit_cap);
 }
 
@@ -708,7 +760,7 @@ rs_sieve_kernel5_rix(ChunkDev c0, const JobDev* __restrict__ jobs,
                      uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
                      HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count,
                      uint32_t hit_cap) {
-    sieve_body<true>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
+    sieve_body<true, BB5>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
                      scratch_words, scratch, work, hits, hit_count, hit_cap);
 }
 
