@@ -5,19 +5,22 @@
 #include <stdint.h>
 
 // Record of one prime written by the set-up phase and read by the BSGS
-// phase (64-bit words).  Residues are Montgomery forms (R = 2^64) unless
-// marked plain.  For p < 2^50 ("fp": the BSGS phase multiplies in double
-// precision) some words hold doubles instead, as noted.
-//   [0] p   [1] mp = -p^-1 mod 2^64   [2] fp: 1/p, else unused   [3] g^64
-//   [4] g^64 mod p (plain; fp: as a double)
+// phase (64-bit words).  For p >= 2^50 residues are Montgomery forms
+// (R = 2^64) unless marked plain.  For p < 2^50 ("fp": the BSGS phase
+// multiplies in double precision) every residue but [3] is plain, held as
+// a double (an exact integer in (-p, p)).
+//   [0] p   [1] mp = -p^-1 mod 2^64   [2] fp: 1/p, else unused
+//   [3] g^64 (Montgomery form, an integer in [0, p), in both cases)
+//   [4] g^64 mod p (plain)
 //   [5] its Shoup constant floor(g^64 2^64 / p) (fp: the double of the
 //       second-entry bound (p >> 32) kmax + kmax + 1)
-//   [6 .. 14)   g^c for c = 0..7          (g = b^Q; fp: doubles)
-//   [14 .. 22)  g^(8a) for a = 0..7       (fp: plain, doubles)
-//   [22]        fp: p as a double
-//   [23 .. 23+Q)        BH[r] = b^r * H * 2^32, H = g^(x0 + m - 1) (fp: doubles)
-//   [23+Q .. 23+Q+2T)   G^t mod p (plain; fp: doubles) and its Shoup
-//                       constant, t < T, G = g^m, interleaved
+//   [6 .. 14)   g^c for c = 0..7          (g = b^Q)
+//   [14 .. 22)  g^(8a) for a = 0..7       (plain)
+//   [22]        p as a double
+//   [23 .. 23+Q)        BH[r] = b^r * H * 2^32, H = g^(x0 + m - 1), negated
+//                       for k*b^n+1
+//   [23+Q .. 23+Q+2T)   G^t mod p (plain) and its Shoup constant (fp:
+//                       unused), t < T, G = g^m, interleaved
 //   [23+Q+2T .. +NW)    row masks: bit i of word w is set iff row 64w+i passes
 //                       the Jacobi filter, (k/p) = +1 for even r and
 //                       (k*b/p) = +1 for odd r (NW = ceil(nrows / 64))

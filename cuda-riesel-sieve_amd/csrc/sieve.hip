@@ -108,7 +108,8 @@ __device__ __forceinline__ bool bucket_match(uint4 e, u32 fp2) {
 // Emits every baby step j with g^j == target (all 64 lanes take part:
 // lane l checks j = l + 64 v) as a hit of giant step tt of the row whose
 // entry is (k, yent) in the slice starting at row r0.  With n-list bitmaps
-// only n in the row's list is emitted.
+// only n in the row's list is emitted.  g64 is the Montgomery form of g^64,
+// so the chain keeps vstart's form: plain for p < 2^50, else Montgomery.
 __device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32 k, u32 yent,
                                        int r0, int tt, u64 vstart, u64 g64, u64 p, u64 mp,
                                        HitDev* __restrict__ hits, u32* __restrict__ hit_count,
@@ -322,9 +323,10 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
     return A;
 }
 
-// Set-up of one prime (one lane): writes the record described in kernels.h.
-__device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restrict__ R) {
-    const u64 mp = mont_mprime(p);
+// Powers of the record for p >= 2^50, as Montgomery forms (exact 64-bit
+// products).  Returns whether p divides b.
+__device__ __forceinline__ bool prep_powers_int(const JobDev& J, u64 p, u64 mp,
+                                                u64* __restrict__ R) {
     const u64 one = (0ull - p) % p;                  // 2^64 mod p
     u64 r2 = one << 1;                               // Mont(2)
     if (r2 >= p) r2 -= p;
@@ -334,16 +336,12 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     const u64 g = mont_pow(B, (u32)J.Q, one, p, mp);
     // baby-step start powers: lane l of the BSGS wave starts at
     // g^(l & 7) * g^(8 (l >> 3))
-    // below 2^50 the BSGS phase multiplies in double precision: its
-    // constants are stored as doubles (exact integers below 2^50)
-    const bool fpy = p < kFpMaxP;
-    auto dbits = [](u64 v) { return (u64)__double_as_longlong((double)v); };
     u64 w, ws;
     shoup_pair(g, p, mp, &w, &ws);
     u64 x = one;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        R[REC_GP + c] = fpy ? dbits(x) : x;
+        R[REC_GP + c] = x;
         x = shoup_mul(x, w, ws, p);
     }
     const u64 g8 = x;
@@ -351,16 +349,16 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     x = one;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        R[REC_G8P + a] = fpy ? dbits(redc(x, p, mp)) : x;  // fp: plain
+        R[REC_G8P + a] = x;
         x = shoup_mul(x, w, ws, p);
     }
     const u64 g64 = x;
     shoup_pair(g64, p, mp, &w, &ws);
-    const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;  // second baby entries
-    R[REC_GW] = fpy ? dbits(w) : w;
-    R[REC_GWS] = fpy ? dbits(bound) : ws;
-    R[REC_PD] = dbits(p);
-    R[REC_PINV] = (u64)__double_as_longlong(fpy ? 1.0 / (double)p : 0.0);
+    R[REC_GW] = w;
+    R[REC_GWS] = ws;
+    R[REC_PD] = (u64)__double_as_longlong((double)p);
+    R[REC_PINV] = 0;
+    R[REC_G64] = g64;
     const u64 G = mont_pow(g64, (u32)(J.m >> 6), one, p, mp);
     const u64 H = mont_pow(g, (u32)(J.x0 + J.m - 1), one, p, mp);
     const u64 c32 = montmul(1ull << 32, r2, p, mp);  // Mont(2^32)
@@ -369,7 +367,7 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     if (J.plus && bh != 0) bh = p - bh;
     shoup_pair(B, p, mp, &w, &ws);
     for (int r = 0; r < J.Q; ++r) {
-        R[REC_BH + r] = fpy ? dbits(bh) : bh;
+        R[REC_BH + r] = bh;
         bh = shoup_mul(bh, w, ws, p);
     }
     shoup_pair(G, p, mp, &w, &ws);
@@ -377,13 +375,83 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     for (int t = 0; t < J.T; ++t) {
         u64 gp, gs;
         shoup_pair(gt, p, mp, &gp, &gs);
-        R[J.off_gt + 2 * t] = fpy ? dbits(gp) : gp;
+        R[J.off_gt + 2 * t] = gp;
         R[J.off_gt + 2 * t + 1] = gs;
         gt = shoup_mul(gt, w, ws, p);
     }
+    return B == 0;
+}
+
+// x^e mod p < 2^50 in double precision (fp_mulmod), for |x| < p and a
+// wave-uniform e; the result lies in (-p, p).
+__device__ __forceinline__ double fp_pow(double x, u32 e, double pd, double pinv) {
+    double r = 1.0;
+    for (int i = 31 - __builtin_clz(e | 1); i >= 0; --i) {
+        r = fp_mulmod(r, r, pd, pinv);
+        if ((e >> i) & 1) r = fp_mulmod(r, x, pd, pinv);
+    }
+    return r;
+}
+
+// Powers of the record for p < 2^50, as plain residues held as doubles in
+// (-p, p) (the BSGS phase multiplies them with fp_mulmod), but for the
+// rescan's g^64.  Returns whether p divides b.
+__device__ __forceinline__ bool prep_powers_fp(const JobDev& J, u64 p, u64* __restrict__ R) {
+    const double pd = (double)p, pinv = 1.0 / pd;
+    auto put = [&](int w, double v) { R[w] = (u64)__double_as_longlong(v); };
+    // b mod p (b < 2^32; a reduction only for p <= b) and 2^32 mod p
+    const u32 b = J.base;
+    const double bd = p > b ? (double)b : fp_mulmod((double)b, 1.0, pd, pinv);
+    const double c32 = fp_mulmod(4294967296.0, 1.0, pd, pinv);
+    const double g = fp_pow(bd, (u32)J.Q, pd, pinv);
+    // baby-step start powers, as in prep_powers_int
+    double x = 1.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        put(REC_GP + c, x);
+        x = fp_mulmod(x, g, pd, pinv);
+    }
+    const double g8 = x;
+    x = 1.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        put(REC_G8P + a, x);
+        x = fp_mulmod(x, g8, pd, pinv);
+    }
+    const double g64 = x;
+    put(REC_GW, g64);
+    // the rescan's multiplier: the Montgomery form g^64 2^64 mod p, an
+    // integer, keeps its chain of plain residues plain
+    const double g64m = fp_mulmod(g64, fp_mulmod(c32, c32, pd, pinv), pd, pinv);
+    R[REC_G64] = fp_bits(g64m < 0.0 ? g64m + pd : g64m) & ((1ull << 52) - 1);
+    put(REC_GWS, (double)((p >> 32) * J.kmax + J.kmax + 1));  // second-entry bound
+    put(REC_PD, pd);
+    put(REC_PINV, pinv);
+    const double G = fp_pow(g64, (u32)(J.m >> 6), pd, pinv);
+    const double H = fp_pow(g, (u32)(J.x0 + J.m - 1), pd, pinv);
+    // k*b^n+1: negate the giant-step starts (-0.0 is harmless: fp_mulmod
+    // products of it are zeros)
+    double bh = fp_mulmod(H, c32, pd, pinv);
+    if (J.plus) bh = -bh;
+    for (int r = 0; r < J.Q; ++r) {
+        put(REC_BH + r, bh);
+        bh = fp_mulmod(bh, bd, pd, pinv);
+    }
+    double gt = 1.0;
+    for (int t = 0; t < J.T; ++t) {
+        put(J.off_gt + 2 * t, gt);
+        gt = fp_mulmod(gt, G, pd, pinv);
+    }
+    return bd == 0.0;
+}
+
+// Set-up of one prime (one lane): writes the record described in kernels.h.
+__device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restrict__ R) {
+    const u64 mp = mont_mprime(p);
+    const bool bzero = p < kFpMaxP ? prep_powers_fp(J, p, R) : prep_powers_int(J, p, mp, R);
     // Jacobi bits; a prime dividing b gets none, so no row is searched
-    const int jb = B == 0 ? 0
-                          : jacobi_pre(J.bjac.x, J.bjac.y,
+    const int jb = bzero ? 0
+                         : jacobi_pre(J.bjac.x, J.bjac.y,
                                        ((u64)J.bjac.w << 32) | J.bjac.z, p);
     // row masks: k (in order) adds its even-r rows if (k/p) = w and its
     // odd-r rows if (kb/p) = w to words w0, w0 + 1, where w = +1, or (-1/p)
@@ -437,7 +505,6 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     if (cur + 1 < nw) RM[cur + 1] = hi;
     R[REC_P] = p;
     R[REC_MP] = mp;
-    R[REC_G64] = g64;
 }
 
 // BSGS of one prime by the whole wavefront.  `pre` holds record words
