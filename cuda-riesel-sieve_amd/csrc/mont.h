@@ -116,11 +116,12 @@ __device__ __forceinline__ u64 shoup32_entry(u64 z, u64 p, u32 mp0) {
 // one multiply-add whose low word is the result.  Written in C, the
 // compiler turns the last step into a multiply and a subtract because only
 // the low word is used; the one-instruction asm keeps the multiply-add.)
+// negp0 must be wave-uniform: it is an SGPR operand of the asm.
 __device__ __forceinline__ u32 shoup32_lo(u32 k, u32 ylo, u32 yq, u32 negp0) {
     const u64 ky = mad32(k, ylo, 0);
     const u32 q = __umulhi(k, yq);
     u64 r, carry;
-    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(q), "v"(negp0), "v"(ky));
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(q), "s"(negp0), "v"(ky));
     return lo32(r);
 }
 

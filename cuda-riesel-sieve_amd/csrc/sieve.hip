@@ -591,16 +591,17 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;
     const u32 plo = lo32(p);
     // steps in blocks of BB: the block's values, then its fill-counter
-    // atomics (all in flight together), then its fingerprints; step(lo, dup,
-    // more) yields the next value's low word and whether it is below the
-    // bound, and advances the chain if more follow
+    // atomics (all in flight together), then its fingerprints; step(lo, dup)
+    // yields the next value's low word and whether it is below the bound,
+    // and advances the chain (one product past the last step, cheaper than
+    // selecting it away)
     auto baby = [&](auto step) {
         int s0 = 0;
         for (; s0 + BB <= L; s0 += BB) {
             u32 lo[BB], ta[BB], c[BB], bk[BB];
             bool dup[BB];
 #pragma unroll
-            for (int u = 0; u < BB; ++u) step(lo[u], dup[u], s0 + u + 1 < L);
+            for (int u = 0; u < BB; ++u) step(lo[u], dup[u]);
 #pragma unroll
             for (int u = 0; u < BB; ++u) {
                 bk[u] = bucket_of(lo[u], (u32)J.logNB);
@@ -620,24 +621,24 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
         for (; s0 < L; ++s0) {
             u32 lo;
             bool dup;
-            step(lo, dup, s0 + 1 < L);
+            step(lo, dup);
             insert(lo);
             if (dup) insert(lo + plo);
         }
     };
     if (fpm) {
         const double wd = rec_d(REC_GW), bd = rec_d(REC_GWS);  // g^64 and the bound
-        baby([&](u32& lo, bool& dup, bool more) {
+        baby([&](u32& lo, bool& dup) {
             const double c = vd < 0.0 ? vd + pd : vd;
             lo = lo32(fp_bits(c));
             dup = c < bd;
-            if (more) vd = fp_mulmod(vd, wd, pd, pinv);
+            vd = fp_mulmod(vd, wd, pd, pinv);
         });
     } else {
-        baby([&](u32& lo, bool& dup, bool more) {
+        baby([&](u32& lo, bool& dup) {
             lo = lo32(v);
             dup = v < bound;
-            if (more) v = shoup_mul(v, gw, gws, p);
+            v = shoup_mul(v, gw, gws, p);
         });
     }
     const bool chain = __ballot(ovf) != 0;
