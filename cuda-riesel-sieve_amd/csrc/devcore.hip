@@ -86,7 +86,7 @@ uint4 jacobi_consts(uint32_t a) {
 }
 
 uint32_t magic_div(int d) {
-    return d == 1 ? 0u : (uint32_t)((0x100000000ull + d - 1) / d);
+    return (uint32_t)((0x100000000ull + d - 1) / d);  // d >= 2
 }
 
 // Fills the job constants from a plan, table size m (0: cost model) and
@@ -119,9 +119,10 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.m = m;
     J.x0 = pl->x0;
     if (tw == 3) tw = 2;
-    // T in whole giant-step tiles: the extra steps search x past x1, whose
-    // n exceed maxN and are dropped
-    J.T = ((pl->x1 - pl->x0 + 1 + m - 1) / m + tw - 1) / tw * tw;
+    // T in whole giant-step tiles, at least 2 (so idx / T is one high
+    // product): the extra steps search x past x1, whose n exceed maxN and
+    // are dropped
+    J.T = std::max(2, ((pl->x1 - pl->x0 + 1 + m - 1) / m + tw - 1) / tw * tw);
     J.minN = pl->minN;
     J.maxN = pl->maxN;
     J.nrows = nrows;

@@ -41,7 +41,7 @@ struct JobDev {
     int nrows;
     int nk;       // distinct k
     int logNB;    // baby table: NB = 2^logNB buckets of 8 16-bit fingerprints
-    uint32_t invT;  // ceil(2^32 / T) for T >= 2, else 0: idx / T for idx < 2^16
+    uint32_t invT;  // ceil(2^32 / T) (T >= 2): idx / T for idx < 2^16
     int tw;         // giant-step tile: consecutive t of one row per lane, 1, 2 or 4
                     // (T is rounded up to a multiple of tw)
     uint32_t kmax;  // largest k: baby values below about p*kmax/2^32 get a

@@ -54,11 +54,10 @@ __device__ __forceinline__ u64 readlane64(u64 x, int l) {
     return ((u64)hi << 32) | lo;
 }
 
-// idx / T for idx < 2^16: umulhi with invT = ceil(2^32 / T) when T >= 2
-// (invT is 0 for T == 1, where the quotient is idx itself).
-__device__ __forceinline__ int div_T(u32 idx, u32 invT) {
-    return invT ? (int)__umulhi(idx, invT) : (int)idx;
-}
+// idx / T for idx < 2^16 and 2 <= T < 2^16: umulhi with invT = ceil(2^32 / T)
+// (invT = (2^32 + e) / T with e < T, so idx * invT / 2^32 exceeds idx / T by
+// less than 1/T).
+__device__ __forceinline__ int div_T(u32 idx, u32 invT) { return (int)__umulhi(idx, invT); }
 
 // Table hash of a residue: the bucket is taken from bits 4 and up of its
 // low word and the 16-bit fingerprint is its high half.  Empty slots hold
