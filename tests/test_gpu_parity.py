@@ -143,6 +143,26 @@ def tiny_job(rs, base, kn):
     return rs.Plan(rs.Abcd.from_lists(base, kn))
 
 
+def test_single_window_job_two_giant_steps(rs, orc, gpu):
+    """A job whose x-span fits one baby window still gets two giant steps
+    per row at tile 1 (the second searches past maxN, whose n are dropped):
+    it equals the exhaustive search on small primes."""
+    plan = tiny_job(rs, 6, {5: list(range(100, 400, 3)), 11: [101, 131, 377], 35: [200, 399]})
+    primes = [q for q in range(5, 40000, 2) if is_prime(q)]
+    offs = np.array([q - 3 for q in primes], np.uint32)
+    rk, rr, _ = plan.rows()
+    job = orc.Job(plan.base, plan.Q, 256, plan.minN, plan.maxN, rk, rr, plan.c)
+    brute = []
+    for q in primes:
+        brute += orc.brute_prime(job, q)
+    assert len(brute) > 0
+    sv = rs.Sieve(plan, m=256, tile=1)
+    assert sv.T == 2
+    sv.submit(3, offs)
+    assert sv.collect() == brute
+    sv.close()
+
+
 def test_tiny_primes_vs_exhaustive(rs, orc, gpu):
     """Small p (orders of b^Q below m, p | k, p | b) against brute force."""
     plan = tiny_job(rs, 6, {5: list(range(100, 700, 3)), 11: list(range(101, 650, 2)),
