@@ -646,13 +646,20 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
     // 2^32, which is BH[r] times the plain G^t by Shoup's method
     const u32 mp0 = lo32(mp);
     if (fpm) {  // BH and G^t are doubles in the record
-        for (int idx = lane; idx < Q * T; idx += 64) {
+        auto yent = [&](int idx) {
             const int r = div_T((u32)idx, J.invT), t = idx - r * T;
             double zd = fp_mulmod(__longlong_as_double((long long)R[REC_BH + r]),
                                   __longlong_as_double((long long)R[J.off_gt + 2 * t]), pd, pinv);
             zd = zd < 0.0 ? zd + pd : zd;
             const u64 z = fp_bits(zd) & ((1ull << 52) - 1);
             Y[r * J.ys + t] = shoup32_entry(z, p, mp0);
+        };
+        if (Q * T <= 192) {  // at most three rounds, unrolled
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (64 * j < Q * T && 64 * j + lane < Q * T) yent(64 * j + lane);
+        } else {
+            for (int idx = lane; idx < Q * T; idx += 64) yent(idx);
         }
     } else {
         for (int idx = lane; idx < Q * T; idx += 64) {
