@@ -54,6 +54,28 @@ __device__ __forceinline__ u64 readlane64(u64 x, int l) {
     return ((u64)hi << 32) | lo;
 }
 
+// Loads from the job's tables, which no kernel writes: through the
+// constant address space for wave-uniform addresses (scalar loads), and
+// the global one for per-lane addresses (instead of flat loads, which
+// also count as LDS operations in the wait counters).  Vector types go
+// through plain integer loads.
+template <typename T>
+__device__ __forceinline__ T const_ld(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ T global_ld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ uint2 global_ld(const uint2* p) {
+    const u64 v = global_ld((const u64*)p);
+    return make_uint2((u32)v, (u32)(v >> 32));
+}
+__device__ __forceinline__ uint4 const_ld(const uint4* p) {
+    const u64 a = const_ld((const u64*)p), b = const_ld((const u64*)p + 1);
+    return make_uint4((u32)a, (u32)(a >> 32), (u32)b, (u32)(b >> 32));
+}
+
 // idx / T for idx < 2^16 and 2 <= T < 2^16: umulhi with invT = ceil(2^32 / T)
 // (invT = (2^32 + e) / T with e < T, so idx * invT / 2^32 exceeds idx / T by
 // less than 1/T).
@@ -125,7 +147,8 @@ __device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32
             bool keep = n >= J.minN && n <= J.maxN;
             if (keep && J.bitmap) {
                 const long long xp = xx - J.x0;  // x' of the bitmap, < 32 words
-                keep = (J.bitmap[(long long)row * J.words + (xp >> 5)] >> (31 - (xp & 31))) & 1;
+                keep = (global_ld(J.bitmap + (long long)row * J.words + (xp >> 5)) >>
+                        (31 - (xp & 31))) & 1;
             }
             if (keep) {
                 const u32 at = atomicAdd(hit_count, 1u);
@@ -161,6 +184,10 @@ __device__ __forceinline__ S uniform_load(const S* p) {
 
 // Lane mask of a per-lane condition (straight to a scalar register pair).
 __device__ __forceinline__ u64 lanes(bool c) { return __builtin_amdgcn_ballot_w64(c); }
+
+// Whether this lane's bit of a wave-uniform mask is set (the mask becomes
+// the execution mask of the guarded code, no vector instruction).
+__device__ __forceinline__ bool in_mask(u64 m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
 // Giant steps of one prime: the A*T grid of (active row a, step t) is cut
 // into runs of TW consecutive t of one row (T is a multiple of TW), one run
@@ -233,7 +260,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
                 bool hit = false;
-                if ((full[u] >> lane) & 1) {
+                if (in_mask(full[u])) {
                     u32 b = (ba[u] - tab_addr) >> 4;
                     while (true) {
                         b = (b + 1) & nbm;
@@ -289,7 +316,7 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
         const u32 mlo = (u32)__builtin_amdgcn_readlane(lo32(wm), w);
         const u32 mhi = (u32)__builtin_amdgcn_readlane(hi32(wm), w);
         const u64 mask = ((u64)mhi << 32) | mlo;
-        if ((mask >> lane) & 1) {
+        if (in_mask(mask)) {
             const u32 below = __builtin_amdgcn_mbcnt_hi(hi32(mask),
                                                         __builtin_amdgcn_mbcnt_lo(lo32(mask), 0));
             entry(A + below);
@@ -313,7 +340,7 @@ __device__ __forceinline__ int compact_rows(const JobDev& J, const uint2* rows_l
 #pragma unroll
         for (int w = 0; w < ROW_SLICE / 64; ++w) {
             const int row = r0 + 64 * w + lane;
-            ent[w] = row < r1 ? J.rowent[row] : make_uint2(0, 0);
+            ent[w] = row < r1 ? global_ld(J.rowent + row) : make_uint2(0, 0);
         }
 #pragma unroll
         for (int w = 0; w < ROW_SLICE / 64; ++w)
@@ -462,7 +489,7 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
     int cur = 0;
     u64 lo = 0, hi = 0;  // words cur, cur + 1
     for (int i = 0; i < J.nk; ++i) {
-        const int w0 = (int)J.kw0[i];
+        const int w0 = (int)const_ld(J.kw0 + i);
         if (w0 != cur) {
             RM[cur] = lo;
             if (w0 == cur + 1) {
@@ -474,8 +501,8 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
             hi = 0;
             cur = w0;
         }
-        const uint4 kj = J.kjac[i];
-        const u32 tabw = J.kjt[i];
+        const uint4 kj = const_ld(J.kjac + i);
+        const u32 tabw = const_ld(J.kjt + i);
         int jkv;
         if (jb == 0) {
             jkv = 0;
@@ -492,13 +519,13 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
             if (r >= kj.x) r -= kj.x;
             if (r >= kj.x) r -= kj.x;
             const u32 x = (u32)r;
-            jkv = (J.jtab[tabw + (x >> 5)] >> (x & 31)) & 1 ? t : -t;
+            jkv = (global_ld(J.jtab + tabw + (x >> 5)) >> (x & 31)) & 1 ? t : -t;
         } else {
             jkv = jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
         }
         const u64* const km = J.kmask + 4 * i;
-        if (jkv == want) lo |= km[0], hi |= km[1];
-        if (jkv * jb == want) lo |= km[2], hi |= km[3];
+        if (jkv == want) lo |= const_ld(km), hi |= const_ld(km + 1);
+        if (jkv * jb == want) lo |= const_ld(km + 2), hi |= const_ld(km + 3);
     }
     RM[cur] = lo;
     if (cur + 1 < nw) RM[cur + 1] = hi;
@@ -728,7 +755,8 @@ __device__ __forceinline__ void sieve_body(
     const bool rows_in_lds = shared_words > 0;
     if (rows_in_lds) {
         const JobDev J0 = uniform_load(jobs + c0.job);
-        for (int i = threadIdx.x; i < J0.nrows; i += blockDim.x) rows_lds[i] = J0.rowent[i];
+        for (int i = threadIdx.x; i < J0.nrows; i += blockDim.x)
+            rows_lds[i] = global_ld(J0.rowent + i);
         __syncthreads();
     }
     // the kernel has no static LDS (checked by the host), so the dynamic
@@ -774,7 +802,7 @@ __device__ __forceinline__ void sieve_body(
         if (n <= 0) break;  // device-counted chunk: groups past the end are empty
 
         // set-up, one lane per prime
-        if (lane < n) prep_record(J, C.low + C.offs[q0 + lane], S + lane * J.W);
+        if (lane < n) prep_record(J, C.low + global_ld(C.offs + q0 + lane), S + lane * J.W);
         wave_sync_global();
 
         // BSGS, the wavefront per prime; record words 0..63 of the next prime
