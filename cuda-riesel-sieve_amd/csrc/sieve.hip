@@ -131,7 +131,8 @@ __device__ __forceinline__ bool bucket_match(uint4 e, u32 fp2) {
 // entry is (k, yent) in the slice starting at row r0.  With n-list bitmaps
 // only n in the row's list is emitted.  g64 is the Montgomery form of g^64,
 // so the chain keeps vstart's form: plain for p < 2^50, else Montgomery.
-__device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32 k, u32 yent,
+__device__ __forceinline__ void rescan(const JobDev& J, const JobDev* __restrict__ jobs, int job,
+                                       u64 target, u32 k, u32 yent,
                                        int r0, int tt, u64 vstart, u64 g64, u64 p, u64 mp,
                                        HitDev* __restrict__ hits, u32* __restrict__ hit_count,
                                        u32 hit_cap) {
@@ -142,12 +143,17 @@ __device__ __forceinline__ void rescan(const JobDev& J, int job, u64 target, u32
     for (int v = 0; v < (J.m >> 6); ++v) {
         if (w == target) {
             const int j = lane + 64 * v;
-            const long long xx = (long long)J.x0 + (long long)tt * J.m + (J.m - 1 - j);
+            // the job constants only a hit needs are read here, not kept
+            // in registers through the giant steps
+            const JobDev* const jp = jobs + job;
+            const int x0 = const_ld(&jp->x0);
+            const long long xx = (long long)x0 + (long long)tt * J.m + (J.m - 1 - j);
             const long long n = (long long)J.Q * xx + (int)r;
-            bool keep = n >= J.minN && n <= J.maxN;
-            if (keep && J.bitmap) {
-                const long long xp = xx - J.x0;  // x' of the bitmap, < 32 words
-                keep = (global_ld(J.bitmap + (long long)row * J.words + (xp >> 5)) >>
+            bool keep = n >= const_ld(&jp->minN) && n <= const_ld(&jp->maxN);
+            const uint32_t* const bitmap = const_ld(&jp->bitmap);
+            if (keep && bitmap) {
+                const long long xp = xx - x0;  // x' of the bitmap, < 32 words
+                keep = (global_ld(bitmap + (long long)row * const_ld(&jp->words) + (xp >> 5)) >>
                         (31 - (xp & 31))) & 1;
             }
             if (keep) {
@@ -204,7 +210,8 @@ __device__ __forceinline__ bool in_mask(u64 m) { return __builtin_amdgcn_inverse
 // the reduced product.  The loop is wave-uniform because rescan() needs
 // every lane.
 template <int TW, bool CHAIN, bool EXACT>
-__device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int r0,
+__device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __restrict__ jobs,
+                                            int job, int A, int r0,
                                             u32 tab_addr,
                                             const u32* __restrict__ cnt,
                                             const char* __restrict__ Yb,
@@ -284,7 +291,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, int job, int A, int
                 // the exact product of the matched item
                 const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
                 const u64 target = shoup32_full(k, yv, p);
-                rescan(J, job, target, k, yent, r0, tt, vstart, g64, p, mp, hits, hit_count,
+                rescan(J, jobs, job, target, k, yent, r0, tt, vstart, g64, p, mp, hits, hit_count,
                        hit_cap);
             }
         }
@@ -539,7 +546,8 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
 // wavefront's LDS (counts, Y, row list, record) at wbase.  With idx (only
 // with rows_in_lds) the row list holds row numbers into rows_lds.
 template <int BB>
-__device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
+__device__ __forceinline__ void bsgs_prime(const JobDev& J, const JobDev* __restrict__ jobs,
+                                           int job, u64 pre,
                                            const u64* __restrict__ Rg, uint4* __restrict__ tab,
                                            u32 tab_addr, u32* __restrict__ wbase,
                                            const uint2* __restrict__ rows_lds,
@@ -706,7 +714,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, int job, u64 pre,
         wave_sync();
         const char* const Yb = (const char*)Y;
 #define RS_GIANT(TWV, CH, EX)                                                                \
-    giant_steps<TWV, CH, EX>(J, job, A, r0, tab_addr, cnt, Yb, list, rows_lds, idx,           \
+    giant_steps<TWV, CH, EX>(J, jobs, job, A, r0, tab_addr, cnt, Yb, list, rows_lds, idx,           \
                              vstart, g64, p, mp, hits, hit_count, hit_cap)
 #define RS_GIANT_TW(CH, EX)      \
     if (J.tw >= 4)               \
@@ -811,7 +819,7 @@ __device__ __forceinline__ void sieve_body(
         for (int i = 0; i < n; ++i) {
             const u64 cur = pre;
             if (i + 1 < n) pre = lane < J.W ? S[(i + 1) * J.W + lane] : 0;
-            bsgs_prime<BB>(J, C.job, cur, S + i * J.W, tab, tab_addr, wbase, rows_lds, rows_in_lds,
+            bsgs_prime<BB>(J, jobs, C.job, cur, S + i * J.W, tab, tab_addr, wbase, rows_lds, rows_in_lds,
                        IDX, hits, hit_count, hit_cap);
         }
         wave_sync_global();  // records are rewritten by the next group
