@@ -16,8 +16,11 @@
 //    reference keeps a chained table per thread in global memory,
 //    :358-411), form the Y table b^r*H*G^t, compact the rows that pass the
 //    Jacobi filter (:516-521) and spread the (row, t) giant steps over the
-//    lanes; a giant step is one 32x64 Montgomery product k*Y and one bucket
-//    read (:534-608).  A fingerprint match is resolved by an exact rescan of
+//    lanes; a giant step is the low word of one Shoup product k*Y (three
+//    32-bit multiplies against the precomputed quotient of Y) and one bucket
+//    read (:534-608).  Below p = 2^50 the baby chain, the Y table and the
+//    set-up multiply in double precision (fp_mulmod), above it with exact
+//    64-bit Montgomery and Shoup products.  A fingerprint match is resolved by an exact rescan of
 //    the baby steps (the reference rescans on a 16-bit key match too, :578)
 //    and every solution is appended to the hit buffer (the reference keeps
 //    the last hit per prime only, :615-621).
