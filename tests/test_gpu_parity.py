@@ -354,6 +354,44 @@ def test_p_around_2_50_both_baby_chains(rs, orc, gpu):
     assert all(rs.verify_factor(p, k, b, n) for p, k, n in got)
 
 
+def test_random_jobs_vs_oracle(rs, orc, gpu):
+    """Seeded random jobs (base, k count and size, n window, sign, m, tile
+    with and without row-number lists) over 2^16-wide windows of p from 2^20
+    to 2^62 against the oracle: layouts, tiles and both arithmetic paths on
+    shapes no fixed case has.  Small windows give hits; at large p both
+    sides must agree on (almost always) none."""
+    rng = random.Random(20261204)
+    lows = [(1 << 20) + 1] * 4 + [10**9 + 7] * 2 + [
+        10**12 + 39, (1 << 50) - (1 << 16) + 1, (1 << 50) + 1, (1 << 56) + 1,
+        (1 << 62) - (1 << 17)]
+    offs_of = {}
+    total = 0
+    for case in range(30):
+        base = rng.randrange(3, 1031)
+        nk = rng.choice([1, 3, 8, 25, 60])
+        kmax = rng.choice([10**3, 10**6, 10**9, (1 << 31) - 1])
+        nmin = rng.randrange(1000, 200000)
+        width = rng.choice([200, 3000, 40000])
+        c = rng.choice([-1, 1])
+        m = 64 * rng.randrange(1, 33)
+        tile = rng.choice([1, 2, 4]) | (rs.TILE_ROW_INDEX if rng.random() < 0.3 else 0)
+        low = rng.choice(lows)
+        if low not in offs_of:
+            offs_of[low] = rs.PrimeGen(low + (1 << 16), threads=16).range(low, low + (1 << 16))
+        offs = offs_of[low]
+        plan = rs.Plan(rs.Abcd.synth(base, nk, 3, kmax, nmin, nmin + width, 1000, seed=case,
+                                     c=c))
+        sv = rs.Sieve(plan, m=m, tile=tile, max_batch=1 << 16)
+        sv.submit(low, offs)
+        got = sv.collect()
+        sv.close()
+        want = run_oracle(orc, plan, m, low, offs)
+        assert got == want, (case, base, nk, kmax, nmin, width, c, m, tile, low)
+        assert all(rs.verify_factor(p, k, base, n, c) for p, k, n in got[:200]), case
+        total += len(got)
+    assert total > 200
+
+
 def test_row_index_layout_all_tiles(rs, orc, gpu):
     """Active rows listed by row number into the shared row table
     (TILE_ROW_INDEX): every tile with and without it equals the oracle on a
