@@ -132,16 +132,19 @@ __device__ __forceinline__ bool bucket_match(uint4 e, u32 fp2) {
 // Emits every baby step j with g^j == target (all 64 lanes take part:
 // lane l checks j = l + 64 v) as a hit of giant step tt of the row whose
 // entry is (k, yent) in the slice starting at row r0.  With n-list bitmaps
-// only n in the row's list is emitted.  g64 is the Montgomery form of g^64,
-// so the chain keeps vstart's form: plain for p < 2^50, else Montgomery.
+// only n in the row's list is emitted.  p, mp and g^64 come from the
+// prime's record R in LDS (a hit is rare: they need not stay in registers
+// through the giant steps); g^64 there is a Montgomery form, so the chain
+// keeps vstart's form: plain for p < 2^50, else Montgomery.
 __device__ __forceinline__ void rescan(const JobDev& J, const JobDev* __restrict__ jobs, int job,
-                                       u64 target, u32 k, u32 yent,
-                                       int r0, int tt, u64 vstart, u64 g64, u64 p, u64 mp,
-                                       HitDev* __restrict__ hits, u32* __restrict__ hit_count,
-                                       u32 hit_cap) {
+                                       const u64* R, u64 yv, u32 k, u32 yent, int r0, int tt,
+                                       u64 vstart, HitDev* __restrict__ hits,
+                                       u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
     const u32 r = ((yent & 0x7ffffu) >> 3) / (u32)J.ys;
     const u32 row = (u32)r0 + (yent >> 19);
+    const u64 p = R[REC_P], mp = R[REC_MP], g64 = R[REC_G64];
+    const u64 target = shoup32_full(k, yv, p);  // the exact product of the item
     u64 w = vstart;
     for (int v = 0; v < (J.m >> 6); ++v) {
         if (w == target) {
@@ -220,7 +223,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
                                             const char* __restrict__ Yb,
                                             const uint2* __restrict__ list,
                                             const uint2* __restrict__ rows_lds, bool idx,
-                                            u64 vstart, u64 g64, u64 p, u64 mp,
+                                            const u64* R, u32 negp0, u64 vstart,
                                             HitDev* __restrict__ hits, u32* __restrict__ hit_count,
                                             u32 hit_cap) {
     const int lane = threadIdx.x & 63;
@@ -230,7 +233,6 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
     asm volatile("" : "+v"(bmask));  // in a vector register: one and-or per address
     const int T = J.T;
     const int total = A * T;
-    const u32 negp0 = 0u - lo32(p);
     // each lane's run (row a, first step t0) is carried from round to
     // round: a round of 64*TW items moves it by rstep rows and tstep steps,
     // tstep = 0 when EXACT (t0 < T always, so only a is clamped for lanes
@@ -291,11 +293,8 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
                 const u32 k = __builtin_amdgcn_readlane(row.x, src);
                 const u32 yent = __builtin_amdgcn_readlane(row.y, src);
                 const int tt = __builtin_amdgcn_readlane(t0, src) + u;
-                // the exact product of the matched item
                 const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
-                const u64 target = shoup32_full(k, yv, p);
-                rescan(J, jobs, job, target, k, yent, r0, tt, vstart, g64, p, mp, hits, hit_count,
-                       hit_cap);
+                rescan(J, jobs, job, R, yv, k, yent, r0, tt, vstart, hits, hit_count, hit_cap);
             }
         }
         a += rstep;
@@ -570,7 +569,6 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, const JobDev* __rest
     if (lane < W) R[lane] = pre;
     for (int w = 64 + lane; w < W; w += 64) R[w] = Rg[w];
     const u64 p = readlane64(pre, REC_P), mp = readlane64(pre, REC_MP);
-    const u64 g64 = readlane64(pre, REC_G64);
     const u64 gw = readlane64(pre, REC_GW), gws = readlane64(pre, REC_GWS);
 
     // clear table and counts
@@ -718,7 +716,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, const JobDev* __rest
         const char* const Yb = (const char*)Y;
 #define RS_GIANT(TWV, CH, EX)                                                                \
     giant_steps<TWV, CH, EX>(J, jobs, job, A, r0, tab_addr, cnt, Yb, list, rows_lds, idx,           \
-                             vstart, g64, p, mp, hits, hit_count, hit_cap)
+                             R, 0u - lo32(p), vstart, hits, hit_count, hit_cap)
 #define RS_GIANT_TW(CH, EX)      \
     if (J.tw >= 4)               \
         RS_GIANT(4, CH, EX);     \
