@@ -141,23 +141,39 @@ extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table
 }
 
 // One persistent launch over count primes (offsets in device memory).
-static rs_status sieve_launch(rs_sieve* s, uint64_t low, const uint32_t* d_offs, int64_t count) {
+// One persistent launch of chunk c0 (ngroups groups of 64 primes) between
+// two timing events; primes: the count credited to the timing (0 when the
+// device counts them).
+static rs_status timed_launch(rs_sieve* s, const ChunkDev& c0, long long ngroups,
+                              int64_t primes) {
     Timing t{};
-    HIPCHK(hipEventCreate(&t.a));
-    HIPCHK(hipEventCreate(&t.b));
-    t.primes = count;
-    HIPCHK(hipEventRecord(t.a, s->stream));
+    t.primes = primes;
+    if (hipEventCreate(&t.a) != hipSuccess) return RS_ERR_HIP;
+    if (hipEventCreate(&t.b) != hipSuccess) {
+        (void)hipEventDestroy(t.a);
+        return RS_ERR_HIP;
+    }
+    rs_status st = hipEventRecord(t.a, s->stream) == hipSuccess ? RS_OK : RS_ERR_HIP;
+    if (st == RS_OK) st = launch(s->L, s->stream, s->job.d_self, c0, nullptr, 1, ngroups);
+    if (st == RS_OK && hipEventRecord(t.b, s->stream) != hipSuccess) st = RS_ERR_HIP;
+    if (st != RS_OK) {  // the events are kept only on success
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+        return st;
+    }
+    s->timings.push_back(t);
+    return RS_OK;
+}
+
+// One persistent launch over count primes (offsets in device memory).
+static rs_status sieve_launch(rs_sieve* s, uint64_t low, const uint32_t* d_offs, int64_t count) {
     ChunkDev c0{};
     c0.low = low;
     c0.offs = d_offs;
     c0.count = count;
     c0.group0 = 0;
     c0.job = 0;
-    rs_status st = launch(s->L, s->stream, s->job.d_self, c0, nullptr, 1, (count + 63) / 64);
-    if (st != RS_OK) return st;
-    HIPCHK(hipEventRecord(t.b, s->stream));
-    s->timings.push_back(t);
-    return RS_OK;
+    return timed_launch(s, c0, (count + 63) / 64, count);
 }
 
 extern "C" rs_status rs_sieve_submit_device(rs_sieve* s, uint64_t low,
@@ -236,23 +252,17 @@ extern "C" rs_status rs_sieve_submit_range(rs_sieve* s, uint64_t lo, uint64_t hi
         rs_status st = gpu_primes_enqueue(s->gpg, a, b, s->d_range, s->range_cap, s->stream,
                                           &cnt_dev);
         if (st != RS_OK) return st;
-        Timing t{};
-        HIPCHK(hipEventCreate(&t.a));
-        HIPCHK(hipEventCreate(&t.b));
-        t.primes = 0;  // counted on the device, see d_range_primes
-        HIPCHK(hipEventRecord(t.a, s->stream));
         ChunkDev c0{};
         c0.low = a;
         c0.offs = s->d_range;
         c0.count = s->range_cap;
         c0.count_dev = cnt_dev;
-        st = launch(s->L, s->stream, s->job.d_self, c0, nullptr, 1, (s->range_cap + 63) / 64);
+        // the primes are counted on the device, see d_range_primes
+        st = timed_launch(s, c0, (s->range_cap + 63) / 64, 0);
         if (st != RS_OK) return st;
-        HIPCHK(hipEventRecord(t.b, s->stream));
         hipLaunchKernelGGL(rs_accum_count, dim3(1), dim3(64), 0, s->stream, cnt_dev,
                            s->d_range_primes);
         HIPCHK(hipGetLastError());
-        s->timings.push_back(t);
         a = b;
     }
     return RS_OK;

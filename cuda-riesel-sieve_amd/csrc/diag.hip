@@ -140,24 +140,39 @@ extern "C" rs_status rs_dev_free(int device, void* ptr) {
     return RS_OK;
 }
 
+// Owners of a device buffer and of two events, released on every return.
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+}  // namespace
+
 extern "C" rs_status rs_dev_mad_rate(int device, int blocks, int iters, double* ops_per_s) {
     if (!ops_per_s || blocks < 1 || iters < 1) return RS_ERR_ARG;
     HIPCHK(hipSetDevice(device));
-    uint64_t* out = nullptr;
-    HIPCHK(hipMalloc(&out, (size_t)blocks * 256 * 8));
-    hipEvent_t a, b;
-    HIPCHK(hipEventCreate(&a));
-    HIPCHK(hipEventCreate(&b));
-    hipLaunchKernelGGL(rs_mad_probe, dim3(blocks), dim3(256), 0, 0, out, iters);  // warm-up
-    HIPCHK(hipEventRecord(a, 0));
-    hipLaunchKernelGGL(rs_mad_probe, dim3(blocks), dim3(256), 0, 0, out, iters);
-    HIPCHK(hipEventRecord(b, 0));
-    HIPCHK(hipEventSynchronize(b));
+    DevBuf out;
+    if (hipMalloc(&out.p, (size_t)blocks * 256 * 8) != hipSuccess) return RS_ERR_NOMEM;
+    uint64_t* const o = (uint64_t*)out.p;
+    EventPair ev;
+    HIPCHK(hipEventCreate(&ev.a));
+    HIPCHK(hipEventCreate(&ev.b));
+    hipLaunchKernelGGL(rs_mad_probe, dim3(blocks), dim3(256), 0, 0, o, iters);  // warm-up
+    HIPCHK(hipEventRecord(ev.a, 0));
+    hipLaunchKernelGGL(rs_mad_probe, dim3(blocks), dim3(256), 0, 0, o, iters);
+    HIPCHK(hipEventRecord(ev.b, 0));
+    HIPCHK(hipEventSynchronize(ev.b));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    (void)hipFree(out);
+    HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
     *ops_per_s = (double)blocks * 256.0 * 8.0 * iters / (ms * 1e-3);
     return RS_OK;
 }
@@ -165,31 +180,27 @@ extern "C" rs_status rs_dev_mad_rate(int device, int blocks, int iters, double* 
 extern "C" rs_status rs_dev_copy_rate(int device, int64_t bytes, double* bytes_per_s) {
     if (!bytes_per_s || bytes < 16 || bytes % 16 != 0) return RS_ERR_ARG;
     HIPCHK(hipSetDevice(device));
-    uint4 *src = nullptr, *dst = nullptr;
-    if (hipMalloc(&src, (size_t)bytes) != hipSuccess) return RS_ERR_NOMEM;
-    if (hipMalloc(&dst, (size_t)bytes) != hipSuccess) {
-        (void)hipFree(src);
+    DevBuf sbuf, dbuf;
+    if (hipMalloc(&sbuf.p, (size_t)bytes) != hipSuccess ||
+        hipMalloc(&dbuf.p, (size_t)bytes) != hipSuccess)
         return RS_ERR_NOMEM;
-    }
-    HIPCHK(hipMemset(src, 1, (size_t)bytes));
-    hipEvent_t a, b;
-    HIPCHK(hipEventCreate(&a));
-    HIPCHK(hipEventCreate(&b));
+    const uint4* const src = (const uint4*)sbuf.p;
+    uint4* const dst = (uint4*)dbuf.p;
+    HIPCHK(hipMemset(sbuf.p, 1, (size_t)bytes));
+    EventPair ev;
+    HIPCHK(hipEventCreate(&ev.a));
+    HIPCHK(hipEventCreate(&ev.b));
     const long long n = bytes / 16;
     int cus = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     const int grid = cus * 16;
     hipLaunchKernelGGL(rs_copy_probe, dim3(grid), dim3(256), 0, 0, src, dst, n);  // warm-up
-    HIPCHK(hipEventRecord(a, 0));
+    HIPCHK(hipEventRecord(ev.a, 0));
     hipLaunchKernelGGL(rs_copy_probe, dim3(grid), dim3(256), 0, 0, src, dst, n);
-    HIPCHK(hipEventRecord(b, 0));
-    HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipEventRecord(ev.b, 0));
+    HIPCHK(hipEventSynchronize(ev.b));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    (void)hipFree(src);
-    (void)hipFree(dst);
+    HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
     *bytes_per_s = 2.0 * (double)bytes / (ms * 1e-3);  // read + write
     return RS_OK;
 }
