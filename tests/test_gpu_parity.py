@@ -303,6 +303,26 @@ def test_r745_shaped_window(rs, orc, gpu):
     assert got == want
 
 
+def test_full_size_r745_layouts_agree(rs, gpu):
+    """The whole R745-shaped benchmark range, p in [1e12, 1e12 + 2^34) (about
+    6.2e8 primes, generated on the device), sieved twice with different
+    layouts: m = 512, tile 4, full row entries (the 4-wave build) and m = 768,
+    tile 2 with row-number lists (a 5-wave build).  The factor set is fixed by
+    the mathematics, so the two must be equal at full size, and every factor
+    must verify by modpow."""
+    plan = rs.Plan(rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2))
+    lo, hi = 10**12, 10**12 + (1 << 34)
+    res = []
+    for m, tile in ((512, 4), (768, 2 | rs.TILE_ROW_INDEX)):
+        sv = rs.Sieve(plan, m=m, tile=tile)
+        sv.submit_range(lo, hi)
+        res.append(sv.collect())
+        sv.close()
+    assert res[0] == res[1]
+    assert len(res[0]) > 500
+    assert all(rs.verify_factor(p, k, 745, n) for p, k, n in res[0])
+
+
 def test_large_p_near_2_62_and_2_63(rs, orc, gpu):
     plan = rs.Plan(rs.Abcd.synth(2, 12, 3, 10**9, 1000000, 1003000, 1000, seed=3))
     for low in ((1 << 62) - (1 << 18), (1 << 63) - (1 << 19)):
