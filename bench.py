@@ -161,7 +161,7 @@ def main():
     hbm = rs.dev_copy_rate(copy_bytes)
     # HBM bytes per prime from the committed counter summary (FETCH_SIZE and
     # WRITE_SIZE scaled by the copy-probe calibration), times primes per launch
-    traffic, prof_cfg = None, "?"
+    traffic, prof_cfg, valu_busy = None, "?", None
     prof = os.path.join(ROOT, "profiles", "rs_sieve_kernel_r745_pmc.txt")
     if os.path.exists(prof):
         for line in open(prof):
@@ -169,13 +169,23 @@ def main():
                 traffic = float(line.split(":")[1]) * per_launch_primes
             if line.startswith("config:"):
                 prof_cfg = line.split(":", 1)[1].strip()
+            if "% busy)" in line:  # VALU issue share from the counter summary
+                valu_busy = float(line.rsplit("(", 1)[1].split("%")[0]) / 100
+    # the published vector FP64 rate, 78.6 TFLOP/s, is 39.3e12 full-rate VALU
+    # lane-operations per second (1024 SIMDs x 16 lanes x 2.4 GHz)
+    published = 78.6e12 / 2
     roof = {"bound": "valu", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "Gmul/s",
             "frac": achieved / peak, "traffic": traffic,
             "traffic_source": f"profiles/rs_sieve_kernel_r745_pmc.txt ({prof_cfg})",
             "kernel": "rs_sieve_kernel", "muls_per_prime": muls_per_prime,
             "primes_per_launch": per_launch_primes,
             "mean_launch_ms": bsgs_s_per_launch * 1e3,
-            "hbm_copy_GBps": hbm / 1e9, "copy_probe_bytes": copy_bytes}
+            "hbm_copy_GBps": hbm / 1e9, "copy_probe_bytes": copy_bytes,
+            "published_valu_peak": published / 1e9,
+            "frac_of_published_valu_peak": achieved / published,
+            "valu_issue_busy_frac": valu_busy,
+            "valu_issue_source": "profiles/rs_sieve_kernel_r745_pmc.txt (VALU instructions "
+                                 "per prime x 4 cycles x primes/s over 1024 SIMDs x 2.4 GHz)"}
 
     # the same chunks handed over in host memory (the reference's flow:
     # pinned double-buffered copies overlapped with the sieve)
