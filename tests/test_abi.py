@@ -69,3 +69,12 @@ def test_cli_help_and_bad_flags():
     r = subprocess.run([exe, "-i", "/nonexistent.abcd", "-p", "1000", "-P", "2000"],
                        capture_output=True, text=True, timeout=30)
     assert r.returncode == 1 and "Unable to open file" in r.stdout
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    why, before touching a device."""
+    r = subprocess.run(["python3", os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
