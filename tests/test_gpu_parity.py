@@ -525,6 +525,19 @@ def test_device_resident_and_repeat(rs, gpu):
     assert t["primes"] == 2 * len(offs) and t["kernel_ms"] > 0
 
 
+def test_layout_beyond_lds_is_rejected(rs, gpu):
+    """A baby table that cannot fit the 160 KB of LDS of a compute unit (m =
+    32768: 16384 buckets, 256 KB) is refused at creation with RS_ERR_ARG, not
+    launched; the next sieve on the device works."""
+    plan = rs.Plan(rs.Abcd.synth(745, 4, 3, 10**6, 10000, 20000, 1000, seed=1))
+    with pytest.raises(rs.RsError) as e:
+        rs.Sieve(plan, m=32768)
+    assert e.value.status == rs.RS_ERR_ARG
+    low = 10**9 + 1
+    offs = rs.PrimeGen(low + (1 << 16)).range(low, low + (1 << 16))
+    assert run_device(rs, plan, low, offs, m=512) == run_device(rs, plan, low, offs, m=256)
+
+
 def test_hit_buffer_overflow_is_reported(rs, gpu):
     plan = rs.Plan(rs.Abcd.synth(5, 10, 3, 10**5, 2000, 60000, 200, seed=4, odd_k=False))
     low = 10**6 + 1
