@@ -96,8 +96,7 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     if (!pl) return RS_ERR_ARG;
     if (m <= 0) m = rs_plan_auto_m(pl);
     const bool row_index = (tw & RS_TILE_ROW_INDEX) != 0;
-    const bool dense = (tw & RS_TILE_DENSE) != 0;
-    tw &= ~(RS_TILE_ROW_INDEX | RS_TILE_DENSE);
+    tw &= ~RS_TILE_ROW_INDEX;
     if (tw <= 0) tw = 2;
     if (tw > 4) return RS_ERR_ARG;
     if (m < 64 || m % 64 != 0 || m > (1 << 16)) return RS_ERR_ARG;
@@ -129,21 +128,11 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.nrows = nrows;
     J.nk = (int)kvals.size();
     // buckets: average load between 1.5 and 3 of the 8 slots, counting the
-    // second entries of baby values below p*kmax/2^32 (a kmax/2^32 share).
-    // dense: the fewest buckets that still hold 2m entries with a slot to
-    // spare, the worst case (below p = kmax every baby value goes in twice);
-    // an insert or a chained probe walks to a bucket that is not full, so
-    // the table must never fill
+    // second entries of baby values below p*kmax/2^32 (a kmax/2^32 share)
     const int entries = m + (int)(((uint64_t)m * kvals.back() >> 32) + 1);
     int logNB = 0;
-    if (dense) {
-        while ((8 << logNB) <= 2 * m) ++logNB;
-    } else {
-        while ((1 << logNB) * 3 < entries) ++logNB;
-    }
+    while ((1 << logNB) * 3 < entries) ++logNB;
     J.logNB = std::max(logNB, 2);
-    J.dense = dense ? 1 : 0;
-    if ((8ll << J.logNB) <= 2ll * m) return RS_ERR_ARG;
     J.ys = J.T + (J.T % 2 == 0 ? 1 : 0);
     if (J.Q * J.ys >= 65536 || J.T > 65535) return RS_ERR_ARG;
     J.invT = magic_div(J.T);

@@ -132,9 +132,7 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int t
 extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
                                    int* lds_bytes_per_wave, int* tile) {
     if (!s) return RS_ERR_ARG;
-    if (tile)
-        *tile = s->job.J.tw | (s->job.J.list_idx ? RS_TILE_ROW_INDEX : 0) |
-                (s->job.J.dense ? RS_TILE_DENSE : 0);
+    if (tile) *tile = s->job.J.tw | (s->job.J.list_idx ? RS_TILE_ROW_INDEX : 0);
     if (m) *m = s->job.J.m;
     if (T) *T = s->job.J.T;
     if (table_slots) *table_slots = 8 << s->job.J.logNB;
@@ -426,12 +424,6 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
             }
             if (!fits) break;
         }
-        // the dense table at the best pair, and with the larger m it allows
-        const int bm = tu.best_m, bt = tu.best_tile;
-        for (int m : {bm, round64(bm * 5 / 4), round64(bm * 3 / 2)}) {
-            rs_status st = tu.time(m, bt | RS_TILE_DENSE, &ns);
-            if (st != RS_OK && st != RS_ERR_ARG) return st;
-        }
     } else {
         // coordinate descent: the tile at m0 (or the largest m that fits),
         // then m in steps of m0/4 up and down while the time improves
@@ -458,13 +450,11 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
             }
             if (tu.best_m != m) break;  // improved going up: skip going down
         }
-        // the row-index layout and the dense table at the best pair
+        // the row-index layout at the best pair
         if (pl->rows_k.size() <= (size_t)ROW_SLICE) {
             st = tu.time(tu.best_m, tu.best_tile | RS_TILE_ROW_INDEX, &ns);
             if (st != RS_OK && st != RS_ERR_ARG) return st;
         }
-        st = tu.time(tu.best_m, tu.best_tile | RS_TILE_DENSE, &ns);
-        if (st != RS_OK && st != RS_ERR_ARG) return st;
     }
     if (ncand) *ncand = tu.n;
     *best_m = tu.best_m;
@@ -550,9 +540,7 @@ extern "C" rs_status rs_batch_create(int device, int njobs, const rs_plan* const
 extern "C" rs_status rs_batch_job_info(const rs_batch* b, int job, int* m, int* T,
                                        int* lds_bytes, int* tile) {
     if (!b || job < 0 || job >= (int)b->jobs.size()) return RS_ERR_ARG;
-    if (tile)
-        *tile = b->jobs[job].J.tw | (b->jobs[job].J.list_idx ? RS_TILE_ROW_INDEX : 0) |
-                (b->jobs[job].J.dense ? RS_TILE_DENSE : 0);
+    if (tile) *tile = b->jobs[job].J.tw | (b->jobs[job].J.list_idx ? RS_TILE_ROW_INDEX : 0);
     if (m) *m = b->jobs[job].J.m;
     if (T) *T = b->jobs[job].J.T;
     if (lds_bytes) *lds_bytes = b->jobs[job].J.lds_words * 4;

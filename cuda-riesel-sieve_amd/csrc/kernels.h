@@ -55,7 +55,6 @@ struct JobDev {
     int list_idx;   // 1: single-job launches list active rows by row number
                     // (RS_TILE_ROW_INDEX; at most ROW_SLICE rows)
     int tab_words;  // of which the table: 4 NB, a power of two
-    int dense;      // 1: the table was sized for load 3-6 (RS_TILE_DENSE)
     int ys;         // Y row stride in entries: T, or T + 1 when T is even, so
                     // that rows of different r start on different LDS banks
     const uint4* kjac;        // per distinct k (ascending): odd part n, 2-adic

@@ -114,8 +114,6 @@ _lib = None
 
 # tile flag: active rows listed by row number (rsieve.h RS_TILE_ROW_INDEX)
 TILE_ROW_INDEX = 16
-# tile flag: baby table of half the buckets (rsieve.h RS_TILE_DENSE)
-TILE_DENSE = 32
 
 class RsError(RuntimeError):
     def __init__(self, what, status):

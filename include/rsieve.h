@@ -178,21 +178,13 @@ int rs_format_factor(const rs_factor* f, uint32_t b, int c, char* buf, int cap);
  * wavefront lists its active rows as row numbers into the workgroup's
  * shared row table instead of copies of the rows, a smaller LDS layout
  * (more resident wavefronts) for one more LDS read per giant-step round;
- * which is faster depends on the job (the autotuner times both); and
- * optionally | RS_TILE_DENSE: the smallest baby table that holds the worst
- * case of 2m entries (average load up to 4 of 8 slots instead of 1.5-3),
- * half the LDS of the default where its rounding to a power of two
- * overshoots (m = 768, 1536, ...), for more resident wavefronts, at the
- * price of more buckets overflowing into the next (the giant steps then
- * also check fill counts) and more fingerprint rescans.  Results do not
- * depend on the flags.
+ * which is faster depends on the job (the autotuner times both).
  * Replaces the set-up at :1367-1512.  max_batch bounds the primes copied
  * from host memory per launch (the reference's arraySize, :851);
  * hit_capacity bounds the device append buffer (the reference keeps one
  * slot per prime, :615-621). */
 typedef struct rs_sieve rs_sieve;
 #define RS_TILE_ROW_INDEX 16
-#define RS_TILE_DENSE 32
 
 rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int tile,
                           int64_t max_batch, int64_t hit_capacity,
@@ -234,10 +226,9 @@ void rs_sieve_destroy(rs_sieve* s);
  * effort 0 times the full grid: multiples of 64 from 1/2 to 2 times the
  * cost model's m that fit one wavefront's LDS limit, each with tiles 1, 2
  * and 4, each with and without RS_TILE_ROW_INDEX (plans of at most 512
- * rows), then RS_TILE_DENSE at the best pair and at 5/4 and 3/2 of its m.
- * effort 1 is a coordinate descent (tile at the model's m, then m in steps
- * of m/4 while the time improves, then the row-index layout and
- * RS_TILE_DENSE at the best pair), typically 7-10 timings.  cand_m,
+ * rows).  effort 1 is a coordinate descent (tile at the model's m, then m
+ * in steps of m/4 while the time improves, then the row-index layout at
+ * the best pair), typically 6-9 timings.  cand_m,
  * cand_tile, ns_per_prime (optional, capacity cap) receive the measured
  * candidates, *ncand their number. */
 rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,

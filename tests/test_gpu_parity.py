@@ -376,7 +376,7 @@ def test_p_around_2_50_both_baby_chains(rs, orc, gpu):
 
 def test_random_jobs_vs_oracle(rs, orc, gpu):
     """Seeded random jobs (base, k count and size, n window, sign, m, tile
-    with and without row-number lists, every third with the dense table) over 2^16-wide windows of p from 2^20
+    with and without row-number lists) over 2^16-wide windows of p from 2^20
     to 2^62 against the oracle: layouts, tiles and both arithmetic paths on
     shapes no fixed case has.  Small windows give hits; at large p both
     sides must agree on (almost always) none."""
@@ -395,7 +395,6 @@ def test_random_jobs_vs_oracle(rs, orc, gpu):
         c = rng.choice([-1, 1])
         m = 64 * rng.randrange(1, 33)
         tile = rng.choice([1, 2, 4]) | (rs.TILE_ROW_INDEX if rng.random() < 0.3 else 0)
-        tile |= rs.TILE_DENSE if case % 3 == 2 else 0
         low = rng.choice(lows)
         if low not in offs_of:
             offs_of[low] = rs.PrimeGen(low + (1 << 16), threads=16).range(low, low + (1 << 16))
@@ -415,25 +414,18 @@ def test_random_jobs_vs_oracle(rs, orc, gpu):
 
 def test_row_index_layout_all_tiles(rs, orc, gpu):
     """Active rows listed by row number into the shared row table
-    (TILE_ROW_INDEX) and the dense baby table (TILE_DENSE, whose overflowing
-    buckets put most primes on the chained giant-step path): every tile with
-    and without them equals the oracle on a window, and the exhaustive search
-    on tiny primes."""
+    (TILE_ROW_INDEX): every tile with and without it equals the oracle on a
+    window, and the exhaustive search on tiny primes."""
     plan = rs.Plan(rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2))
     assert plan.nrows <= 512
     low = 10**9 + 1
     offs = rs.PrimeGen(low + 2**20).range(low, low + 2**20)
     want = run_oracle(orc, plan, 512, low, offs)
     assert len(want) > 0
-    # the dense table differs from the default at m = 768 (256 buckets, not
-    # 512); the hit set does not depend on m
     for tile in (1, 2, 4):
-        for flag in (0, rs.TILE_ROW_INDEX, rs.TILE_DENSE, rs.TILE_ROW_INDEX | rs.TILE_DENSE):
-            m = 768 if flag & rs.TILE_DENSE else 512
-            sv = rs.Sieve(plan, m=m, tile=tile | flag)
+        for flag in (0, rs.TILE_ROW_INDEX):
+            sv = rs.Sieve(plan, m=512, tile=tile | flag)
             assert sv.tile == tile | flag
-            if flag & rs.TILE_DENSE:
-                assert sv.slots == 8 * 256
             sv.submit(low, offs)
             assert sv.collect() == want, (tile, flag)
             sv.close()
@@ -446,13 +438,10 @@ def test_row_index_layout_all_tiles(rs, orc, gpu):
     brute = []
     for q in primes:
         brute += orc.brute_prime(job, q)
-    # dense at m = 768: below p = kmax every baby value goes in twice, 1536
-    # entries in 2048 slots, so most probes follow overflow chains
-    for m, flag in ((128, rs.TILE_ROW_INDEX), (768, rs.TILE_DENSE)):
-        sv = rs.Sieve(tiny, m=m, tile=4 | flag)
-        sv.submit(3, toffs)
-        assert sv.collect() == brute, flag
-        sv.close()
+    sv = rs.Sieve(tiny, m=128, tile=4 | rs.TILE_ROW_INDEX)
+    sv.submit(3, toffs)
+    assert sv.collect() == brute
+    sv.close()
 
 
 def test_many_rows_sliced_all_tiles(rs, orc, gpu):
