@@ -259,9 +259,10 @@ def test_cli_sierpinski_file(tmp_path, gpu):
         assert (int(k) * pow(int(b), int(n), int(p)) + 1) % int(p) == 0
 
 
-def test_config1_full(rs, orc, gpu):
-    """b=745, 4 odd k in [3, 1e6], n in [10000, 20000]
-    presieved by odd primes < 1000, p = all primes in [1e9, 1e9 + 1e7]."""
+def test_small_base745_job_full_range(rs, orc, gpu):
+    """The CPU-sized base-745 workload in full: 4 odd k in [3, 1e6], n in
+    [10000, 20000] presieved by odd primes < 1000, p = all primes in
+    [1e9, 1e9 + 1e7]."""
     abcd = rs.Abcd.synth(745, 4, 3, 10**6, 10000, 20000, 1000, seed=1)
     plan = rs.Plan(abcd)
     low = 10**9
@@ -292,8 +293,8 @@ def test_r745_known_answer_and_window(rs, orc, gpu):
 
 
 def test_r745_shaped_window(rs, orc, gpu):
-    """BASELINE config 2 shape: 22 odd k in [3, 1e7], n in [180000, 250000]
-    presieved by odd primes < 1e5, a 2^22 window of p at 1e12."""
+    """The R745-shaped benchmark job: 22 odd k in [3, 1e7], n in [180000,
+    250000] presieved by odd primes < 1e5, a 2^22 window of p at 1e12."""
     abcd = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=2)
     plan = rs.Plan(abcd)
     low = 10**12 + 1
