@@ -413,8 +413,9 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
         const int rix = pl->rows_k.size() <= (size_t)ROW_SLICE ? RS_TILE_ROW_INDEX : 0;
         for (int m : ms) {
             bool fits = true;
-            for (int tile : {1, 2, 4, 1 | rix, 2 | rix, 4 | rix}) {
-                if (tile > 4 && !rix) continue;
+            const int tiles[6] = {1, 2, 4, 1 | rix, 2 | rix, 4 | rix};
+            for (int i = 0; i < (rix ? 6 : 3); ++i) {
+                const int tile = tiles[i];
                 rs_status st = tu.time(m, tile, &ns);
                 if (st == RS_ERR_ARG) {  // larger m will not fit either
                     fits = false;

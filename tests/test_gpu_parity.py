@@ -539,6 +539,31 @@ def test_layout_beyond_lds_is_rejected(rs, gpu):
     assert run_device(rs, plan, low, offs, m=512) == run_device(rs, plan, low, offs, m=256)
 
 
+def test_autotune_grid_and_descent(rs, orc, gpu):
+    """The autotuner's full grid times each (m, tile) pair once (with
+    row-number tiles only for plans of at most 512 rows), its descent returns
+    a pair it timed, and the tuned sieve equals the oracle."""
+    low = 10**9 + 7
+    offs = rs.PrimeGen(low + (1 << 18)).range(low, low + (1 << 18))
+    g = rs.GpuPrimeGen(low + (1 << 18) + 1)
+    buf = g.device_range(low, low + (1 << 18))
+    for nk, width in ((6, 20000), (120, 60000)):  # few rows; more than 512 rows
+        plan = rs.Plan(rs.Abcd.synth(1030, nk, 3, 10**6, 100000, 100000 + width, 1000, seed=7))
+        for effort in (0, 1):
+            m, tile, cands = rs.autotune(plan, low, buf, effort=effort)
+            pairs = [(cm, ct) for cm, ct, _ in cands]
+            assert len(pairs) == len(set(pairs)), (nk, effort)
+            assert (m, tile) in pairs
+            assert min(cands, key=lambda c: c[2])[:2] == (m, tile)
+            if plan.nrows > 512:
+                assert all(ct & rs.TILE_ROW_INDEX == 0 for _, ct in pairs)
+        sv = rs.Sieve(plan, m=m, tile=tile)
+        sv.submit(low, offs)
+        assert sv.collect() == run_oracle(orc, plan, m, low, offs), nk
+        sv.close()
+    buf.free()
+
+
 def test_hit_buffer_overflow_is_reported(rs, gpu):
     plan = rs.Plan(rs.Abcd.synth(5, 10, 3, 10**5, 2000, 60000, 200, seed=4, odd_k=False))
     low = 10**6 + 1
