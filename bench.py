@@ -72,8 +72,9 @@ def main():
                     help="Poisson arrival rate as a fraction of the plain job rate")
     ap.add_argument("--serve-active", type=int, default=16)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
-    ap.add_argument("--serve-sweep", type=str, default="",
-                    help="comma-separated loads for a Poisson rate sweep (e.g. 0.3,0.5,0.7,0.9)")
+    ap.add_argument("--serve-sweep", type=str, default="0.4,0.7,1.0,1.3",
+                    help="comma-separated loads (fractions of the plain job rate) of the "
+                         "Poisson rate sweep; empty: none")
     ap.add_argument("--serve-quantum-ms", type=float, default=40.0,
                     help="device time per job per server step (0: whole 2^30 windows)")
     ap.add_argument("--no-batch", action="store_true")
@@ -82,6 +83,15 @@ def main():
     ap.add_argument("--real-chunks", type=int, default=8)
     ap.add_argument("--edge-chunks", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-cli", action="store_true")
+    ap.add_argument("--no-msweep", action="store_true")
+    ap.add_argument("--edge-m", type=int, default=0, help="edge table size (0 = autotuned)")
+    ap.add_argument("--edge-tile", type=int, default=0)
+    ap.add_argument("--cli-span-log2", type=int, default=35,
+                    help="p-range of the drop-in command-line runs")
+    ap.add_argument("--only", choices=["plain", "real", "edge", "batched", "serving", "cli",
+                                       "msweep", "cpu"],
+                    help="run one leg alone and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -98,11 +108,23 @@ def main():
     K, W = args.steps, args.warmup
     chunk = 1 << args.chunk_log2
     p0 = 10**12
+    if args.only and args.only not in ("plain", "cpu"):
+        leg = {"real": lambda: bench_real_r745(args, rs, chunk),
+               "edge": lambda: bench_edge(args, rs, chunk),
+               "batched": lambda: bench_batched(args, rs, p0, chunk),
+               "serving": lambda: bench_serving(args, rs, p0, chunk),
+               "cli": lambda: bench_cli(args),
+               "msweep": lambda: bench_msweep(args, rs, p0)}[args.only]
+        print(json.dumps({args.only: leg()}), flush=True)
+        return
     t = time.time()
     abcd = rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=args.seed)
     plan = rs.Plan(abcd)
     log(f"job: Q={plan.Q} rows={plan.nrows} n=[{plan.minN},{plan.maxN}] "
         f"({time.time() - t:.1f}s)")
+    if args.only == "cpu":
+        print(json.dumps({"cpu": bench_cpu(args, rs, plan, args.m or 512)}), flush=True)
+        return
     t = time.time()
     gen = rs.PrimeGen(p0 + K * chunk + 1, threads=min(16, os.cpu_count() or 1))
     bufs, counts = [], []
@@ -113,8 +135,6 @@ def main():
         counts.append(len(offs))
         bufs.append(rs.DeviceBuffer(0, offs))
         host_offs.append(offs)
-        if i == 0:
-            sample_offs = offs
     gen_s = time.time() - t
     nprimes = sum(counts)
     log(f"primes: {nprimes} in [{p0}, {p0 + K * chunk}) generated+uploaded in {gen_s:.1f}s")
@@ -127,7 +147,8 @@ def main():
                 "candidates": [[cm, ct, round(ns, 4)] for cm, ct, ns in cands]}
         log(f"autotune: m={m} tile={tile} ({len(cands)} candidates, {time.time() - ts:.1f}s)")
     sv = rs.Sieve(plan, m=m, tile=tile, max_batch=args.max_batch, hit_capacity=1 << 22)
-    log(f"sieve: m={sv.m} tile={sv.tile} T={sv.T} slots={sv.slots} lds/wave={sv.lds_bytes}B")
+    log(f"sieve: m={sv.m} tile={sv.tile} T={sv.T} slots={sv.slots} lds/wave={sv.lds_bytes}B "
+        f"kernel={sv.kernel}")
     for w in range(W):
         sv.submit_device(p0 + (w % K) * chunk, bufs[w % K])
         sv.collect()
@@ -144,117 +165,62 @@ def main():
         f"kernel {tm['kernel_ms']:.1f} ms")
     prange_rate = K * chunk / elapsed
     primes_rate = nprimes / elapsed
+    roof = roofline(rs, plan, sv, tm, K, nprimes)
 
-    # algorithmic multiplies per prime in the sieve kernel's BSGS phase: m
-    # baby steps and Q*T Y products at 11 32-bit multiplies each (full 64x64
-    # Montgomery), A*T giant steps at 5 (32x64 Montgomery), A = rows/2
-    # expected Jacobi survivors (the set-up phase is not counted)
-    A = plan.nrows / 2.0
-    muls_per_prime = 11 * sv.m + 11 * plan.Q * sv.T + 5 * A * sv.T
-    launches = K  # one persistent launch per chunk
-    per_launch_primes = nprimes / launches
-    bsgs_s_per_launch = tm["kernel_ms"] / 1e3 / launches
-    achieved = muls_per_prime * per_launch_primes / bsgs_s_per_launch
-    peak = rs.dev_mad_rate()
-    # streaming copy of a known size: HBM reference for the traffic counters
-    copy_bytes = 1 << 30
-    hbm = rs.dev_copy_rate(copy_bytes)
-    # HBM bytes per prime from the committed counter summary (FETCH_SIZE and
-    # WRITE_SIZE scaled by the copy-probe calibration), times primes per launch
-    traffic, prof_cfg, valu_busy = None, "?", None
-    prof = os.path.join(ROOT, "profiles", "rs_sieve_kernel_r745_pmc.txt")
-    if os.path.exists(prof):
-        for line in open(prof):
-            if line.startswith("traffic_bytes_per_prime:"):
-                traffic = float(line.split(":")[1]) * per_launch_primes
-            if line.startswith("config:"):
-                prof_cfg = line.split(":", 1)[1].strip()
-            if "% busy)" in line:  # VALU issue share from the counter summary
-                valu_busy = float(line.rsplit("(", 1)[1].split("%")[0]) / 100
-    # the published vector FP64 rate, 78.6 TFLOP/s, is 39.3e12 full-rate VALU
-    # lane-operations per second (1024 SIMDs x 16 lanes x 2.4 GHz)
-    published = 78.6e12 / 2
-    roof = {"bound": "valu", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "Gmul/s",
-            "frac": achieved / peak, "traffic": traffic,
-            "traffic_source": f"profiles/rs_sieve_kernel_r745_pmc.txt ({prof_cfg})",
-            "kernel": "rs_sieve_kernel", "muls_per_prime": muls_per_prime,
-            "primes_per_launch": per_launch_primes,
-            "mean_launch_ms": bsgs_s_per_launch * 1e3,
-            "hbm_copy_GBps": hbm / 1e9, "copy_probe_bytes": copy_bytes,
-            "published_valu_peak": published / 1e9,
-            "frac_of_published_valu_peak": achieved / published,
-            "valu_issue_busy_frac": valu_busy,
-            "valu_issue_source": "profiles/rs_sieve_kernel_r745_pmc.txt (VALU instructions "
-                                 "per prime x 4 cycles x primes/s over 1024 SIMDs x 2.4 GHz)"}
-
-    # the same chunks handed over in host memory (the reference's flow:
-    # pinned double-buffered copies overlapped with the sieve)
-    sv.collect()
-    sv.timing()
-    t0 = time.perf_counter()
-    for i in range(K):
-        sv.submit(p0 + i * chunk, host_offs[i])
-    hits_host = sv.collect()
-    host_s = time.perf_counter() - t0
-    if hits_host != hits:
-        raise RuntimeError("host-submitted chunks disagree with the device-resident run")
-    pcie = {"value": K * chunk / host_s, "unit": "p/sec", "primes_per_sec": nprimes / host_s,
-            "what": "same chunks from host memory through pinned double buffers "
-                    f"({args.max_batch} primes per copy), PCIe copies included"}
-    log(f"host submits (PCIe included): {K * chunk / host_s:.3e} p/s")
+    pcie = end_to_end = None
+    if args.only != "plain":
+        # the same chunks handed over in host memory (the reference's flow:
+        # pinned double-buffered copies overlapped with the sieve)
+        sv.collect()
+        sv.timing()
+        t0 = time.perf_counter()
+        for i in range(K):
+            sv.submit(p0 + i * chunk, host_offs[i])
+        hits_host = sv.collect()
+        host_s = time.perf_counter() - t0
+        if hits_host != hits:
+            raise RuntimeError("host-submitted chunks disagree with the device-resident run")
+        pcie = {"value": K * chunk / host_s, "unit": "p/sec", "primes_per_sec": nprimes / host_s,
+                "what": "same chunks from host memory through pinned double buffers "
+                        f"({args.max_batch} primes per copy), PCIe copies included"}
+        log(f"host submits (PCIe included): {K * chunk / host_s:.3e} p/s")
+        # end to end on the device: primes generated on the GPU, then sieved
+        # (same p-range, no host generation, no PCIe), as an extra figure
+        sv.collect()
+        sv.timing()
+        t0 = time.perf_counter()
+        for i in range(K):
+            sv.submit_range(p0 + i * chunk, p0 + (i + 1) * chunk)
+        hits_e2e = sv.collect()
+        e2e_s = time.perf_counter() - t0
+        e2e_primes = sv.timing()["primes"]
+        if hits_e2e != hits or e2e_primes != nprimes:
+            raise RuntimeError("device-generated primes disagree with the host-generated run")
+        end_to_end = {"value": K * chunk / e2e_s, "unit": "p/sec",
+                      "primes_per_sec": e2e_primes / e2e_s,
+                      "what": "GPU prime generation + sieve of the same p-range, timed together"}
+        log(f"end to end (GPU primes): {K * chunk / e2e_s:.3e} p/s")
     del host_offs
+    run_m = sv.m
+    sv.close()
+    for b in bufs:
+        b.free()
 
-    # end to end on the device: primes generated on the GPU, then sieved
-    # (same p-range, no host generation, no PCIe), as an extra figure
-    sv.collect()
-    sv.timing()
-    t0 = time.perf_counter()
-    for i in range(K):
-        sv.submit_range(p0 + i * chunk, p0 + (i + 1) * chunk)
-    hits_e2e = sv.collect()
-    e2e_s = time.perf_counter() - t0
-    e2e_primes = sv.timing()["primes"]
-    if hits_e2e != hits or e2e_primes != nprimes:
-        raise RuntimeError("device-generated primes disagree with the host-generated run")
-    end_to_end = {"value": K * chunk / e2e_s, "unit": "p/sec", "primes_per_sec": e2e_primes / e2e_s,
-                  "what": "GPU prime generation + sieve of the same p-range, timed together"}
-    log(f"end to end (GPU primes): {K * chunk / e2e_s:.3e} p/s")
-
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-
-        rk, rr, _ = plan.rows()
-        job = oracle.Job(plan.base, plan.Q, sv.m, plan.minN, plan.maxN, rk, rr, plan.c)
-        n = 2000
-        while True:
-            ts = time.perf_counter()
-            oracle.bsgs_many(job, p0, sample_offs[:n], threads=args.cpu_threads)
-            dt = time.perf_counter() - ts
-            if dt > args.cpu_seconds / 4 or n >= len(sample_offs):
-                break
-            n = min(len(sample_offs), int(n * max(2.0, args.cpu_seconds / 4 / max(dt, 1e-3))))
-        span = int(sample_offs[n - 1]) + 1
-        cpu = {"value": span / dt, "unit": "p/sec", "cores": args.cpu_threads, "kind": "port",
-               "primes_per_sec": n / dt,
-               "sample": f"first {n} primes of [1e12, 1e12+2^30) (p-range {span}), "
-                         f"same job and m, {dt:.2f}s"}
-        log(f"cpu: {n} primes in {dt:.2f}s on {args.cpu_threads} threads")
-        # one core, on a sample sized to about a quarter of the budget
-        n1 = max(1000, min(len(sample_offs), int(n / args.cpu_threads / 4)))
-        ts = time.perf_counter()
-        oracle.bsgs_many(job, p0, sample_offs[:n1], threads=1)
-        dt1 = time.perf_counter() - ts
-        span1 = int(sample_offs[n1 - 1]) + 1
-        cpu["one_core"] = {"value": span1 / dt1, "unit": "p/sec", "cores": 1,
-                           "primes_per_sec": n1 / dt1,
-                           "sample": f"first {n1} primes of [1e12, 1e12+2^30), {dt1:.2f}s"}
-        log(f"cpu: {n1} primes in {dt1:.2f}s on 1 thread")
-
-    real = None if args.no_real else bench_real_r745(args, rs, chunk)
-    edge = None if args.no_edge else bench_edge(args, rs, chunk)
-    batched = None if args.no_batch else bench_batched(args, rs, p0, chunk)
-    serving = None if args.no_serve else bench_serving(args, rs, p0, chunk)
+    legs = not args.only
+    cpu = None if args.no_cpu or not legs else bench_cpu(args, rs, plan, run_m)
+    if cpu:
+        # the survey's count of Montgomery products per prime (the oracle's
+        # restatement of the reference kernel at this m, 11 multiplies each)
+        mm = cpu.pop("_montmuls_per_prime")
+        roof["survey_montmuls_per_prime"] = mm
+        roof["survey_frac_of_published_valu_peak"] = (
+            11 * mm * primes_rate / roof["published_valu_peak"] / 1e9)
+    real = None if args.no_real or not legs else bench_real_r745(args, rs, chunk)
+    cli = None if args.no_cli or not legs else bench_cli(args)
+    edge = None if args.no_edge or not legs else bench_edge(args, rs, chunk)
+    msweep = None if args.no_msweep or not legs else bench_msweep(args, rs, p0)
+    batched = None if args.no_batch or not legs else bench_batched(args, rs, p0, chunk)
+    serving = None if args.no_serve or not legs else bench_serving(args, rs, p0, chunk)
 
     out = {
         "metric": "primes/sec sieved (22 k, n 180k-250k) plain vs batched-serve; p99 job latency",
@@ -272,7 +238,8 @@ def main():
         "config": {"workload": "R745-shaped plain BSGS: base 745, 22 odd k in [3,1e7] "
                                f"(seed {args.seed}), n in [180000,250000] presieved by odd "
                                f"primes < 1e5, p in [1e12, 1e12+{K}*2^{args.chunk_log2})",
-                   "Q": plan.Q, "rows": plan.nrows, "m": sv.m, "T": sv.T, "tile": sv.tile,
+                   "Q": plan.Q, "rows": plan.nrows, "m": run_m, "T": roof["T"],
+                   "tile": roof["tile"], "kernel": roof["kernel"],
                    "primes": nprimes, "chunk": chunk, "global_batch": counts[0],
                    "parallelism": "single GPU, wavefront per prime"},
         "primes_per_sec": primes_rate,
@@ -285,11 +252,155 @@ def main():
         "cpu_baseline": cpu,
         "autotune": tune,
         "r745_abcd": real,
+        "drop_in_cli": cli,
         "edge_base2": edge,
+        "m_sweep": msweep,
         "batched": batched,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def profile_summary(kernel, workload):
+    """Counter figures of the committed rocprofv3 summary for this kernel
+    build and workload (profiles/<kernel>_<workload>_pmc.txt), or {}."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    out["source"] = os.path.relpath(path, ROOT)
+    for line in open(path):
+        key, _, val = line.partition(":")
+        key = key.strip()
+        if key in ("traffic_bytes_per_prime", "valu_insts_per_prime", "valu_issue_busy_frac",
+                   "lds_bank_conflict_cycles_per_prime", "salu_insts_per_prime"):
+            out[key] = float(val.split()[0])
+        elif key == "config":
+            out["config"] = val.strip()
+    return out
+
+
+def roofline(rs, plan, sv, tm, launches, nprimes):
+    """The dominant kernel's roofline line.  The kernel is bound by VALU
+    instruction issue (exact 32-bit integer and FP64 arithmetic), neither
+    HBM (4 bytes in per prime) nor MFMA.  achieved = algorithmic 32-bit
+    multiplies per prime x primes per launch / mean launch time (HIP events
+    on the sieve's own stream); peak = the published VALU lane-operation
+    rate of MI355X (78.6 TFLOP/s vector FP64 = 39.3e12 full-rate lane
+    operations/s: 1024 SIMDs x 16 lanes x 2.4 GHz)."""
+    # multiplies per prime of the BSGS phase: m baby steps and Q*T Y-table
+    # products at 11 (full 64x64 Montgomery), A*T giant steps at 5, A =
+    # rows/2 expected Jacobi survivors (the set-up phase is not counted)
+    A = plan.nrows / 2.0
+    muls = 11 * sv.m + 11 * plan.Q * sv.T + 5 * A * sv.T
+    per_launch = nprimes / launches
+    launch_s = tm["kernel_ms"] / 1e3 / launches
+    achieved = muls * per_launch / launch_s
+    published = 78.6e12 / 2
+    mad = rs.dev_mad_rate()
+    copy_bytes = 1 << 30
+    hbm = rs.dev_copy_rate(copy_bytes)
+    prof = profile_summary(sv.kernel, "r745")
+    traffic = prof.get("traffic_bytes_per_prime")
+    return {"bound": "valu", "achieved": achieved / 1e9, "peak": published / 1e9,
+            "unit": "Gmul/s", "frac": achieved / published,
+            "traffic": traffic * per_launch if traffic else None,
+            "traffic_bytes_per_prime": traffic, "profile": prof,
+            "kernel": sv.kernel, "m": sv.m, "T": sv.T, "tile": sv.tile,
+            "muls_per_prime": muls, "primes_per_launch": per_launch,
+            "mean_launch_ms": launch_s * 1e3,
+            "published_valu_peak": published / 1e9,
+            "measured_mad_rate": mad / 1e9, "frac_of_measured_mad_rate": achieved / mad,
+            "hbm_copy_GBps": hbm / 1e9, "copy_probe_bytes": copy_bytes,
+            "hbm_frac": (traffic * per_launch / launch_s / 8e12) if traffic else None}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def bench_cpu(args, rs, plan, m):
+    """The CPU baseline: the oracle (the reference algorithm restated in C++
+    with exact 128-bit products, OpenMP over primes) on this host, at one
+    core and at all cores, BSGS phase alone (primes generated beforehand) and
+    end to end (prime generation + BSGS + modpow verification of every hit):
+    the small base-745 workload in full (4 k, n 10000-20000, every prime of
+    [1e9, 1e9+1e7]); a fixed 2^26-wide window of the R745-shaped workload
+    (all cores; one core on its first primes); sr_745.abcd over the
+    reference's default test range."""
+    import oracle
+
+    threads = args.cpu_threads
+
+    def job_of(pl, mm):
+        rk, rr, _ = pl.rows()
+        return oracle.Job(pl.base, pl.Q, mm, pl.minN, pl.maxN, rk, rr, pl.c)
+
+    def run(pl, mm, lo, hi, nth, limit=None):
+        """BSGS-only and end-to-end seconds for the primes of [lo, hi)."""
+        job = job_of(pl, mm)
+        t = time.perf_counter()
+        offs = rs.PrimeGen(hi, threads=nth).range(lo, hi)
+        gen_s = time.perf_counter() - t
+        if limit is not None:
+            offs = offs[:limit]
+        t = time.perf_counter()
+        hits, mmc = oracle.bsgs_many(job, lo, offs, threads=nth)
+        bsgs_s = time.perf_counter() - t
+        t = time.perf_counter()
+        for p, k, n in hits:
+            oracle.verify(p, k, pl.base, n, pl.c)
+        ver_s = time.perf_counter() - t
+        span = (int(offs[-1]) + 1) if limit is not None else hi - lo
+        if limit is not None:  # generation scaled to the primes sieved
+            gen_s *= len(offs) / max(1, limit)
+        e2e = gen_s + bsgs_s + ver_s
+        return {"cores": nth, "primes": int(len(offs)), "p_range": span,
+                "bsgs_s": bsgs_s, "end_to_end_s": e2e,
+                "bsgs_p_per_sec": span / bsgs_s, "end_to_end_p_per_sec": span / e2e,
+                "bsgs_primes_per_sec": len(offs) / bsgs_s, "factors": len(hits),
+                "montmuls_per_prime": mmc / max(1, len(offs))}
+
+    out = {"kind": "port", "cpu_model": cpu_model(), "threads_all": threads}
+    # the R745-shaped window (the headline workload)
+    lo = 10**12
+    w_all = run(plan, m, lo, lo + (1 << 26), threads)
+    w_one = run(plan, m, lo, lo + (1 << 26), 1, limit=60000)
+    log(f"cpu: R745-shaped 2^26 window {w_all['bsgs_s']:.2f}s on {threads} threads, "
+        f"{w_one['primes']} primes {w_one['bsgs_s']:.2f}s on 1")
+    # the small base-745 workload in full
+    c1 = rs.Plan(rs.Abcd.synth(745, 4, 3, 10**6, 10000, 20000, 1000, seed=1))
+    c1m = c1.auto_m()
+    c1_all = run(c1, c1m, 10**9, 10**9 + 10**7 + 1, threads)
+    c1_one = run(c1, c1m, 10**9, 10**9 + 10**7 + 1, 1)
+    log(f"cpu: small base-745 workload {c1_all['bsgs_s']:.2f}s / {c1_one['bsgs_s']:.2f}s")
+    # sr_745.abcd over the reference's default range
+    r745 = rs.Plan(rs.Abcd.read(os.path.join(ROOT, "tests", "golden", "sr_745.abcd")), 18)
+    d0 = 102297149770000
+    r_all = run(r745, 512, d0, d0 + 2500000, threads)
+    r_one = run(r745, 512, d0, d0 + 2500000, 1)
+    log(f"cpu: sr_745.abcd default range {r_all['bsgs_s']:.2f}s / {r_one['bsgs_s']:.2f}s")
+    out.update({
+        "value": w_all["bsgs_p_per_sec"], "unit": "p/sec", "cores": threads,
+        "primes_per_sec": w_all["bsgs_primes_per_sec"],
+        "sample": f"R745-shaped job (same job and m = {m}), every prime of [1e12, 1e12+2^26) "
+                  f"({w_all['primes']} primes), BSGS phase, {w_all['bsgs_s']:.2f}s",
+        "r745_shaped_window": {"all_cores": w_all, "one_core": w_one},
+        "small_base745_full": {"workload": "base 745, 4 odd k in [3,1e6] (seed 1), n in "
+                                           "[10000,20000] presieved by odd primes < 1000, every "
+                                           f"prime of [1e9, 1e9+1e7], m = {c1m}",
+                               "all_cores": c1_all, "one_core": c1_one},
+        "sr745_default_range": {"workload": "sr_745.abcd, Q=18, m=512, every prime of "
+                                            "[102297149770000, +2.5e6)",
+                                "all_cores": r_all, "one_core": r_one},
+        "_montmuls_per_prime": w_all["montmuls_per_prime"]})
+    return out
 
 
 def timed_device_chunks(rs, sv, lows, bufs):
@@ -353,8 +464,11 @@ def bench_edge(args, rs, chunk):
     bufs = [gpg.device_range(lo, lo + chunk) for lo in lows]
     gen_s = time.time() - t
     nprimes = sum(b.count for b in bufs)
-    m, tile, cands = rs.autotune(plan, lows[0], bufs[0], count=min(bufs[0].count, 1 << 20),
-                                 effort=1)
+    if args.edge_m:
+        m, tile, cands = args.edge_m, args.edge_tile, []
+    else:
+        m, tile, cands = rs.autotune(plan, lows[0], bufs[0], count=min(bufs[0].count, 1 << 20),
+                                     effort=1)
     sv = rs.Sieve(plan, m=m, tile=tile, max_batch=1 << 10, hit_capacity=1 << 20)
     hits, dt, tm = timed_device_chunks(rs, sv, lows, bufs)
     out = {"workload": f"base 2, 500 odd k in [3,1e9] (seed 3), n in [1e6,2e6] presieved by "
@@ -362,7 +476,7 @@ def bench_edge(args, rs, chunk):
                        f"p in [2^62 - {K}*2^30, 2^62)",
            "value": K * chunk / dt, "unit": "p/sec", "primes_per_sec": nprimes / dt,
            "primes": nprimes, "m": sv.m, "tile": sv.tile, "T": sv.T,
-           "lds_bytes_per_wave": sv.lds_bytes, "factors": len(hits),
+           "lds_bytes_per_wave": sv.lds_bytes, "kernel": sv.kernel, "factors": len(hits),
            "kernel_ms": tm["kernel_ms"], "gpu_primegen_s": gen_s, "synth_s": synth_s,
            "autotune": [[cm, ct, round(ns, 3)] for cm, ct, ns in cands]}
     log(f"edge: m={sv.m} tile={sv.tile} T={sv.T} lds/wave={sv.lds_bytes}B "
@@ -407,6 +521,7 @@ def bench_batched(args, rs, p0, chunk):
         sv.submit_device(p0, buf)
         plain_hits.append(sv.collect())
     plain_s = time.perf_counter() - t0
+    plain_kernel_ms = sum(sv.timing()["kernel_ms"] for sv in svs)
     for sv in svs:
         sv.close()
     log(f"batched: plain back-to-back {plain_s:.3f}s")
@@ -417,7 +532,13 @@ def bench_batched(args, rs, p0, chunk):
     bat.run()  # captures the graph (one launch per occupancy class), then launches it
     hits = bat.collect()
     batched_s = time.perf_counter() - t0
-    log(f"batched: one graph {batched_s:.3f}s")
+    t0 = time.perf_counter()
+    bat.run()  # replays the captured graph
+    replay_hits = bat.collect()
+    replay_s = time.perf_counter() - t0
+    if replay_hits != hits:
+        raise RuntimeError("graph replay hits differ from the first run")
+    log(f"batched: one graph {batched_s:.3f}s (capture included), replay {replay_s:.3f}s")
     per_job = [[] for _ in plans]
     for (j, p, k, n) in hits:
         per_job[j].append((p, k, n))
@@ -429,10 +550,13 @@ def bench_batched(args, rs, p0, chunk):
     return {"workload": f"{len(plans)} heterogeneous jobs (base U[3,1030], k-count log-U[1,200], "
                         "n-width log-U[1e4,5e5] from n=1e5, presieve < 1e5; seed 4), each over "
                         "the p-chunk [1e12, 1e12+2^30)",
-            "plain_s": plain_s, "batched_s": batched_s,
+            "plain_s": plain_s, "batched_s": batched_s, "replay_s": replay_s,
             "plain_primes_per_sec": primes / plain_s,
             "batched_primes_per_sec": primes / batched_s,
-            "batched_vs_plain": plain_s / batched_s,
+            "replay_primes_per_sec": primes / replay_s,
+            "batched_vs_plain": plain_s / batched_s, "replay_vs_plain": plain_s / replay_s,
+            "plain_kernel_ms": plain_kernel_ms,
+            "plain_kernel_busy_frac": plain_kernel_ms / 1e3 / plain_s,
             "factors": len(hits), "hits_equal_plain": True,
             "ms": ms, "tiles": tiles, "tune_s": tune_s}
 
@@ -466,11 +590,12 @@ def serve_run(rs, plans, ms, tiles, ns, p0, p_hi, chunk, arrivals, args):
         if active == 0 and queued == 0 and nxt < n:
             time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
     results = [srv.results(j) for j in ids]
-    primes = srv.stats()["primes"]
+    st = srv.stats()
     srv.close()
     lat = np.array([d - a for d, a in zip(done_at, arrivals)])
     makespan = max(done_at) - min(arrivals)
-    return lat, results, makespan, primes, max_active
+    serve_run.kernel_ms = st["kernel_ms"]  # sieve launches of this run
+    return lat, results, makespan, st["primes"], max_active
 
 
 def bench_serving(args, rs, p0, chunk):
@@ -488,6 +613,7 @@ def bench_serving(args, rs, p0, chunk):
     buf.free()
     # plain back-to-back: one job at a time, device-generated primes per chunk
     plain_hits, plain_job_s = [], []
+    plain_kernel_ms = 0.0
     t0 = time.perf_counter()
     for pl, m, tl in zip(plans, ms, tiles):
         ts = time.perf_counter()
@@ -495,7 +621,9 @@ def bench_serving(args, rs, p0, chunk):
         for c in range(args.serve_chunks):
             sv.submit_range(p0 + c * chunk, p0 + (c + 1) * chunk)
         plain_hits.append(sv.collect())
-        primes_job = sv.timing()["primes"]
+        tmj = sv.timing()
+        primes_job = tmj["primes"]
+        plain_kernel_ms += tmj["kernel_ms"]
         sv.close()
         plain_job_s.append(time.perf_counter() - ts)
     plain_s = time.perf_counter() - t0
@@ -506,6 +634,7 @@ def bench_serving(args, rs, p0, chunk):
                        f"{args.serve_budget_gib} GiB, at most {args.serve_active} active, "
                        f"windows of ~{args.serve_quantum_ms} ms device time (at most 2^30)",
            "plain_s": plain_s, "plain_primes_per_sec": total_primes / plain_s,
+           "plain_sieve_kernel_busy_frac": plain_kernel_ms / 1e3 / plain_s,
            "plain_job_s_p50": float(np.percentile(plain_job_s, 50)),
            "plain_job_s_p99": float(np.percentile(plain_job_s, 99))}
     # saturated: every job arrives at t = 0
@@ -517,6 +646,7 @@ def bench_serving(args, rs, p0, chunk):
                         "latency_p50_s": float(np.percentile(lat, 50)),
                         "latency_p99_s": float(np.percentile(lat, 99)),
                         "max_active": act,
+                        "sieve_kernel_busy_frac": serve_run.kernel_ms / 1e3 / span,
                         "vs_plain": plain_s / span}
     log(f"serving: saturated makespan {span:.2f}s p99 {np.percentile(lat, 99):.2f}s")
     # Poisson arrivals at a fraction of the plain job rate
@@ -529,6 +659,7 @@ def bench_serving(args, rs, p0, chunk):
                       "aggregate_primes_per_sec": primes / span, "makespan_s": span,
                       "latency_p50_s": float(np.percentile(lat, 50)),
                       "latency_p99_s": float(np.percentile(lat, 99)),
+                      "sieve_kernel_busy_frac": serve_run.kernel_ms / 1e3 / span,
                       "max_active": act}
     log(f"serving: poisson rate {rate:.2f}/s p50 {np.percentile(lat, 50):.2f}s "
         f"p99 {np.percentile(lat, 99):.2f}s")
@@ -544,13 +675,92 @@ def bench_serving(args, rs, p0, chunk):
             sweep.append({"load": load, "rate_jobs_per_s": rate_l,
                           "latency_p50_s": float(np.percentile(lat, 50)),
                           "latency_p99_s": float(np.percentile(lat, 99)),
-                          "aggregate_primes_per_sec": primes / span, "max_active": act})
+                          "aggregate_primes_per_sec": primes / span, "max_active": act,
+                          "vs_plain_rate": primes / span / (total_primes / plain_s)})
             log(f"serving sweep: load {load} p50 {np.percentile(lat, 50):.2f}s "
                 f"p99 {np.percentile(lat, 99):.2f}s")
         out["sweep"] = sweep
     out["hits_equal_plain"] = True
     out["factors"] = sum(len(h) for h in plain_hits)
     out["ms"], out["tiles"] = ms, tiles
+    return out
+
+
+def bench_cli(args):
+    """The reference's own flow through the drop-in host: rsieve on
+    sr_745.abcd at the README's A100 flags (-b 5 -m 8 -s 192 -Q 18) from
+    102297149770000 over a 2^span p-range: the CPU prime generator
+    overlapped with the sieve, pinned double-buffered copies, verification
+    and the factor file.  The rate is the one rsieve prints, computed as the
+    reference does (kernel_v0.24.6.cu:1552-1559, :1783): p-range over the
+    wall time of the loop.  Then the same with -g (primes generated on the
+    GPU).  Both runs must report the same factors."""
+    import re
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+    abcd = os.path.join(ROOT, "tests", "golden", "sr_745.abcd")
+    p0 = 102297149770000
+    p1 = p0 + (1 << args.cli_span_log2)
+    out = {"workload": f"rsieve -i sr_745.abcd -b 5 -m 8 -s 192 -Q 18 -p {p0} -P {p1} "
+                       f"(2^{args.cli_span_log2} p-range)"}
+    found = {}
+    for name, extra in (("cpu_primes", []), ("gpu_primes", ["-g"])):
+        with tempfile.TemporaryDirectory() as d:
+            ff = os.path.join(d, "factors.txt")
+            cmd = [exe, "-i", abcd, "-b", "5", "-m", "8", "-s", "192", "-Q", "18", "-p", str(p0),
+                   "-P", str(p1), "-O", ff] + extra
+            t = time.perf_counter()
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+            wall = time.perf_counter() - t
+            if r.returncode != 0:
+                raise RuntimeError(f"rsieve failed ({r.returncode}): {r.stderr[-500:]}")
+            prog = re.search(r"Progress = (\d+) at (\d+) p/sec", r.stdout)
+            secs = re.search(r"Time taken: ([0-9.e+-]+)s", r.stdout)
+            nf = int(re.search(r"Factors found: (\d+)", r.stdout).group(1))
+            found[name] = open(ff).read().splitlines() if os.path.exists(ff) else []
+        out[name] = {"value": float(prog.group(2)), "unit": "p/sec",
+                     "p_range": int(prog.group(1)), "loop_s": float(secs.group(1)),
+                     "process_wall_s": wall, "factors": nf,
+                     "vs_a100_published": float(prog.group(2)) / A100_PUBLISHED,
+                     "flags": " ".join(a for a in cmd[1:] if a not in ("-O", ff))}
+        log(f"cli {name}: {out[name]['value']:.3e} p/s, {nf} factors, {wall:.1f}s")
+    if sorted(found["cpu_primes"]) != sorted(found["gpu_primes"]):
+        raise RuntimeError("rsieve with CPU and GPU primes wrote different factors")
+    return out
+
+
+def bench_msweep(args, rs, p0):
+    """Table sizes m = 2^10 .. 2^14 for the R745-shaped job (tiles 4 and 4
+    with row-number lists) on the primes of [1e12, 1e12+2^28): kernel ns per
+    prime, or why a layout is refused."""
+    plan = rs.Plan(rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=args.seed))
+    gpg = rs.GpuPrimeGen(p0 + (1 << 28) + 1)
+    buf = gpg.device_range(p0, p0 + (1 << 28))
+    out = []
+    for m in (1024, 2048, 4096, 8192, 16384):
+        for tile in (4, 4 | rs.TILE_ROW_INDEX):
+            rec = {"m": m, "tile": tile}
+            try:
+                sv = rs.Sieve(plan, m=m, tile=tile, max_batch=1 << 10, hit_capacity=1 << 20)
+            except rs.RsError as e:
+                rec["refused"] = rs.strerror(e.status)
+                out.append(rec)
+                continue
+            sv.submit_device(p0, buf)  # warm-up
+            sv.collect()
+            sv.timing()
+            sv.submit_device(p0, buf)
+            sv.collect()
+            tm = sv.timing()
+            rec.update({"T": sv.T, "kernel": sv.kernel, "lds_bytes_per_wave": sv.lds_bytes,
+                        "ns_per_prime": tm["kernel_ms"] * 1e6 / buf.count})
+            sv.close()
+            out.append(rec)
+            log(f"msweep: m={m} tile={tile} {rec['ns_per_prime']:.3f} ns/prime "
+                f"({sv.kernel}, {rec['lds_bytes_per_wave']} B)")
+    buf.free()
     return out
 
 

@@ -99,8 +99,10 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     tw &= ~RS_TILE_ROW_INDEX;
     if (tw <= 0) tw = 2;
     if (tw > 4) return RS_ERR_ARG;
-    if (m < 64 || m % 64 != 0 || m > (1 << 16)) return RS_ERR_ARG;
+    if (tw == 3) tw = 2;
     if (pl->Q < 2 || pl->Q % 2 != 0 || pl->minN < 1 || pl->rows_k.empty()) return RS_ERR_ARG;
+    // index limits of the giant-step grid (rsh::shape_ok)
+    if (!rsh::shape_ok(pl, m, tw)) return RS_ERR_ARG;
     const int nrows = (int)pl->rows_k.size();
     std::vector<uint32_t> kvals, rowinfo(nrows), rowk(nrows);
     for (int32_t k : pl->rows_k)
@@ -118,11 +120,10 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.Q = pl->Q;
     J.m = m;
     J.x0 = pl->x0;
-    if (tw == 3) tw = 2;
     // T in whole giant-step tiles, at least 2 (so idx / T is one high
     // product): the extra steps search x past x1, whose n exceed maxN and
     // are dropped
-    J.T = std::max(2, ((pl->x1 - pl->x0 + 1 + m - 1) / m + tw - 1) / tw * tw);
+    J.T = rsh::giant_T(pl, m, tw);
     J.minN = pl->minN;
     J.maxN = pl->maxN;
     J.nrows = nrows;
@@ -134,14 +135,10 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     while ((1 << logNB) * 3 < entries) ++logNB;
     J.logNB = std::max(logNB, 2);
     J.ys = J.T + (J.T % 2 == 0 ? 1 : 0);
-    if (J.Q * J.ys >= 65536 || J.T > 65535) return RS_ERR_ARG;
     J.invT = magic_div(J.T);
     J.tw = tw;
     J.kmax = kvals.back();
     J.plus = pl->c == 1;
-    // flattened giant-step indices of a row slice (and their division by T)
-    // stay below 2^16
-    if ((long long)std::min<int>(nrows, ROW_SLICE) * J.T + 64 * 4 >= 65536) return RS_ERR_ARG;
     J.W = rec_words(J.Q, J.T, nrows);
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + 2 * J.T;

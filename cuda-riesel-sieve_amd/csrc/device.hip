@@ -140,7 +140,11 @@ extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table
     return RS_OK;
 }
 
-// One persistent launch over count primes (offsets in device memory).
+extern "C" const char* rs_sieve_kernel_name(const rs_sieve* s) {
+    if (!s) return "";
+    return s->L.row_index ? "rs_sieve_kernel5_rix" : s->L.w5 ? "rs_sieve_kernel5" : "rs_sieve_kernel";
+}
+
 // One persistent launch of chunk c0 (ngroups groups of 64 primes) between
 // two timing events; primes: the count credited to the timing (0 when the
 // device counts them).
@@ -327,6 +331,48 @@ extern "C" rs_status rs_sieve_timing(rs_sieve* s, double* kernel_ms, int64_t* pr
 
 extern "C" void rs_sieve_destroy(rs_sieve* s) { free_sieve(s); }
 
+// ------------------------------------------------------------ row masks
+
+extern "C" __global__ void rs_test_row_masks(const JobDev*, const uint64_t*, uint64_t*,
+                                             uint64_t*, long long);
+
+extern "C" rs_status rs_dev_row_masks(int device, const rs_plan* pl, const uint64_t* p,
+                                      int64_t count, uint64_t* masks) {
+    if (!pl || count < 0 || (count > 0 && (!p || !masks))) return RS_ERR_ARG;
+    for (int64_t i = 0; i < count; ++i)
+        if (p[i] % 2 == 0 || p[i] < 3 || p[i] >= (1ull << 63)) return RS_ERR_ARG;
+    if (count == 0) return RS_OK;
+    HIPCHK(hipSetDevice(device));
+    // the masks do not depend on m: any table size within the shape limits
+    int m = rs_plan_auto_m(pl);
+    for (int mm = 64; m < 64 && mm <= (1 << 16); mm += 64)
+        if (rsh::shape_ok(pl, mm, 2)) m = mm;
+    if (m < 64) return RS_ERR_ARG;
+    DevJob dj;
+    rs_status st = make_job(pl, m, 0, false, &dj);
+    const int nw = (dj.J.nrows + 63) / 64;
+    uint64_t *dp = nullptr, *drec = nullptr, *dout = nullptr;
+    if (st == RS_OK &&
+        (hipMalloc(&dp, (size_t)count * 8) != hipSuccess ||
+         hipMalloc(&drec, (size_t)count * dj.J.W * 8) != hipSuccess ||
+         hipMalloc(&dout, (size_t)count * nw * 8) != hipSuccess))
+        st = RS_ERR_NOMEM;
+    if (st == RS_OK && hipMemcpy(dp, p, (size_t)count * 8, hipMemcpyHostToDevice) != hipSuccess)
+        st = RS_ERR_HIP;
+    if (st == RS_OK) {
+        hipLaunchKernelGGL(rs_test_row_masks, dim3((unsigned)((count + 63) / 64)), dim3(64), 0,
+                           0, dj.d_self, dp, drec, dout, (long long)count);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpy(masks, dout, (size_t)count * nw * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            st = RS_ERR_HIP;
+    }
+    (void)hipFree(dp);
+    (void)hipFree(drec);
+    (void)hipFree(dout);
+    free_job(dj);
+    return st;
+}
+
 // ------------------------------------------------------------ autotuner
 
 namespace {
@@ -346,8 +392,12 @@ struct Tuner {
     int best_m = 0, best_tile = 0;
 
     // Mean kernel ns per prime of (m, tile) on the sample; RS_ERR_ARG when
-    // the layout does not fit one wavefront's LDS limit.
+    // the layout does not fit one wavefront's LDS limit (nor will a larger
+    // m), kSkip when the shape is outside the kernel's index limits (m too
+    // small for the x-window: a larger m may fit).
+    static const rs_status kSkip = 1;
     rs_status time(int m, int tile, double* ns) {
+        if (!rsh::shape_ok(pl, m, std::max(1, tile & ~RS_TILE_ROW_INDEX))) return kSkip;
         rs_sieve* s = nullptr;
         rs_status st = rs_sieve_create(device, pl, m, tile, 1, 1 << 16, &s);
         if (st != RS_OK) return st;
@@ -412,22 +462,21 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
         }
         const int rix = pl->rows_k.size() <= (size_t)ROW_SLICE ? RS_TILE_ROW_INDEX : 0;
         for (int m : ms) {
-            bool fits = true;
+            // an m none of whose layouts fits LDS ends the grid (larger ones
+            // do not fit either); shapes outside the index limits are skipped
+            bool any_fits = false;
             const int tiles[6] = {1, 2, 4, 1 | rix, 2 | rix, 4 | rix};
             for (int i = 0; i < (rix ? 6 : 3); ++i) {
-                const int tile = tiles[i];
-                rs_status st = tu.time(m, tile, &ns);
-                if (st == RS_ERR_ARG) {  // larger m will not fit either
-                    fits = false;
-                    break;
-                }
-                if (st != RS_OK) return st;
+                rs_status st = tu.time(m, tiles[i], &ns);
+                if (st == RS_OK || st == Tuner::kSkip) any_fits = true;
+                else if (st != RS_ERR_ARG) return st;
             }
-            if (!fits) break;
+            if (!any_fits) break;
         }
     } else {
         // coordinate descent: the tile at m0 (or the largest m that fits),
         // then m in steps of m0/4 up and down while the time improves
+        if (m0 < 64) return RS_ERR_ARG;  // no shape fits the kernel's limits
         int m = m0;
         rs_status st = RS_ERR_ARG;
         while (m >= 64) {
@@ -435,16 +484,17 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
             if (st != RS_ERR_ARG) break;
             m = m > 64 ? round64(m * 3 / 4) : 0;
         }
-        if (st != RS_OK) return st;
+        if (st != RS_OK) return st == Tuner::kSkip ? RS_ERR_ARG : st;
         for (int tile : {1, 4})
-            if ((st = tu.time(m, tile, &ns)) != RS_OK && st != RS_ERR_ARG) return st;
+            if ((st = tu.time(m, tile, &ns)) != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip)
+                return st;
         const int tile = tu.best_tile;
         const int step = std::max(64, round64(m / 4));
         for (int dir : {+1, -1}) {
             double prev = tu.best;
             for (int mm = m + dir * step; mm >= 64; mm += dir * step) {
                 st = tu.time(mm, tile, &ns);
-                if (st == RS_ERR_ARG) break;
+                if (st == RS_ERR_ARG || st == Tuner::kSkip) break;
                 if (st != RS_OK) return st;
                 if (ns >= prev) break;
                 prev = ns;
@@ -454,7 +504,7 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
         // the row-index layout at the best pair
         if (pl->rows_k.size() <= (size_t)ROW_SLICE) {
             st = tu.time(tu.best_m, tu.best_tile | RS_TILE_ROW_INDEX, &ns);
-            if (st != RS_OK && st != RS_ERR_ARG) return st;
+            if (st != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip) return st;
         }
     }
     if (ncand) *ncand = tu.n;

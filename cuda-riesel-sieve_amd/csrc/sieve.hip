@@ -891,6 +891,23 @@ extern "C" __global__ void rs_test_halfmul(const uint32_t* k, const uint64_t* y,
     out[i] = halfmul(k[i], y[i], p[i], lo32(mont_mprime(p[i])));
 }
 
+// The set-up phase's row masks for primes p[i] (one lane per prime, the
+// sieve's own prep_record on the job's constants): out[i * NW + w] = mask
+// word w of p[i]'s record, NW = ceil(rows / 64); rec holds W words of
+// scratch per prime.
+extern "C" __global__ void rs_test_row_masks(const JobDev* __restrict__ jobs,
+                                             const uint64_t* __restrict__ p,
+                                             uint64_t* __restrict__ rec,
+                                             uint64_t* __restrict__ out, long long n) {
+    const JobDev J = uniform_load(jobs);  // before any lane leaves
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64* const R = rec + i * J.W;
+    prep_record(J, p[i], R);
+    const int nw = (J.nrows + 63) >> 6;
+    for (int w = 0; w < nw; ++w) out[i * nw + w] = R[J.off_j + w];
+}
+
 extern "C" __global__ void rs_test_jacobi(const uint32_t* a, const uint64_t* p, int32_t* out,
                                           long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

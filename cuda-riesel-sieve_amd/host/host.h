@@ -54,4 +54,14 @@ struct SplitMix64 {
 
 int hw_threads(int requested);
 
+// Job-shape limits of the device kernels (the giant-step grid of a row
+// slice is indexed in 16 bits, and a row's Y offset 8 r ys in 19 bits).
+// giant_T is the giant steps per row for table size m and tile tw (1, 2 or
+// 4): ceil(X/m) for the x-window X, rounded up to a multiple of tw, at
+// least 2.  shape_ok is true iff a job of this plan with (m, tw) is within
+// the limits: m a multiple of 64 in [64, 65536], T <= 65535, Q ys < 65536
+// (ys = T, or T + 1 when T is even), min(rows, 512) T + 4096 < 65536.
+int giant_T(const rs_plan* pl, int m, int tw);
+bool shape_ok(const rs_plan* pl, int m, int tw);
+
 }  // namespace rsh

@@ -2,7 +2,9 @@
 // program's main (kernel_v0.24.6.cu:688-1807) over librsieve.
 //
 // Flags as at :705-786: -i FILE (ABCD input, default sr_745.abcd), -m M
-// (baby steps m = 2 << M, :931; without -m the cost model picks m), -p P0,
+// (baby steps m = 2 << M, :931; without -m, or with M <= 1, the cost model
+// picks m, where the reference sizes its tables from the free memory,
+// :917-929; M = 2..4 runs m = 64, this build's smallest table), -p P0,
 // -P P1, -Q Q (even), -O FILE (factor file, default factors.txt), -b/-t
 // SCALE (primes per launch = 128*2^(b-1) * 128*2^(t-1), :817-818, :849-851),
 // -s SCALE (at least 4096*s sieving primes, :1274), -v (no verification:
@@ -34,7 +36,7 @@ static void usage() {
     printf(
         "\nrsieve -- a BSGS sieve for multiple sequences of the form k*b^n-1 (MI355X)\n\n"
         "-i FILE  : Read in abcd sieve file called FILE.\n"
-        "-m m     : Use 2<<m baby steps (hash table elements); default: cost model.\n"
+        "-m m     : Use 2<<m baby steps (hash table elements); m <= 1 or no -m: cost model.\n"
         "-p P0    : Start sieving from P0. Must have a corresponding P1.\n"
         "-P P1    : Finish sieving at P1. Must have a corresponding P0.\n"
         "                If no -p and -P arguments then we default to 102297149770000 for testing.\n"
@@ -96,8 +98,9 @@ int main(int argc, char* argv[]) {
         printf("Bad input parameter : 1 <= BlockScale, ThreadScale <= 31.\n");
         return 1;
     }
-    if (hashFlag < 0 || hashFlag > 14) {
-        printf("Bad input parameter : -m must be in [0, 14] (m = 2 << M).\n");
+    // :917-931: M <= 1 sizes the table automatically, M >= 2 takes m = 2 << M
+    if (hashFlag < 0 || hashFlag > 15) {
+        printf("Bad input parameter : -m must be in [0, 15] (m = 2 << M; M <= 1: automatic).\n");
         return 1;
     }
     if (smallPrimeScale < 1 || smallPrimeScale > 4096) {
@@ -142,10 +145,13 @@ int main(int argc, char* argv[]) {
     if ((st = rs_plan_create(abcd, Qin, &plan)) != RS_OK) return fail("planning", st);
     int Q, nrows, words;
     rs_plan_info(plan, nullptr, &Q, &nrows, &words, nullptr, nullptr);
-    const int m = hashFlag ? (2 << hashFlag) : 0;
+    // The factor set does not depend on m, so the reference's small tables
+    // (-m 2..4: m = 8..32) run with this build's smallest, m = 64.
+    int m = hashFlag <= 1 ? 0 : (2 << hashFlag);
     if (m != 0 && m < 64) {
-        printf("Bad input parameter : this build needs m >= 64 baby steps (-m 5 or more).\n");
-        return 1;
+        printf("Note: -m %d asks for %d baby steps; this build's smallest table, 64, is used "
+               "(the factors found are the same).\n", hashFlag, m);
+        m = 64;
     }
     printf("Q=%d. Subsequences= %d. Range = %d\n", Q, nrows, maxN / Q - minN / Q + 1);
     if (low < 3) low = 3;

@@ -133,6 +133,20 @@ extern "C" rs_status rs_plan_tminmax(const rs_plan* pl, int m, int* tmin, int* t
     return RS_OK;
 }
 
+int rsh::giant_T(const rs_plan* pl, int m, int tw) {
+    if (tw < 1) tw = 1;
+    const int X = pl->x1 - pl->x0 + 1;
+    return std::max(2, ((X + m - 1) / m + tw - 1) / tw * tw);
+}
+
+bool rsh::shape_ok(const rs_plan* pl, int m, int tw) {
+    if (!pl || m < 64 || m % 64 != 0 || m > (1 << 16) || pl->rows_k.empty()) return false;
+    const long long T = giant_T(pl, m, tw);
+    const long long ys = T + (T % 2 == 0 ? 1 : 0);
+    const long long rows = std::min<long long>((long long)pl->rows_k.size(), 512);
+    return T <= 65535 && pl->Q * ys < 65536 && rows * T + 4096 < 65536;
+}
+
 // Cost model for the table size m (multiple of 64): per prime the kernel
 // does m baby steps and Q*T Y-table products (full 64x64 Montgomery, ~11
 // 32-bit multiplies each), A*T giant steps (32x64 half products, ~5
@@ -140,14 +154,16 @@ extern "C" rs_status rs_plan_tminmax(const rs_plan* pl, int m, int* tmin, int* t
 // baby step and one probe per giant step; T = ceil(X/m) with X the x-window.
 // The table (2 slots per baby step, rounded up to a power of two, 4 bytes
 // each) and the Y table live in LDS, so m is capped to keep a wave's share
-// under 32 KiB.
+// under 32 KiB.  Only shapes within the kernels' limits (shape_ok at the
+// default tile 2) are considered; 0 when none is.
 extern "C" int rs_plan_auto_m(const rs_plan* pl) {
     if (!pl) return 0;
     const double X = pl->x1 - pl->x0 + 1;
     const double A = 0.5 * pl->rows_k.size();
-    int best = 64;
+    int best = 0;
     double bestCost = 1e300;
     for (int m = 64; m <= 16384; m += 64) {
+        if (!rsh::shape_ok(pl, m, 2)) continue;
         const int T = (int)std::ceil(X / m);
         int M = 1;
         while (M < 2 * m) M <<= 1;

@@ -1,7 +1,10 @@
 // util.cpp -- host arithmetic helpers, factor verification and output
 // formatting.
+#include <sched.h>
+
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <thread>
 
 #include "host.h"
@@ -58,9 +61,18 @@ std::vector<uint32_t> primes_upto(uint32_t n) {
     return out;
 }
 
+// requested > 0, else the CPUs this process may run on (its affinity set),
+// at most OMP_NUM_THREADS when that is set (a host's share of its cores),
+// at most 64.
 int hw_threads(int requested) {
     if (requested > 0) return requested;
     int n = (int)std::thread::hardware_concurrency();
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof set, &set) == 0) n = CPU_COUNT(&set);
+    if (const char* e = getenv("OMP_NUM_THREADS")) {
+        const int v = atoi(e);
+        if (v > 0) n = std::min(n, v);
+    }
     return std::max(1, std::min(n, 64));
 }
 

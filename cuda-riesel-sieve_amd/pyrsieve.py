@@ -70,6 +70,7 @@ _SIGS = {
     "rs_format_factor": (i32, [P(Factor), u32, i32, ctypes.c_char_p, i32]),
     "rs_sieve_create": (i32, [i32, vp, i32, i32, i64, i64, P(vp)]),
     "rs_sieve_info": (i32, [vp, P(i32), P(i32), P(i32), P(i32), P(i32)]),
+    "rs_sieve_kernel_name": (ctypes.c_char_p, [vp]),
     "rs_sieve_submit": (i32, [vp, u64, P(u32), i64]),
     "rs_sieve_submit_device": (i32, [vp, u64, vp, i64]),
     "rs_sieve_submit_range": (i32, [vp, u64, u64]),
@@ -99,6 +100,7 @@ _SIGS = {
     "rs_dev_montmul": (i32, [i32, P(u64), P(u64), P(u64), P(u64), i64]),
     "rs_dev_halfmul": (i32, [i32, P(u32), P(u64), P(u64), P(u64), i64]),
     "rs_dev_jacobi": (i32, [i32, P(u32), P(u64), P(i32), i64]),
+    "rs_dev_row_masks": (i32, [i32, vp, P(u64), i64, P(u64)]),
     "rs_dev_alloc": (i32, [i32, i64, P(vp)]),
     "rs_dev_upload": (i32, [i32, vp, vp, i64]),
     "rs_dev_download": (i32, [i32, vp, vp, i64]),
@@ -487,6 +489,7 @@ class Sieve:
                                                   ctypes.byref(tw)))
         self.m, self.T, self.slots, self.lds_bytes = mm.value, T.value, slots.value, lds.value
         self.tile = tw.value
+        self.kernel = lib().rs_sieve_kernel_name(self.h).decode()
 
     def submit(self, low, offsets):
         offs = np.ascontiguousarray(offsets, np.uint32)
@@ -640,6 +643,17 @@ def dev_jacobi(a, p, device=0):
     out = np.zeros(len(a), np.int32)
     _chk("rs_dev_jacobi", lib().rs_dev_jacobi(device, _p(a, u32), _p(p, u64), _p(out, i32),
                                               len(a)))
+    return out
+
+
+def dev_row_masks(plan, primes, device=0):
+    """The sieve kernel's own row filter for each prime: a (len(primes), NW)
+    uint64 array, bit j of word w set iff row 64 w + j passes."""
+    p = np.ascontiguousarray(primes, np.uint64)
+    nw = (plan.nrows + 63) // 64
+    out = np.zeros((len(p), nw), np.uint64)
+    _chk("rs_dev_row_masks", lib().rs_dev_row_masks(device, plan.h, _p(p, u64), len(p),
+                                                    _p(out, u64)))
     return out
 
 

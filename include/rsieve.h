@@ -119,7 +119,8 @@ rs_status rs_plan_rows(const rs_plan* pl, int32_t* rows_k, int32_t* rows_r,
 int rs_plan_contains(const rs_plan* pl, uint32_t k, uint32_t n);
 /* tMin/tMax of :1148-1158 for m = 2^s (reported for the -m table). */
 rs_status rs_plan_tminmax(const rs_plan* pl, int m, int* tmin, int* tmax);
-/* Table size m picked by the cost model for this plan (multiple of 64). */
+/* Table size m picked by the cost model for this plan (a multiple of 64
+ * within the job-shape limits below), or 0 when no m is within them. */
 int rs_plan_auto_m(const rs_plan* pl);
 void rs_plan_free(rs_plan* pl);
 
@@ -185,12 +186,29 @@ int rs_format_factor(const rs_factor* f, uint32_t b, int c, char* buf, int cap);
  * slot per prime, :615-621). */
 typedef struct rs_sieve rs_sieve;
 #define RS_TILE_ROW_INDEX 16
+/* Job-shape limits (rs_sieve_create, rs_batch_create, rs_autotune and the
+ * server return RS_ERR_ARG outside them; rs_plan_auto_m only picks shapes
+ * inside them): with X = floor(maxN/Q) - floor(minN/Q) + 1 and T the giant
+ * steps per row, ceil(X/m) rounded up to a multiple of the tile and at
+ * least 2,
+ *   m a multiple of 64 in [64, 65536],
+ *   T <= 65535 and Q * ys < 65536, ys = T (T odd) or T + 1 (T even),
+ *   min(rows, 512) * T + 4096 < 65536 (a row slice's giant steps are
+ *     numbered in 16 bits),
+ *   at most 65535 distinct k,
+ * and the layout of one wavefront (table, Y table, row list, record; see
+ * rs_sieve_info) must fit the device's LDS.  A small m raises T: a wide
+ * n window with many rows needs a larger m. */
 
 rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int tile,
                           int64_t max_batch, int64_t hit_capacity,
                           rs_sieve** out);
 rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
                         int* lds_bytes_per_wave, int* tile);
+/* Name of the kernel build the job's launches run (static string; "" for
+ * NULL): rs_sieve_kernel, rs_sieve_kernel5 (layouts small enough for five
+ * wavefronts per SIMD) or rs_sieve_kernel5_rix (RS_TILE_ROW_INDEX). */
+const char* rs_sieve_kernel_name(const rs_sieve* s);
 /* Primes p = low + offsets[i] from HOST memory (p odd, 3 <= p < 2^63, p ∤ b).
  * Copied through pinned double buffers on a copy stream, then sieved on the
  * compute stream; returns once the copy of this call's data is issued.
@@ -337,6 +355,15 @@ rs_status rs_dev_halfmul(int device, const uint32_t* k, const uint64_t* y,
 /* out[i] = Jacobi (a[i] / p[i]) in {-1, 0, 1} computed on the device. */
 rs_status rs_dev_jacobi(int device, const uint32_t* a, const uint64_t* p,
                         int32_t* out, int64_t count);
+/* The sieve kernel's own row filter (the Legendre test of :491-521, as the
+ * set-up phase computes it: per-k Jacobi constants, quadratic-character
+ * tables, (2/p)^z, reciprocity and, for k*b^n+1, (-1/p)) for the plan's
+ * rows and odd primes 3 <= p[i] < 2^63: bit j of masks[i * NW + w],
+ * NW = ceil(rows / 64), is set iff row 64 w + j, (k, r), can hold a factor
+ * of p[i], i.e. c k b^r is a quadratic residue mod p[i] (c = 1 for
+ * k*b^n-1, -1 for k*b^n+1).  Host buffers; for tests. */
+rs_status rs_dev_row_masks(int device, const rs_plan* pl, const uint64_t* p,
+                           int64_t count, uint64_t* masks);
 
 /* Device buffers for HBM-resident inputs (plain hipMalloc/hipMemcpy; no
  * reference counterpart beyond the cudaMalloc/cudaMemcpy of :1387-1512). */

@@ -66,6 +66,8 @@ def test_cli_help_and_bad_flags():
     r = subprocess.run([exe, "-p", "100", "-P", "50"], capture_output=True, text=True,
                        timeout=30)
     assert r.returncode == 1 and "P1 < P0" in r.stdout
+    r = subprocess.run([exe, "-m", "16"], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 1 and "-m must be in [0, 15]" in r.stdout
     r = subprocess.run([exe, "-i", "/nonexistent.abcd", "-p", "1000", "-P", "2000"],
                        capture_output=True, text=True, timeout=30)
     assert r.returncode == 1 and "Unable to open file" in r.stdout

@@ -590,6 +590,24 @@ def test_cli_default_range_known_answer(tmp_path, gpu):
     assert line not in r.stdout
 
 
+def test_cli_readme_small_m_lines(tmp_path, gpu):
+    """The README's lines with -m 4 (m = 32, below this build's smallest
+    table) and -m 1 (automatic size) run, print the same factors as -m 8 and
+    find the reference's debug prime on the default range."""
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+    outs = {}
+    for mflag in ("8", "4", "1"):
+        out = tmp_path / f"f{mflag}.txt"
+        r = subprocess.run([exe, "-i", R745, "-b", "5", "-m", mflag, "-s", "256", "-Q", "18",
+                            "-v", "-O", str(out)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, (mflag, r.stdout[-2000:], r.stderr)
+        outs[mflag] = sorted(out.read_text().splitlines())
+        if mflag == "4":
+            assert "smallest table, 64, is used" in r.stdout
+    assert f"{KNOWN_P} | 18300*745^242670-1" in outs["8"]
+    assert outs["4"] == outs["8"] == outs["1"]
+
+
 def test_factor_removal_after_sieving(rs, orc, tmp_path, gpu):
     """Dat mode: verified factors' candidates leave the ABCD; sieving the
     rewritten file over the same primes finds none of its candidates, and
