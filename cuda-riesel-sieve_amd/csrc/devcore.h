@@ -18,6 +18,9 @@
 extern "C" __global__ void rs_sieve_kernel(RS_SIEVE_KERNEL_ARGS);
 extern "C" __global__ void rs_sieve_kernel5(RS_SIEVE_KERNEL_ARGS);
 extern "C" __global__ void rs_sieve_kernel5_rix(RS_SIEVE_KERNEL_ARGS);
+// one shared table per workgroup of 4 or 8 wavefronts (single-chunk launches)
+extern "C" __global__ void rs_sieve_group4_kernel(RS_SIEVE_KERNEL_ARGS);
+extern "C" __global__ void rs_sieve_group8_kernel(RS_SIEVE_KERNEL_ARGS);
 extern "C" __global__ void rs_zero_u32(uint32_t*, int);
 extern "C" __global__ void rs_jacobi_table(uint32_t n, uint32_t words, uint32_t* out);
 
@@ -50,8 +53,10 @@ struct Launcher {
     int shared_words = 0;    // block-shared words after the wavefront regions
     bool w5 = false;         // rs_sieve_kernel5 (row_index: rs_sieve_kernel5_rix)
     bool row_index = false;
+    int nwv = 0;             // > 0: rs_sieve_group<nwv>_kernel, one table per block
     size_t lds_bytes = 0;
-    int scratch_words = 0;  // per wavefront: 64 records
+    int scratch_words = 0;  // per wavefront: 64 records (nwv > 0: per block,
+                            // scratch_words_group)
     uint64_t* scratch = nullptr;
     unsigned long long* work = nullptr;
     HitDev* hits = nullptr;
@@ -68,6 +73,8 @@ struct MultiLauncher {
     int w_cap = 0;                // largest record (words) the scratch holds
     size_t max_waves = 0;         // resident waves of the smallest layout
     std::map<int, Launcher> geo;  // per-wave LDS words -> grid shape
+    uint64_t* gscratch = nullptr; // scratch of shared-table launches (grown on demand)
+    size_t gscratch_words = 0;
 };
 
 struct ClassRange {
@@ -101,6 +108,16 @@ rs_status init_launcher(Launcher* L, int device, int lds_words, int tab_words, i
 rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_words = 0,
                           bool row_index = false);
 rs_status init_multi(MultiLauncher* M, int device, int w_cap, int64_t hit_capacity);
+// Grid shape of a shared-table job (J.nwv > 0): one workgroup of nwv
+// wavefronts per prime, lds_words_group words of LDS per workgroup.
+rs_status launch_geometry_group(int device, const JobDev& J, Launcher* L);
+// The same, allocating its scratch, work counter and hit buffer.
+rs_status init_launcher_group(Launcher* L, int device, const JobDev& J, int64_t hit_capacity);
+// Launch groups of count primes: 64 per group, 64 nwv for shared tables.
+inline long long groups_of(const Launcher& L, long long count) {
+    const long long g = 64ll * (L.nwv > 0 ? L.nwv : 1);
+    return (count + g - 1) / g;
+}
 // Orders `chunks` (each with .job indexing `jobs`) by occupancy class,
 // numbering groups from 0 within each class, and returns one range per
 // non-empty class.

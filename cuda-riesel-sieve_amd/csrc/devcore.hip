@@ -96,8 +96,12 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     if (!pl) return RS_ERR_ARG;
     if (m <= 0) m = rs_plan_auto_m(pl);
     const bool row_index = (tw & RS_TILE_ROW_INDEX) != 0;
-    tw &= ~RS_TILE_ROW_INDEX;
-    if (tw <= 0) tw = 2;
+    const int nwv = (tw & RS_TILE_GROUP8) ? 8 : (tw & RS_TILE_GROUP4) ? 4 : 0;
+    if ((tw & RS_TILE_GROUP8) && (tw & RS_TILE_GROUP4)) return RS_ERR_ARG;
+    if (nwv && (row_index || m % (64 * nwv) != 0)) return RS_ERR_ARG;
+    tw &= ~(RS_TILE_ROW_INDEX | RS_TILE_GROUP4 | RS_TILE_GROUP8);
+    if (tw < 0) return RS_ERR_ARG;
+    if (tw == 0) tw = 2;
     if (tw > 4) return RS_ERR_ARG;
     if (tw == 3) tw = 2;
     if (pl->Q < 2 || pl->Q % 2 != 0 || pl->minN < 1 || pl->rows_k.empty()) return RS_ERR_ARG;
@@ -139,11 +143,14 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.tw = tw;
     J.kmax = kvals.back();
     J.plus = pl->c == 1;
-    J.W = rec_words(J.Q, J.T, nrows);
+    J.nwv = nwv;
+    J.W = rec_words(J.Q, J.T, nrows, nwv);
     J.off_gt = REC_BH + J.Q;
     J.off_j = J.off_gt + 2 * J.T;
+    J.off_x = J.off_j + (nrows + 63) / 64;
     J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.ys, nrows, J.W, false);
     J.lds_words_idx = lds_words_per_wave(J.logNB, J.Q, J.ys, nrows, J.W, true);
+    J.lds_words_wg = nwv ? lds_words_group(J.logNB, J.Q, J.ys, nrows, J.W) : 0;
     J.list_idx = row_index && nrows <= ROW_SLICE ? 1 : 0;
     J.tab_words = 4 << J.logNB;
     // per k: the 64-row word of its first row and its even-r / odd-r rows
@@ -289,6 +296,48 @@ rs_status launch_geometry(int device, int lds_words, Launcher* L, int shared_wor
     return RS_OK;
 }
 
+static const void* group_kernel(int nwv) {
+    return nwv == 8 ? reinterpret_cast<const void*>(rs_sieve_group8_kernel)
+                    : reinterpret_cast<const void*>(rs_sieve_group4_kernel);
+}
+
+rs_status launch_geometry_group(int device, const JobDev& J, Launcher* L) {
+    if (J.nwv != 4 && J.nwv != 8) return RS_ERR_ARG;
+    L->device = device;
+    int max_lds = 0, cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const size_t bytes = (size_t)J.lds_words_wg * 4;
+    if (bytes > (size_t)max_lds) return RS_ERR_ARG;
+    const void* k = group_kernel(J.nwv);
+    hipFuncAttributes fa{};
+    HIPCHK(hipFuncGetAttributes(&fa, k));
+    if (fa.sharedSizeBytes != 0) return RS_ERR_HIP;  // the table sits at LDS address 0
+    HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64 * J.nwv, bytes));
+    if (per_cu < 1) return RS_ERR_ARG;
+    L->nwv = J.nwv;
+    L->wpb = J.nwv;
+    L->w5 = false;
+    L->row_index = false;
+    L->lds_words = J.lds_words_wg;
+    L->tab_words = J.tab_words;
+    L->shared_words = 0;
+    L->lds_bytes = bytes;
+    L->grid = cus * per_cu;
+    L->scratch_words = (int)scratch_words_group(J.nwv, J.W, J.m);
+    return RS_OK;
+}
+
+static rs_status alloc_buffers(Launcher* L, size_t scratch_words_total, int64_t hit_capacity);
+
+rs_status init_launcher_group(Launcher* L, int device, const JobDev& J, int64_t hit_capacity) {
+    rs_status st = launch_geometry_group(device, J, L);
+    if (st != RS_OK) return st;
+    return alloc_buffers(L, (size_t)L->grid * L->scratch_words, hit_capacity);
+}
+
 static rs_status alloc_buffers(Launcher* L, size_t scratch_words_total, int64_t hit_capacity) {
     if (hipMalloc(&L->scratch, scratch_words_total * 8) != hipSuccess) return RS_ERR_NOMEM;
     HIPCHK(hipMalloc(&L->work, 8));
@@ -330,6 +379,10 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
     // occupancy key of each job's own layout
     std::vector<std::pair<long long, int>> key(jobs.size());
     for (size_t j = 0; j < jobs.size(); ++j) {
+        if (jobs[j].nwv > 0) {  // shared-table jobs run as single launches
+            key[j] = {-1, 0};
+            continue;
+        }
         auto it = M->geo.find(jobs[j].lds_words);
         if (it == M->geo.end()) {
             Launcher g;
@@ -348,7 +401,8 @@ rs_status plan_classes(MultiLauncher* M, const std::vector<JobDev>& jobs,
     // table, the largest rest of the layout and the largest record of its
     // members (their sum may give a grid of fewer waves than the members')
     std::vector<long long> keys;
-    for (auto& k : key) keys.push_back(k.first);
+    for (auto& k : key)
+        if (k.first >= 0) keys.push_back(k.first);
     std::sort(keys.begin(), keys.end(), std::greater<long long>());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     std::vector<ChunkDev> sorted;
@@ -404,9 +458,24 @@ rs_status plan_launches(MultiLauncher* M, const std::vector<JobDev>& jobs,
     singles->clear();
     cls->clear();
     std::vector<ChunkDev> rest;
+    size_t gwords = 0;  // scratch the shared-table launches need
     for (const ChunkDev& c : *chunks) {
         if (c.count == 0) continue;
         const JobDev& J = jobs[c.job];
+        if (J.nwv > 0) {  // shared tables: always a launch of their own
+            SingleLaunch s;
+            rs_status st = launch_geometry_group(M->base.device, J, &s.L);
+            if (st != RS_OK) return st;
+            gwords = std::max(gwords, (size_t)s.L.grid * s.L.scratch_words);
+            s.L.work = M->base.work;
+            s.L.hits = M->base.hits;
+            s.L.hit_count = M->base.hit_count;
+            s.L.hit_cap = M->base.hit_cap;
+            s.c0 = c;
+            s.c0.group0 = 0;
+            singles->push_back(s);
+            continue;
+        }
         const int shared = J.nrows <= ROW_SLICE ? (2 * J.nrows + 3) & ~3 : 0;
         SingleLaunch s;
         const bool rix = shared && J.list_idx;
@@ -434,6 +503,15 @@ rs_status plan_launches(MultiLauncher* M, const std::vector<JobDev>& jobs,
         s.c0.group0 = 0;
         singles->push_back(s);
     }
+    if (gwords > M->gscratch_words) {
+        (void)hipFree(M->gscratch);
+        M->gscratch = nullptr;
+        M->gscratch_words = 0;
+        if (hipMalloc(&M->gscratch, gwords * 8) != hipSuccess) return RS_ERR_NOMEM;
+        M->gscratch_words = gwords;
+    }
+    for (SingleLaunch& s : *singles)
+        if (s.L.nwv > 0) s.L.scratch = M->gscratch;
     *chunks = std::move(rest);
     return chunks->empty() ? RS_OK : plan_classes(M, jobs, chunks, cls);
 }
@@ -441,7 +519,7 @@ rs_status plan_launches(MultiLauncher* M, const std::vector<JobDev>& jobs,
 rs_status launch_singles(const std::vector<SingleLaunch>& singles, hipStream_t stream,
                          const JobDev* jobs) {
     for (const SingleLaunch& s : singles) {
-        rs_status st = launch(s.L, stream, jobs, s.c0, nullptr, 1, (s.c0.count + 63) / 64);
+        rs_status st = launch(s.L, stream, jobs, s.c0, nullptr, 1, groups_of(s.L, s.c0.count));
         if (st != RS_OK) return st;
     }
     return RS_OK;
@@ -460,6 +538,9 @@ rs_status launch_classes(const std::vector<ClassRange>& cls, hipStream_t stream,
 
 void free_multi(MultiLauncher& M) {
     free_launcher(M.base);
+    (void)hipFree(M.gscratch);
+    M.gscratch = nullptr;
+    M.gscratch_words = 0;
     M.geo.clear();
 }
 
@@ -467,10 +548,15 @@ rs_status launch(const Launcher& L, hipStream_t stream, const JobDev* jobs, cons
                  const ChunkDev* chunks, int nchunks, long long ngroups) {
     hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, (uint32_t*)L.work, 2);
     HIPCHK(hipGetLastError());
-    const int grid = (int)std::min<long long>(L.grid, (ngroups + L.wpb - 1) / L.wpb);
-    hipLaunchKernelGGL(L.row_index ? rs_sieve_kernel5_rix
-                       : L.w5      ? rs_sieve_kernel5
-                                   : rs_sieve_kernel,
+    // a block takes groups of 64 primes per wavefront (shared tables: of
+    // 64 nwv primes per block)
+    const int grid = (int)std::min<long long>(
+        L.grid, L.nwv > 0 ? ngroups : (ngroups + L.wpb - 1) / L.wpb);
+    hipLaunchKernelGGL(L.nwv == 8    ? rs_sieve_group8_kernel
+                       : L.nwv == 4  ? rs_sieve_group4_kernel
+                       : L.row_index ? rs_sieve_kernel5_rix
+                       : L.w5        ? rs_sieve_kernel5
+                                     : rs_sieve_kernel,
                        dim3(std::max(1, grid)),
                        dim3(64 * L.wpb), L.lds_bytes, stream, c0, jobs, chunks, nchunks, ngroups,
                        L.lds_words, L.tab_words, L.shared_words,

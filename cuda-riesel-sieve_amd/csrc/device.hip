@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -107,7 +108,9 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int t
     // the row table goes to block-shared LDS when it fits one row slice
     const int nrows = s->job.J.nrows;
     const int shared = nrows <= ROW_SLICE ? (2 * nrows + 3) & ~3 : 0;
-    if (st == RS_OK)
+    if (st == RS_OK && s->job.J.nwv > 0)
+        st = init_launcher_group(&s->L, device, s->job.J, hit_capacity);
+    else if (st == RS_OK)
         st = init_launcher(&s->L, device,
                            shared && s->job.J.list_idx ? s->job.J.lds_words_idx
                                                        : s->job.J.lds_words,
@@ -132,7 +135,9 @@ extern "C" rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int t
 extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
                                    int* lds_bytes_per_wave, int* tile) {
     if (!s) return RS_ERR_ARG;
-    if (tile) *tile = s->job.J.tw | (s->job.J.list_idx ? RS_TILE_ROW_INDEX : 0);
+    if (tile)
+        *tile = s->job.J.tw | (s->job.J.list_idx ? RS_TILE_ROW_INDEX : 0) |
+                (s->job.J.nwv == 4 ? RS_TILE_GROUP4 : s->job.J.nwv == 8 ? RS_TILE_GROUP8 : 0);
     if (m) *m = s->job.J.m;
     if (T) *T = s->job.J.T;
     if (table_slots) *table_slots = 8 << s->job.J.logNB;
@@ -142,6 +147,8 @@ extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table
 
 extern "C" const char* rs_sieve_kernel_name(const rs_sieve* s) {
     if (!s) return "";
+    if (s->L.nwv == 8) return "rs_sieve_group8_kernel";
+    if (s->L.nwv == 4) return "rs_sieve_group4_kernel";
     return s->L.row_index ? "rs_sieve_kernel5_rix" : s->L.w5 ? "rs_sieve_kernel5" : "rs_sieve_kernel";
 }
 
@@ -177,7 +184,7 @@ static rs_status sieve_launch(rs_sieve* s, uint64_t low, const uint32_t* d_offs,
     c0.count = count;
     c0.group0 = 0;
     c0.job = 0;
-    return timed_launch(s, c0, (count + 63) / 64, count);
+    return timed_launch(s, c0, groups_of(s->L, count), count);
 }
 
 extern "C" rs_status rs_sieve_submit_device(rs_sieve* s, uint64_t low,
@@ -262,7 +269,7 @@ extern "C" rs_status rs_sieve_submit_range(rs_sieve* s, uint64_t lo, uint64_t hi
         c0.count = s->range_cap;
         c0.count_dev = cnt_dev;
         // the primes are counted on the device, see d_range_primes
-        st = timed_launch(s, c0, (s->range_cap + 63) / 64, 0);
+        st = timed_launch(s, c0, groups_of(s->L, s->range_cap), 0);
         if (st != RS_OK) return st;
         hipLaunchKernelGGL(rs_accum_count, dim3(1), dim3(64), 0, s->stream, cnt_dev,
                            s->d_range_primes);
@@ -401,7 +408,7 @@ struct Tuner {
         rs_sieve* s = nullptr;
         rs_status st = rs_sieve_create(device, pl, m, tile, 1, 1 << 16, &s);
         if (st != RS_OK) return st;
-        if (s->job.J.lds_words * 4 > max_lds) {
+        if (s->job.J.nwv == 0 && s->job.J.lds_words * 4 > max_lds) {
             rs_sieve_destroy(s);
             return RS_ERR_ARG;
         }
@@ -433,6 +440,62 @@ struct Tuner {
 };
 
 int round64(int m) { return std::max(64, (m + 32) / 64 * 64); }
+
+// Shared-table layouts are worth timing when the balance of baby and giant
+// steps, m ~ sqrt(A X / 2) (A = rows/2 Jacobi survivors, X the x-window,
+// a giant step costing about half a baby step), lies well above the best
+// table one wavefront could hold.
+bool groups_worth(const rs_plan* pl, int best_m) {
+    const double A = 0.5 * (double)pl->rows_k.size(), X = pl->x1 - pl->x0 + 1;
+    const double mopt = std::sqrt(A * X * 0.5);
+    return mopt >= 2048 && mopt > 1.5 * best_m;
+}
+
+// Times shared-table layouts (4 or 8 wavefronts per prime): effort 0 the
+// grid of m in {2048 .. 16384} x both group sizes x tiles 2 and 4 (up to the
+// first m whose layouts do not fit LDS); effort 1 m upwards with 8
+// wavefronts and tile 4 while the time improves, then 4 wavefronts and tile
+// 2 at the best m.
+rs_status tune_groups(Tuner& tu, int effort) {
+    static const int ms[] = {2048, 3072, 4096, 6144, 8192, 12288, 16384};
+    double ns = 0;
+    auto soft = [](rs_status st) {
+        return st == RS_OK || st == RS_ERR_ARG || st == Tuner::kSkip || st == RS_ERR_NOMEM;
+    };
+    if (effort == 0) {
+        for (int m : ms) {
+            bool any_fits = false;
+            for (int g : {RS_TILE_GROUP8, RS_TILE_GROUP4})
+                for (int tile : {4, 2}) {
+                    rs_status st = tu.time(m, tile | g, &ns);
+                    if (st == RS_OK || st == Tuner::kSkip) any_fits = true;
+                    else if (!soft(st)) return st;
+                }
+            if (!any_fits) break;
+        }
+        return RS_OK;
+    }
+    double prev = 1e300;
+    int bm = 0;
+    for (int m : ms) {
+        rs_status st = tu.time(m, 4 | RS_TILE_GROUP8, &ns);
+        if (st == Tuner::kSkip) continue;
+        if (st != RS_OK) {
+            if (!soft(st)) return st;
+            break;  // larger tables do not fit either
+        }
+        if (ns >= prev) break;
+        prev = ns;
+        bm = m;
+    }
+    if (bm) {
+        for (int t : {4 | RS_TILE_GROUP4, 2 | RS_TILE_GROUP8}) {
+            rs_status st = tu.time(bm, t, &ns);
+            if (!soft(st)) return st;
+        }
+    }
+    return RS_OK;
+}
 
 }  // namespace
 
@@ -473,10 +536,9 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
             }
             if (!any_fits) break;
         }
-    } else {
+    } else if (m0 >= 64) {
         // coordinate descent: the tile at m0 (or the largest m that fits),
         // then m in steps of m0/4 up and down while the time improves
-        if (m0 < 64) return RS_ERR_ARG;  // no shape fits the kernel's limits
         int m = m0;
         rs_status st = RS_ERR_ARG;
         while (m >= 64) {
@@ -484,28 +546,34 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
             if (st != RS_ERR_ARG) break;
             m = m > 64 ? round64(m * 3 / 4) : 0;
         }
-        if (st != RS_OK) return st == Tuner::kSkip ? RS_ERR_ARG : st;
-        for (int tile : {1, 4})
-            if ((st = tu.time(m, tile, &ns)) != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip)
-                return st;
-        const int tile = tu.best_tile;
-        const int step = std::max(64, round64(m / 4));
-        for (int dir : {+1, -1}) {
-            double prev = tu.best;
-            for (int mm = m + dir * step; mm >= 64; mm += dir * step) {
-                st = tu.time(mm, tile, &ns);
-                if (st == RS_ERR_ARG || st == Tuner::kSkip) break;
-                if (st != RS_OK) return st;
-                if (ns >= prev) break;
-                prev = ns;
+        if (st != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip) return st;
+        if (st == RS_OK) {
+            for (int tile : {1, 4})
+                if ((st = tu.time(m, tile, &ns)) != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip)
+                    return st;
+            const int tile = tu.best_tile;
+            const int step = std::max(64, round64(m / 4));
+            for (int dir : {+1, -1}) {
+                double prev = tu.best;
+                for (int mm = m + dir * step; mm >= 64; mm += dir * step) {
+                    st = tu.time(mm, tile, &ns);
+                    if (st == RS_ERR_ARG || st == Tuner::kSkip) break;
+                    if (st != RS_OK) return st;
+                    if (ns >= prev) break;
+                    prev = ns;
+                }
+                if (tu.best_m != m) break;  // improved going up: skip going down
             }
-            if (tu.best_m != m) break;  // improved going up: skip going down
+            // the row-index layout at the best pair
+            if (pl->rows_k.size() <= (size_t)ROW_SLICE) {
+                st = tu.time(tu.best_m, tu.best_tile | RS_TILE_ROW_INDEX, &ns);
+                if (st != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip) return st;
+            }
         }
-        // the row-index layout at the best pair
-        if (pl->rows_k.size() <= (size_t)ROW_SLICE) {
-            st = tu.time(tu.best_m, tu.best_tile | RS_TILE_ROW_INDEX, &ns);
-            if (st != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip) return st;
-        }
+    }
+    if (groups_worth(pl, tu.best_m)) {
+        rs_status st = tune_groups(tu, effort);
+        if (st != RS_OK) return st;
     }
     if (ncand) *ncand = tu.n;
     *best_m = tu.best_m;

@@ -24,6 +24,11 @@
 //   [23+Q+2T .. +NW)    row masks: bit i of word w is set iff row 64w+i passes
 //                       the Jacobi filter, (k/p) = +1 for even r and
 //                       (k*b/p) = +1 for odd r (NW = ceil(nrows / 64))
+//   then, for jobs whose baby table is shared by the NV wavefronts of a
+//   workgroup (JobDev.nwv = NV > 0), at off_x:
+//   [off_x .. +NV)      g^(64 w) for w < NV (Montgomery forms; fp: doubles)
+//   [off_x + NV]        g^(64 NV), the baby-step stride, plain (fp: double)
+//   [off_x + NV + 1]    its Shoup constant (fp: unused)
 enum { REC_P = 0, REC_MP = 1, REC_PINV = 2, REC_G64 = 3, REC_GW = 4, REC_GWS = 5, REC_GP = 6,
        REC_G8P = 14, REC_PD = 22, REC_BH = 23 };
 
@@ -71,6 +76,11 @@ struct JobDev {
                               // (row mod ROW_SLICE) << 19)
     const uint32_t* bitmap;   // optional n-list bitmaps (rows x words, MSB first)
     int words;                // 32-bit words per row bitmap
+    int nwv;        // 0: one wavefront per prime with its own table; 4 or 8:
+                    // the wavefronts of a workgroup share one prime's table
+                    // (RS_TILE_GROUP4 / RS_TILE_GROUP8)
+    int off_x;      // record words of the shared-table layout (nwv > 0)
+    int lds_words_wg;  // 32-bit LDS words of one workgroup (nwv > 0)
 };
 
 // A chunk of primes p = low + offs[i], i < count, of one job; group0 is
@@ -99,8 +109,8 @@ struct HitDev {
 // one, in order of k, while the job's tables stay within JTAB_MAX_WORDS.
 enum : uint32_t { JTAB_MAX_N = 1u << 28, JTAB_MAX_WORDS = 1u << 26 };
 
-static inline int rec_words(int Q, int T, int nrows) {
-    return REC_BH + Q + 2 * T + (nrows + 63) / 64;
+static inline int rec_words(int Q, int T, int nrows, int nwv = 0) {
+    return REC_BH + Q + 2 * T + (nrows + 63) / 64 + (nwv ? nwv + 2 : 0);
 }
 
 // LDS words of one wavefront: its table, placed apart from the rest so
@@ -122,4 +132,25 @@ static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W,
     const int NB = 1 << logNB;
     int w = 5 * NB + 2 * Q * ys + list_words(nrows, idx) + 2 * W;
     return (w + 3) & ~3;
+}
+
+// LDS words of one workgroup whose wavefronts share one prime's table
+// (nwv > 0): the table first, at LDS address 0 (so aligned to its size),
+//   4 NB                      NB buckets x 8 uint16 fingerprints (0 = empty)
+//   NB                        bucket fill counts
+//   2 Q ys                    Y entries (uint64)
+//   list_words(nrows, false)  active rows of one slice
+//   2 W                       the prime's record (uint64)
+//   WG_MISC                   the group number and the chain flag
+enum { WG_MISC = 8 };
+static inline int lds_words_group(int logNB, int Q, int ys, int nrows, int W) {
+    const int NB = 1 << logNB;
+    int w = 5 * NB + 2 * Q * ys + list_words(nrows, false) + 2 * W + WG_MISC;
+    return (w + 3) & ~3;
+}
+// Global scratch of one shared-table workgroup, in uint64 words: the
+// records of its 64 nwv primes, then the low words of the current prime's m
+// baby-step values (uint32, in order of j) for exact rescans.
+static inline long long scratch_words_group(int nwv, int W, int m) {
+    return 64ll * nwv * W + (m + 1) / 2;
 }

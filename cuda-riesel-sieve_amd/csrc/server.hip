@@ -197,10 +197,22 @@ static uint64_t window_width(const rs_server* s, const SrvJob& j) {
 // cost-model value down) whose LDS layout fits the server's per-wave cap.
 static rs_status admit(rs_server* s, SrvJob& j) {
     int m = j.m > 0 ? j.m : rs_plan_auto_m(&j.plan);
+    // shared-table jobs (one workgroup per prime) need m in multiples of 64
+    // nwv and a workgroup layout that fits the device; others a wavefront
+    // layout within the server's cap
+    const int nwv = (j.tile & RS_TILE_GROUP8) ? 8 : (j.tile & RS_TILE_GROUP4) ? 4 : 0;
+    const int mstep = 64 * std::max(1, nwv);
     while (true) {
         DevJob dj;
         rs_status st = make_job(&j.plan, m, j.tile, j.dat, &dj);
-        if (st == RS_OK && dj.J.lds_words <= s->lds_cap_words && dj.J.W <= s->w_cap) {
+        bool fits = false;
+        if (st == RS_OK && nwv > 0) {
+            Launcher g;
+            fits = launch_geometry_group(s->device, dj.J, &g) == RS_OK;
+        } else if (st == RS_OK) {
+            fits = dj.J.lds_words <= s->lds_cap_words && dj.J.W <= s->w_cap;
+        }
+        if (fits) {
             j.dj = dj;
             j.m = m;
             j.ns = j.ns_hint > 0 ? j.ns_hint : est_ns_per_prime(j.plan, dj.J);
@@ -208,8 +220,8 @@ static rs_status admit(rs_server* s, SrvJob& j) {
         }
         free_job(dj);
         if (st != RS_OK && st != RS_ERR_ARG) return st;
-        if (m <= 64) return RS_ERR_BUDGET;
-        m -= 64;
+        if (m <= mstep) return RS_ERR_BUDGET;
+        m -= mstep;
     }
 }
 

@@ -827,6 +827,421 @@ __device__ __forceinline__ void sieve_body(
     }
 }
 
+// ------------------------------------------------------------ shared tables
+//
+// Jobs whose best table is larger than one wavefront's share of LDS allows
+// (many rows over a wide n window: the best m grows as sqrt(rows * X)) run
+// with one table per workgroup: the NV wavefronts of a workgroup work on one
+// prime together.  Each wavefront sets up 64 primes of the workgroup's group
+// of 64 NV; then, prime after prime, the 64 NV lanes fill the shared table
+// (lane t inserts g^(t + 64 NV s)), form the Y table, compact each row slice
+// (wavefront w takes the slice's mask words w, w + NV, ...) and spread the
+// giant steps over all lanes, with workgroup barriers between the phases.
+// The baby steps also write the low words of their values, in order of j,
+// to the workgroup's global scratch, so that a fingerprint match is settled
+// by one wavefront reading m/64 words per lane instead of recomputing the
+// m baby steps (with m in the thousands that rescan would cost about as
+// much as the giant steps themselves).
+
+// The shared-table record words at off_x (kernels.h): g^(64 w) for w < NV,
+// then the stride g^(64 NV) with its Shoup constant.
+template <int NV>
+__device__ __forceinline__ void prep_group_powers(const JobDev& J, u64* __restrict__ R) {
+    const u64 p = R[REC_P], mp = R[REC_MP];
+    if (p < kFpMaxP) {
+        const double pd = __longlong_as_double((long long)R[REC_PD]);
+        const double pinv = __longlong_as_double((long long)R[REC_PINV]);
+        const double g64 = __longlong_as_double((long long)R[REC_GW]);
+        double x = 1.0;
+        for (int w = 0; w < NV; ++w) {
+            R[J.off_x + w] = (u64)__double_as_longlong(x);
+            x = fp_mulmod(x, g64, pd, pinv);
+        }
+        R[J.off_x + NV] = (u64)__double_as_longlong(x);
+        R[J.off_x + NV + 1] = 0;
+    } else {
+        const u64 g64 = R[REC_G64];  // Montgomery form
+        u64 x = R[REC_GP];           // g^0: the Montgomery form of 1
+        for (int w = 0; w < NV; ++w) {
+            R[J.off_x + w] = x;
+            x = montmul(x, g64, p, mp);
+        }
+        u64 w, ws;
+        shoup_pair(x, p, mp, &w, &ws);
+        R[J.off_x + NV] = w;
+        R[J.off_x + NV + 1] = ws;
+    }
+}
+
+// Emits every baby step j whose value equals the exact product of a giant
+// step item (entry (k, yent) of the slice starting at row r0, step tt, Y
+// entry yv): the lanes of one wavefront compare the low words `low` (lane l
+// reads j = l + 64 v) and confirm a low-word match exactly, by g^j from the
+// record (square-and-multiply, in the form of the baby values).
+__device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __restrict__ jobs,
+                                             int job, const u64* R, u64 yv, u32 k, u32 yent,
+                                             int r0, int tt, const u32* __restrict__ low,
+                                             HitDev* __restrict__ hits,
+                                             u32* __restrict__ hit_count, u32 hit_cap) {
+    const int lane = threadIdx.x & 63;
+    const u64 p = R[REC_P];
+    const u64 target = shoup32_full(k, yv, p);
+    const u32 tlo = lo32(target);
+    const int L = J.m >> 6;
+    for (int v = 0; v < L; v += 4) {
+        u32 w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[u] = v + u < L ? low[lane + 64 * (v + u)] : ~tlo;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (w[u] != tlo) continue;
+            const int j = lane + 64 * (v + u);
+            u64 gj;
+            if (p < kFpMaxP) {
+                const double pd = __longlong_as_double((long long)R[REC_PD]);
+                const double pinv = __longlong_as_double((long long)R[REC_PINV]);
+                double d = fp_pow(__longlong_as_double((long long)R[REC_GP + 1]), (u32)j, pd, pinv);
+                d = d < 0.0 ? d + pd : d;
+                gj = fp_bits(d) & ((1ull << 52) - 1);
+            } else {
+                gj = mont_pow(R[REC_GP + 1], (u32)j, R[REC_GP], p, R[REC_MP]);
+            }
+            if (gj != target) continue;
+            const u32 r = ((yent & 0x7ffffu) >> 3) / (u32)J.ys;
+            const u32 row = (u32)r0 + (yent >> 19);
+            const JobDev* const jp = jobs + job;
+            const int x0 = const_ld(&jp->x0);
+            const long long xx = (long long)x0 + (long long)tt * J.m + (J.m - 1 - j);
+            const long long n = (long long)J.Q * xx + (int)r;
+            bool keep = n >= const_ld(&jp->minN) && n <= const_ld(&jp->maxN);
+            const uint32_t* const bitmap = const_ld(&jp->bitmap);
+            if (keep && bitmap) {
+                const long long xp = xx - x0;
+                keep = (global_ld(bitmap + (long long)row * const_ld(&jp->words) + (xp >> 5)) >>
+                        (31 - (xp & 31))) & 1;
+            }
+            if (keep) {
+                const u32 at = atomicAdd(hit_count, 1u);
+                if (at < hit_cap) {
+                    HitDev h;
+                    h.p = p;
+                    h.k = k;
+                    h.n = (u32)n;
+                    h.job = (u32)job;
+                    h.pad = 0;
+                    hits[at] = h;
+                }
+            }
+        }
+    }
+}
+
+// Giant steps of one prime over the lanes of the whole workgroup: as
+// giant_steps, with 64 NV runs per round and the table at LDS address 0.
+template <int NV, int TW, bool CHAIN, bool EXACT>
+__device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev* __restrict__ jobs,
+                                                  int job, int A, int r0,
+                                                  const u32* __restrict__ cnt,
+                                                  const char* __restrict__ Yb,
+                                                  const uint2* __restrict__ list, const u64* R,
+                                                  u32 negp0, const u32* __restrict__ low,
+                                                  HitDev* __restrict__ hits,
+                                                  u32* __restrict__ hit_count, u32 hit_cap) {
+    const int lid = threadIdx.x;  // lane of the workgroup
+    const u32 nbm = (1u << J.logNB) - 1;
+    u32 bmask = nbm << 4;
+    asm volatile("" : "+v"(bmask));
+    const int T = J.T;
+    const int total = A * T;
+    int a = div_T((u32)(TW * lid), J.invT);
+    int t0 = TW * lid - a * T;
+    const int rstep = div_T(64u * NV * TW, J.invT), tstep = 64 * NV * TW - rstep * T;
+    for (int base = 0; base < total; base += 64 * NV * TW) {
+        const bool ok = a < A;
+        const u64 okm = lanes(ok);
+        const uint2 row = list[min(a, A - 1)];
+        const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * t0);
+        uint4 e[TW];
+        u32 fp2[TW], ba[TW];
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            const u32 lo = shoup32_lo(row.x, lo32(yp[u]), hi32(yp[u]), negp0);
+            ba[u] = lo & bmask;  // the bucket's LDS byte address
+            fp2[u] = fingerprint2(lo);
+        }
+#pragma unroll
+        for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
+        u64 mm[TW], full[TW];
+        u64 anyf = 0;
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            mm[u] = lanes(bucket_match(e[u], fp2[u])) & okm;
+            if (CHAIN) {
+                full[u] = lanes(cnt[ba[u] >> 4] >= 8) & okm;
+                anyf |= full[u];
+            }
+        }
+        if (CHAIN && anyf) {
+#pragma unroll
+            for (int u = 0; u < TW; ++u) {
+                bool hit = false;
+                if (in_mask(full[u])) {
+                    u32 b = ba[u] >> 4;
+                    while (true) {
+                        b = (b + 1) & nbm;
+                        const uint4 f = lds_bucket(16 * b);
+                        if (bucket_match(f, fp2[u])) hit = true;
+                        if (cnt[b] < 8) break;
+                    }
+                }
+                mm[u] |= lanes(hit);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            u64 mb = mm[u];
+            while (mb) {
+                const int src = __builtin_ctzll(mb);
+                mb &= mb - 1;
+                const u32 k = __builtin_amdgcn_readlane(row.x, src);
+                const u32 yent = __builtin_amdgcn_readlane(row.y, src);
+                const int tt = __builtin_amdgcn_readlane(t0, src) + u;
+                const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
+                rescan_group(J, jobs, job, R, yv, k, yent, r0, tt, low, hits, hit_count,
+                             hit_cap);
+            }
+        }
+        a += rstep;
+        if (!EXACT) {
+            t0 += tstep;
+            if (t0 >= T) t0 -= T, a += 1;
+        }
+    }
+}
+
+// BSGS of one prime by the NV wavefronts of the workgroup.  Rg: the
+// prime's record in global scratch; the table at LDS address 0, the rest of
+// the layout (lds_words_group) from wbase; low: the workgroup's m low
+// words in global scratch.
+template <int NV, int BB>
+__device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* __restrict__ jobs,
+                                                 int job, const u64* __restrict__ Rg,
+                                                 uint4* __restrict__ tab, u32* __restrict__ wbase,
+                                                 u32* __restrict__ low,
+                                                 HitDev* __restrict__ hits,
+                                                 u32* __restrict__ hit_count, u32 hit_cap) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const u32 NB = 1u << J.logNB;
+    const u32 nbm = NB - 1;
+    const int Q = J.Q, T = J.T, m = J.m, W = J.W;
+    u32* const cnt = wbase;
+    u64* const Y = (u64*)(cnt + NB);
+    uint2* const list = (uint2*)(Y + Q * J.ys);
+    u64* const R = (u64*)((u32*)list + list_words(J.nrows, false));
+    u32* const misc = (u32*)(R + W);  // [0]: some insert overflowed its bucket
+
+    for (int w = tid; w < W; w += 64 * NV) R[w] = Rg[w];
+    for (u32 b = tid; b < NB; b += 64 * NV) tab[b] = make_uint4(0, 0, 0, 0);
+    for (u32 c = 4 * tid; c < NB; c += 256 * NV) *(uint4*)(cnt + c) = make_uint4(0, 0, 0, 0);
+    if (tid == 0) misc[0] = 0;
+    __syncthreads();
+    const u64 p = R[REC_P], mp = R[REC_MP];
+    const bool fpm = p < kFpMaxP;
+    auto lds_d = [](u64 v) { return __longlong_as_double((long long)v); };
+    const double pd = fpm ? lds_d(R[REC_PD]) : 0.0, pinv = fpm ? lds_d(R[REC_PINV]) : 0.0;
+
+    // baby steps: lane t inserts g^j, j = t + 64 NV s, starting from
+    // g^(t & 7) g^(8 ((t >> 3) & 7)) g^(64 w)
+    const int L = m / (64 * NV);
+    double vd = 0.0;
+    u64 v = 0;
+    if (fpm) {
+        vd = fp_mulmod(fp_mulmod(lds_d(R[REC_GP + (lane & 7)]), lds_d(R[REC_G8P + (lane >> 3)]),
+                                 pd, pinv),
+                       lds_d(R[J.off_x + wv]), pd, pinv);
+    } else {
+        v = montmul(montmul(R[REC_GP + (lane & 7)], R[REC_G8P + (lane >> 3)], p, mp),
+                    R[J.off_x + wv], p, mp);
+    }
+    bool ovf = false;
+    auto place = [&](u32 b, u32 ta, u32 c) {
+        if (c >= 8) {
+            ovf = true;
+            do {
+                b = (b + 1) & nbm;
+                c = atomicAdd(&cnt[b], 1u);
+            } while (c >= 8);
+            ta = 16 * b;
+        }
+        return ta + 2 * c;
+    };
+    auto store_fp = [](u32 at, u32 lo) {
+        *(__attribute__((address_space(3))) unsigned short*)(size_t)at = fingerprint(lo);
+    };
+    auto insert = [&](u32 lo) {
+        const u32 b = bucket_of(lo, (u32)J.logNB);
+        store_fp(place(b, 16 * b, atomicAdd(&cnt[b], 1u)), lo);
+    };
+    const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;
+    const u32 plo = lo32(p);
+    auto baby = [&](auto step) {
+        int s0 = 0;
+        for (; s0 + BB <= L; s0 += BB) {
+            u32 lo[BB], ta[BB], c[BB], bk[BB];
+            bool dup[BB];
+#pragma unroll
+            for (int u = 0; u < BB; ++u) {
+                step(lo[u], dup[u]);
+                low[tid + 64 * NV * (s0 + u)] = lo[u];
+            }
+#pragma unroll
+            for (int u = 0; u < BB; ++u) {
+                bk[u] = bucket_of(lo[u], (u32)J.logNB);
+                ta[u] = 16 * bk[u];
+                asm volatile("" : "+v"(ta[u]));
+                c[u] = atomicAdd(&cnt[bk[u]], 1u);
+            }
+#pragma unroll
+            for (int u = 0; u < BB; ++u) store_fp(place(bk[u], ta[u], c[u]), lo[u]);
+#pragma unroll
+            for (int u = 0; u < BB; ++u)
+                if (dup[u]) insert(lo[u] + plo);
+        }
+        for (; s0 < L; ++s0) {
+            u32 lo;
+            bool dup;
+            step(lo, dup);
+            low[tid + 64 * NV * s0] = lo;
+            insert(lo);
+            if (dup) insert(lo + plo);
+        }
+    };
+    if (fpm) {
+        const double wd = lds_d(R[J.off_x + NV]), bd = lds_d(R[REC_GWS]);
+        baby([&](u32& lo, bool& dup) {
+            const double c = vd < 0.0 ? vd + pd : vd;
+            lo = lo32(fp_bits(c));
+            dup = c < bd;
+            vd = fp_mulmod(vd, wd, pd, pinv);
+        });
+    } else {
+        const u64 gw = R[J.off_x + NV], gws = R[J.off_x + NV + 1];
+        baby([&](u32& lo, bool& dup) {
+            lo = lo32(v);
+            dup = v < bound;
+            v = shoup_mul(v, gw, gws, p);
+        });
+    }
+    if (__ballot(ovf) != 0 && lane == 0) atomicOr(&misc[0], 1u);
+
+    // Y table over all lanes of the workgroup (as in bsgs_prime)
+    const u32 mp0 = lo32(mp);
+    for (int idx = tid; idx < Q * T; idx += 64 * NV) {
+        const int r = div_T((u32)idx, J.invT), t = idx - r * T;
+        u64 z;
+        if (fpm) {
+            double zd = fp_mulmod(lds_d(R[REC_BH + r]), lds_d(R[J.off_gt + 2 * t]), pd, pinv);
+            zd = zd < 0.0 ? zd + pd : zd;
+            z = fp_bits(zd) & ((1ull << 52) - 1);
+        } else {
+            z = shoup_mul(R[REC_BH + r], R[J.off_gt + 2 * t], R[J.off_gt + 2 * t + 1], p);
+        }
+        Y[r * J.ys + t] = shoup32_entry(z, p, mp0);
+    }
+    __syncthreads();  // table, low words and Y complete
+    const bool chain = misc[0] != 0;
+
+    for (int r0 = 0; r0 < J.nrows; r0 += ROW_SLICE) {
+        const int r1 = min(J.nrows, r0 + ROW_SLICE);
+        const int nw = (r1 - r0 + 63) >> 6;
+        // mask words of the slice; wavefront w compacts words w, w + NV, ...
+        // after the rows of the words before them
+        int A = 0;
+        for (int w = 0; w < nw; ++w) {
+            const u64 mask = R[J.off_j + (r0 >> 6) + w];
+            if ((w % NV) == wv) {
+                if (in_mask(mask)) {
+                    const u32 below = __builtin_amdgcn_mbcnt_hi(
+                        hi32(mask), __builtin_amdgcn_mbcnt_lo(lo32(mask), 0));
+                    list[A + below] = global_ld(J.rowent + r0 + 64 * w + lane);
+                }
+            }
+            A += __popcll(mask);
+        }
+        __syncthreads();
+        const char* const Yb = (const char*)Y;
+#define RS_GIANT_G(TWV, CH, EX)                                                              \
+    giant_steps_group<NV, TWV, CH, EX>(J, jobs, job, A, r0, cnt, Yb, list, R, 0u - lo32(p), \
+                                       low, hits, hit_count, hit_cap)
+#define RS_GIANT_G_TW(CH, EX)  \
+    if (J.tw >= 4)             \
+        RS_GIANT_G(4, CH, EX); \
+    else if (J.tw == 2)        \
+        RS_GIANT_G(2, CH, EX); \
+    else                       \
+        RS_GIANT_G(1, CH, EX);
+        if (chain) {
+            RS_GIANT_G_TW(true, false)
+        } else if ((64 * NV * J.tw) % T == 0) {
+            RS_GIANT_G_TW(false, true)
+        } else {
+            RS_GIANT_G_TW(false, false)
+        }
+#undef RS_GIANT_G_TW
+#undef RS_GIANT_G
+        __syncthreads();  // the next slice rewrites the list, the next prime the table
+    }
+}
+
+// The persistent shared-table kernel body: single-chunk launches (c0).
+// Each workgroup takes groups of 64 NV consecutive primes from the work
+// counter; its scratch holds their records and the low words.
+template <int NV, int BB>
+__device__ __forceinline__ void sieve_body_group(
+    ChunkDev c0, const JobDev* __restrict__ jobs, long long ngroups, int lds_words,
+    int scratch_words, uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
+    HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
+    extern __shared__ uint32_t lds[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const JobDev J = uniform_load(jobs + c0.job);
+    uint4* const tab = (uint4*)lds;
+    u32* const wbase = lds + 4 * (1 << J.logNB);
+    u32* const gmisc = lds + lds_words - 4;  // the group number (WG_MISC words at the end)
+    u64* const S = scratch + (long long)blockIdx.x * scratch_words;
+    u32* const low = (u32*)(S + 64ll * NV * J.W);
+    while (true) {
+        if (tid == 0) {
+            const unsigned long long g = atomicAdd(work, 1ull);
+            gmisc[0] = (u32)g;
+            gmisc[1] = (u32)(g >> 32);
+        }
+        __syncthreads();
+        const long long grp =
+            ((long long)(u32)__builtin_amdgcn_readfirstlane(gmisc[1]) << 32) |
+            (u32)__builtin_amdgcn_readfirstlane(gmisc[0]);
+        __syncthreads();  // every lane has read it before it is rewritten
+        if (grp >= ngroups) break;
+        const long long q0 = grp * 64 * NV;
+        const long long cnt =
+            c0.count_dev ? (long long)(u32)__builtin_amdgcn_readfirstlane(*c0.count_dev)
+                         : c0.count;
+        const int n = (int)min((long long)(64 * NV), cnt - q0);
+        if (n <= 0) break;  // device-counted chunk: groups past the end are empty
+        // set-up, one lane per prime: wavefront w takes primes 64 w ..
+        const int i = 64 * wv + lane;
+        if (i < n) {
+            u64* const R = S + (long long)i * J.W;
+            prep_record(J, c0.low + global_ld(c0.offs + q0 + i), R);
+            prep_group_powers<NV>(J, R);
+        }
+        __syncthreads();
+        for (int k = 0; k < n; ++k)
+            bsgs_prime_group<NV, BB>(J, jobs, c0.job, S + (long long)k * J.W, tab, wbase, low,
+                                     hits, hit_count, hit_cap);
+    }
+}
+
 #ifndef RS_BB4
 #define RS_BB4 4
 #endif
@@ -874,6 +1289,27 @@ rs_sieve_kernel5_rix(ChunkDev c0, const JobDev* __restrict__ jobs,
     sieve_body<true, BB5>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
                      scratch_words, scratch, work, hits, hit_count, hit_cap);
 }
+
+// One workgroup of 4 or 8 wavefronts per prime, one shared table
+// (RS_TILE_GROUP4 / RS_TILE_GROUP8), at least 4 wavefronts per SIMD.
+#define RS_GROUP_KERNEL(NAME, NV)                                                            \
+    extern "C" __global__ void __launch_bounds__(64 * NV)                                   \
+        __attribute__((amdgpu_waves_per_eu(4)))                                             \
+        NAME(ChunkDev c0, const JobDev* __restrict__ jobs, const ChunkDev* __restrict__ chunks, \
+             int nchunks, long long ngroups, int lds_words, int tab_words, int shared_words,   \
+             int scratch_words, uint64_t* __restrict__ scratch,                                \
+             unsigned long long* __restrict__ work, HitDev* __restrict__ hits,                 \
+             uint32_t* __restrict__ hit_count, uint32_t hit_cap) {                             \
+        (void)chunks;                                                                       \
+        (void)nchunks;                                                                      \
+        (void)tab_words;                                                                    \
+        (void)shared_words;                                                                 \
+        sieve_body_group<NV, BB4>(c0, jobs, ngroups, lds_words, scratch_words, scratch, work, \
+                                  hits, hit_count, hit_cap);                                \
+    }
+RS_GROUP_KERNEL(rs_sieve_group4_kernel, 4)
+RS_GROUP_KERNEL(rs_sieve_group8_kernel, 8)
+#undef RS_GROUP_KERNEL
 
 // ------------------------------------------------------------ test kernels
 

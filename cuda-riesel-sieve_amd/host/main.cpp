@@ -27,7 +27,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
 #include <ctime>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "rsieve.h"
@@ -51,6 +55,115 @@ static void usage() {
         "-d DEV   : GPU device index (default 0).\n"
         "-h       : Prints this help\n");
 }
+
+// One launch's primes: p = low + offs[i], i < n (n = -1: the p-range
+// [low, next) is generated on the device); next is the following batch's
+// low; done marks the end of the range.
+struct Batch {
+    uint64_t low = 0, next = 0;
+    long long n = 0;
+    std::vector<uint32_t> offs;
+    bool done = true;
+};
+
+// CPU-generated batches from a producer thread: windows of about 8 batches
+// are sieved at once by rs_primes_range (all host threads) and cut into
+// batches of `size` primes as the reference cuts its prime arrays (:1561),
+// at most 4 batches ahead of the sieve.
+class BatchFeed {
+   public:
+    BatchFeed(rs_primegen* g, uint64_t low, uint64_t high, long long size)
+        : g_(g), low_(low), high_(high), size_(size) {}
+    ~BatchFeed() {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+    void start() { th_ = std::thread([this] { run(); }); }
+    // The next batch (done = true after the last), or the generator's error.
+    rs_status pop(Batch* b) {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_.wait(l, [&] { return !q_.empty() || finished_; });
+        if (!q_.empty()) {
+            *b = std::move(q_.front());
+            q_.pop_front();
+            cv_.notify_all();
+            return RS_OK;
+        }
+        b->done = true;
+        return st_;
+    }
+
+   private:
+    void push(Batch&& b) {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_.wait(l, [&] { return q_.size() < 4 || stop_; });
+        q_.push_back(std::move(b));
+        cv_.notify_all();
+    }
+    void finish(rs_status st) {
+        std::lock_guard<std::mutex> l(mu_);
+        st_ = st;
+        finished_ = true;
+        cv_.notify_all();
+    }
+    void run() {
+        uint64_t cur = low_, wlo = low_;
+        std::vector<uint64_t> pend;  // generated primes not yet in a batch
+        size_t head = 0;
+        std::vector<uint32_t> win;
+        while (cur <= high_) {
+            {
+                std::lock_guard<std::mutex> l(mu_);
+                if (stop_) break;
+            }
+            while (pend.size() - head < (size_t)size_ && wlo <= high_) {
+                const double lnp = std::log((double)std::max<uint64_t>(wlo, 16));
+                uint64_t width = (uint64_t)(8.0 * size_ * lnp);
+                width = std::min<uint64_t>(std::max<uint64_t>(width, 1ull << 20), 1ull << 32);
+                const uint64_t whi = std::min<uint64_t>(wlo + width, high_ + 1);
+                win.resize((size_t)((whi - wlo) / std::max(1.0, lnp - 1.5) * 1.2) + 4096);
+                int64_t cnt = 0;
+                rs_status st = rs_primes_range(g_, wlo, whi, win.data(), (int64_t)win.size(), &cnt);
+                if (st == RS_ERR_OVERFLOW) {
+                    win.resize((size_t)cnt);
+                    st = rs_primes_range(g_, wlo, whi, win.data(), cnt, &cnt);
+                }
+                if (st != RS_OK) return finish(st);
+                pend.erase(pend.begin(), pend.begin() + head);
+                head = 0;
+                for (int64_t i = 0; i < cnt; ++i) pend.push_back(wlo + win[i]);
+                wlo = whi;
+            }
+            Batch b;
+            b.low = cur;
+            b.done = false;
+            b.offs.reserve((size_t)size_);
+            while (head < pend.size() && (long long)b.offs.size() < size_ &&
+                   pend[head] - cur < (1ull << 32))
+                b.offs.push_back((uint32_t)(pend[head++] - cur));
+            if (b.offs.empty()) break;  // no prime left in [cur, P1]
+            b.n = (long long)b.offs.size();
+            b.next = cur + b.offs.back() + 2;
+            cur = b.next;
+            push(std::move(b));
+        }
+        finish(RS_OK);
+    }
+
+    rs_primegen* g_;
+    uint64_t low_, high_;
+    long long size_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Batch> q_;
+    bool stop_ = false, finished_ = false;
+    rs_status st_ = RS_OK;
+};
 
 static int fail(const char* what, rs_status st) {
     fprintf(stderr, "%s failed: %s (%d)\n", what, rs_strerror(st), st);
@@ -173,57 +286,47 @@ int main(int argc, char* argv[]) {
         return fail("prime generator set-up", st);
 
     const unsigned long long startLow = low;
-    std::vector<uint32_t> cur((size_t)arraySize), nxt((size_t)arraySize);
     std::vector<rs_factor> hits(1 << 20);
     std::vector<rs_factor> verified;  // candidates to remove with -R
-    // one batch: the first arraySize primes >= lo (lo odd), clipped to <= high
-    auto make_batch = [&](uint64_t lo, std::vector<uint32_t>& buf, long long* n,
-                          uint64_t* nextLow) -> rs_status {
-        if (gpuPrimes) {  // a p-range holding about arraySize primes, sieved on the GPU
-            const double lnp = std::log((double)std::max<uint64_t>(lo, 16));
-            uint64_t width = (uint64_t)(arraySize * lnp);
-            width = std::min<uint64_t>(std::max<uint64_t>(width, 2), 1ull << 32);
-            *nextLow = std::min<uint64_t>(lo + width, high + 1);
-            *n = -1;
+    // Batches as in the reference (:1537-1561): the first arraySize primes >=
+    // low (low odd), offsets from low, the next low = last prime + 2; the
+    // last batch is clipped to P1.  With the CPU generator they come from a
+    // producer thread that sieves windows of about 8 batches at once with
+    // all host threads (one window per call keeps every thread busy) and
+    // cuts them into batches, so generation overlaps the sieve; with -g each
+    // launch is a p-range sieved on the GPU (at least 2^30 wide: the device
+    // generator needs wide ranges).
+    BatchFeed feed(gen, low, high, arraySize);
+    if (!gpuPrimes) feed.start();
+    uint64_t gLow = low;
+    auto next_batch = [&](Batch* b) -> rs_status {
+        if (!gpuPrimes) return feed.pop(b);
+        if (gLow > high) {
+            b->done = true;
             return RS_OK;
         }
-        rs_status s2 = rs_primes_batch(gen, lo, buf.data(), arraySize, nextLow);
-        if (s2 != RS_OK) return s2;
-        long long c = arraySize;
-        while (c > 0 && lo + buf[c - 1] > high) --c;
-        *n = c;
+        const double lnp = std::log((double)std::max<uint64_t>(gLow, 16));
+        uint64_t width = std::max<uint64_t>((uint64_t)(arraySize * lnp), 1ull << 30);
+        width = std::min<uint64_t>(width, 1ull << 32);
+        b->low = gLow;
+        b->next = std::min<uint64_t>(gLow + width, high + 1);
+        b->n = -1;
+        b->done = false;
+        gLow = b->next;
         return RS_OK;
     };
-    long long curN = 0, nxtN = 0;
-    uint64_t curLow = low, nextLow = 0;
-    if ((st = make_batch(curLow, cur, &curN, &nextLow)) != RS_OK) return fail("prime generation", st);
-    int kernelCount = 0;
+    // collects and writes the hits of everything submitted so far (the
+    // batches are consecutive in p, so the sorted hits come out in the order
+    // a collect per batch would give)
     long long factorCount = 0;
-    auto t0 = std::chrono::steady_clock::now();
-    while (true) {
-        auto tb = std::chrono::steady_clock::now();
-        ++kernelCount;
-        unsigned long long progress = 0;
-        if (curN < 0) {  // GPU-generated primes of [curLow, nextLow)
-            if ((st = rs_sieve_submit_range(sv, curLow, nextLow)) != RS_OK)
-                return fail("submit", st);
-            progress = nextLow - curLow;
-        } else {
-            if (curN > 0 && (st = rs_sieve_submit(sv, curLow, cur.data(), curN)) != RS_OK)
-                return fail("submit", st);
-            progress = curN > 0 ? cur[curN - 1] - cur[0] : 0;
-        }
-        const bool more = nextLow <= high && nextLow > curLow;
-        const uint64_t nLow = nextLow;
-        if (more && (st = make_batch(nLow, nxt, &nxtN, &nextLow)) != RS_OK)
-            return fail("prime generation", st);
+    auto drain = [&]() -> rs_status {
         int64_t got = 0, lost = 0;
-        st = rs_sieve_collect(sv, hits.data(), (int64_t)hits.size(), &got, &lost);
-        if (st == RS_ERR_OVERFLOW) {
+        rs_status s2 = rs_sieve_collect(sv, hits.data(), (int64_t)hits.size(), &got, &lost);
+        if (s2 == RS_ERR_OVERFLOW) {
             hits.resize((size_t)got);
-            st = rs_sieve_collect(sv, hits.data(), (int64_t)hits.size(), &got, &lost);
+            s2 = rs_sieve_collect(sv, hits.data(), (int64_t)hits.size(), &got, &lost);
         }
-        if (st != RS_OK) return fail("collect", st);
+        if (s2 != RS_OK) return s2;
         if (lost > 0) fprintf(stderr, "Warning: %" PRId64 " hits dropped (append buffer full)\n", lost);
         FILE* ff = nullptr;
         for (int64_t i = 0; i < got; ++i) {
@@ -234,7 +337,7 @@ int main(int argc, char* argv[]) {
                     fprintf(stderr, "Error: device reported %llu | %u*%u^%u%s, which fails "
                             "verification\n", (unsigned long long)f.p, f.k, base, f.n,
                             form == 1 ? "+1" : "-1");
-                    return 1;
+                    exit(1);
                 }
             }
             if (removedFile) verified.push_back(f);
@@ -244,14 +347,44 @@ int main(int argc, char* argv[]) {
             if (!ff) ff = fopen(factorFile, "a");
             if (!ff) {
                 printf("Unable to open factor file %s\n", factorFile);
-                return 1;
+                exit(1);
             }
             fputs(line, ff);
             ++factorCount;
         }
         if (ff) fclose(ff);
+        return RS_OK;
+    };
+    // hits are collected every kCollect launches (the device keeps them in
+    // its append buffer meanwhile), so the host never waits for a launch it
+    // has just queued
+    const int kCollect = 8;
+    int kernelCount = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    auto tb = t0;
+    Batch cur;
+    if ((st = next_batch(&cur)) != RS_OK) return fail("prime generation", st);
+    while (!cur.done) {
+        ++kernelCount;
+        unsigned long long progress = 0;
+        if (cur.n < 0) {  // GPU-generated primes of [low, next)
+            if ((st = rs_sieve_submit_range(sv, cur.low, cur.next)) != RS_OK)
+                return fail("submit", st);
+            progress = cur.next - cur.low;
+        } else {
+            if (cur.n > 0 &&
+                (st = rs_sieve_submit(sv, cur.low, cur.offs.data(), cur.n)) != RS_OK)
+                return fail("submit", st);
+            progress = cur.n > 0 ? cur.offs[cur.n - 1] - cur.offs[0] : 0;
+        }
+        const uint64_t nLow = cur.next;
+        Batch nxt;
+        if ((st = next_batch(&nxt)) != RS_OK) return fail("prime generation", st);
+        if (nxt.done || kernelCount % kCollect == 0)
+            if ((st = drain()) != RS_OK) return fail("collect", st);
         auto te = std::chrono::steady_clock::now();
         const double dt = std::chrono::duration<double>(te - tb).count();
+        tb = te;
         if (kernelCount % 20 == 0) {
             const double done = 100.0 * (double)(nLow - startLow) / (double)(high - startLow + 1);
             const double soFar = std::chrono::duration<double>(te - t0).count();
@@ -264,16 +397,14 @@ int main(int argc, char* argv[]) {
                    progress / dt, done > 100 ? 100.0 : done, factorCount, etaBuf);
             fflush(stdout);
         }
-        if (!more) break;
-        std::swap(cur, nxt);
-        curN = nxtN;
-        curLow = nLow;
+        cur = std::move(nxt);
     }
+    if (kernelCount == 0 && (st = drain()) != RS_OK) return fail("collect", st);
     const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const unsigned long long swept = high + 1 - startLow;
     printf("\nTime taken: %gs\n", total);
     printf("Factors found: %lld\n", factorCount);
-    printf("Time per kernel: %g\n", total / kernelCount);
+    printf("Time per kernel: %g\n", kernelCount ? total / kernelCount : 0.0);
     printf("Progress = %llu at %.0f p/sec\n\n", swept, swept / total);
     if (removedFile) {
         int64_t removed = 0;

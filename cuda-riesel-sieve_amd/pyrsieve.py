@@ -114,8 +114,12 @@ _lib = None
 
 
 
-# tile flag: active rows listed by row number (rsieve.h RS_TILE_ROW_INDEX)
+# tile flags: active rows listed by row number (rsieve.h RS_TILE_ROW_INDEX);
+# one baby table shared by the 4 or 8 wavefronts of a workgroup
+# (RS_TILE_GROUP4, RS_TILE_GROUP8)
 TILE_ROW_INDEX = 16
+TILE_GROUP4 = 32
+TILE_GROUP8 = 64
 
 class RsError(RuntimeError):
     def __init__(self, what, status):
@@ -461,6 +465,13 @@ class DeviceBuffer:
         self.ptr = vp()
         _chk("rs_dev_alloc", lib().rs_dev_alloc(device, max(4 * count, 4), ctypes.byref(self.ptr)))
         return self
+
+    def download(self, dtype=np.uint32):
+        """The buffer's contents as a numpy array."""
+        out = np.zeros(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        _chk("rs_dev_download", lib().rs_dev_download(self.device, out.ctypes.data_as(vp),
+                                                      self.ptr, out.nbytes))
+        return out
 
     def free(self):
         if self.ptr:

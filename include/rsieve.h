@@ -53,6 +53,15 @@ typedef struct rs_factor {
     uint32_t n;
 } rs_factor;
 
+/* One hit of a batched launch: p | k*b^n-1 for job `job`. */
+typedef struct rs_job_factor {
+    uint64_t p;
+    uint32_t k;
+    uint32_t n;
+    uint32_t job;
+    uint32_t pad;
+} rs_job_factor;
+
 /* ------------------------------------------------------------ ABCD input */
 /* Replaces countKs (:113-137) + the parser (:938-1083): a line whose first
  * space-separated token is "ABCD" starts a k ("ABCD k*b^$a-1 [n0]"), the
@@ -186,6 +195,14 @@ int rs_format_factor(const rs_factor* f, uint32_t b, int c, char* buf, int cap);
  * slot per prime, :615-621). */
 typedef struct rs_sieve rs_sieve;
 #define RS_TILE_ROW_INDEX 16
+/* Shared-table layouts (tile flags, with tile 1, 2 or 4; not with
+ * RS_TILE_ROW_INDEX): the 4 or 8 wavefronts of a workgroup work on one prime
+ * together with one baby table in the workgroup's LDS, for tables larger
+ * than one wavefront's share of LDS allows (jobs with many rows over wide n
+ * windows, whose best m runs into the thousands).  m must be a multiple of
+ * 64 * 4 or 64 * 8; rs_sieve_info then reports the workgroup's LDS bytes. */
+#define RS_TILE_GROUP4 32
+#define RS_TILE_GROUP8 64
 /* Job-shape limits (rs_sieve_create, rs_batch_create, rs_autotune and the
  * server return RS_ERR_ARG outside them; rs_plan_auto_m only picks shapes
  * inside them): with X = floor(maxN/Q) - floor(minN/Q) + 1 and T the giant
@@ -207,7 +224,8 @@ rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
                         int* lds_bytes_per_wave, int* tile);
 /* Name of the kernel build the job's launches run (static string; "" for
  * NULL): rs_sieve_kernel, rs_sieve_kernel5 (layouts small enough for five
- * wavefronts per SIMD) or rs_sieve_kernel5_rix (RS_TILE_ROW_INDEX). */
+ * wavefronts per SIMD), rs_sieve_kernel5_rix (RS_TILE_ROW_INDEX) or
+ * rs_sieve_group4_kernel / rs_sieve_group8_kernel (shared tables). */
 const char* rs_sieve_kernel_name(const rs_sieve* s);
 /* Primes p = low + offsets[i] from HOST memory (p odd, 3 <= p < 2^63, p ∤ b).
  * Copied through pinned double buffers on a copy stream, then sieved on the
@@ -262,14 +280,6 @@ rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
  * shares one LDS layout sized for the largest job. */
 typedef struct rs_batch rs_batch;
 
-/* One hit of a batched launch: p | k*b^n-1 for job `job`. */
-typedef struct rs_job_factor {
-    uint64_t p;
-    uint32_t k;
-    uint32_t n;
-    uint32_t job;
-    uint32_t pad;
-} rs_job_factor;
 
 rs_status rs_batch_create(int device, int njobs, const rs_plan* const* plans,
                           const int* ms, const int* tiles, int64_t hit_capacity,

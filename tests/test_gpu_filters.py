@@ -5,7 +5,7 @@ Row filter: the set-up phase folds the Legendre test of the reference
 (kernel_v0.24.6.cu:491-521) into per-row pass masks from per-k Jacobi
 constants, quadratic-character tables, (2/p)^z, reciprocity and, for
 k*b^n+1, (-1/p).  Each mask bit is compared with Euler's criterion on
-c*k*b^r mod p, row by row, over k on every path (no table, table with p
+c*k*b^n mod p, n = r + Q, row by row, over k on every path (no table, table with p
 above and below the odd part, even k of odd and even valuation, k = 1, a
 table budget that runs out), bases of every 2-adic shape, p in every class
 mod 8 on both sides of 2^50, p dividing k or b, and both forms.
@@ -99,9 +99,12 @@ def expected_masks(plan, primes):
     rk, rr, _ = plan.rows()
     nw = (plan.nrows + 63) // 64
     out = np.zeros((len(primes), nw), np.uint64)
-    c = 1 if plan.c == -1 else -1  # k*b^n-1: k b^r a square; k*b^n+1: -k b^r
+    # a row (k, r) can hold a factor of p iff c k b^n is a square mod p for
+    # its n = Q x + r >= 1 (c = 1 for k*b^n-1, -1 for k*b^n+1); b^Q is a
+    # square, so n = r + Q decides (and p | b fails every row)
+    c = 1 if plan.c == -1 else -1
     for i, p in enumerate(primes):
-        br = [pow(plan.base, r, p) for r in range(plan.Q)]
+        br = [pow(plan.base, r + plan.Q, p) for r in range(plan.Q)]
         words = [0] * nw
         for j, (k, r) in enumerate(zip(rk.tolist(), rr.tolist())):
             x = c * k * br[r] % p
