@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../host/host.h"
@@ -195,6 +196,75 @@ extern "C" rs_status rs_sieve_submit_device(rs_sieve* s, uint64_t low,
     if (low < 3 || low >= (1ull << 63) - (1ull << 32)) return RS_ERR_ARG;
     HIPCHK(hipSetDevice(s->device));
     return sieve_launch(s, low, offsets_dev, count);
+}
+
+// ------------------------------------------------------------ no allocation
+//
+// rs_sieve_submit_device without allocating: the record scratch and the
+// work counter live in the file-scope array `scratch` (one copy per device,
+// held until the process ends), hits go to the caller's buffer.  Host code
+// never passes `scratch` itself to a kernel (its name is a host-side stand-in
+// on the host): the device address of each device's copy is taken once with
+// hipGetSymbolAddress and kept in g_scratch_dev.
+namespace {
+
+const size_t kNoAllocWords = (size_t)16 << 20;  // 128 MiB of uint64
+const int kNoAllocCounterWords = 32;             // work counter, padded
+const int kMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) uint64_t scratch[kNoAllocWords];
+
+static std::mutex g_scratch_mu;
+static uint64_t* g_scratch_dev[kMaxDevices];
+
+extern "C" __global__ void rs_zero_u32(uint32_t*, int);
+
+static rs_status noalloc_scratch(int device, uint64_t** ptr) {
+    if (device < 0 || device >= kMaxDevices) return RS_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = (uint64_t*)p;
+    }
+    *ptr = g_scratch_dev[device];
+    return RS_OK;
+}
+
+extern "C" rs_status rs_sieve_submit_device_noalloc(rs_sieve* s, uint64_t low,
+                                                    const uint32_t* offsets_dev, int64_t count,
+                                                    rs_job_factor* hits_dev, int64_t hit_cap,
+                                                    uint32_t* nhits_dev) {
+    if (!s || count < 0 || (count > 0 && !offsets_dev) || !hits_dev || !nhits_dev ||
+        hit_cap < 1 || hit_cap > 0x7fffffff)
+        return RS_ERR_ARG;
+    if (low < 3 || low >= (1ull << 63) - (1ull << 32)) return RS_ERR_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    uint64_t* base = nullptr;
+    rs_status st = noalloc_scratch(s->device, &base);
+    if (st != RS_OK) return st;
+    // as many resident workgroups as the array's records hold
+    Launcher L = s->L;
+    const size_t per_block = (size_t)L.scratch_words * (L.nwv > 0 ? 1 : L.wpb);
+    const size_t blocks = (kNoAllocWords - kNoAllocCounterWords) / per_block;
+    if (blocks < 1) return RS_ERR_ARG;  // one workgroup's records exceed the array
+    L.grid = (int)std::min<size_t>((size_t)L.grid, blocks);
+    L.work = (unsigned long long*)base;
+    L.scratch = base + kNoAllocCounterWords;
+    L.hits = (HitDev*)hits_dev;
+    L.hit_count = nhits_dev;
+    L.hit_cap = (uint32_t)hit_cap;
+    // the hit count starts at zero (a kernel, not a memset)
+    hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, s->stream, nhits_dev, 1);
+    HIPCHK(hipGetLastError());
+    if (count == 0) return RS_OK;
+    ChunkDev c0{};
+    c0.low = low;
+    c0.offs = offsets_dev;
+    c0.count = count;
+    return launch(L, s->stream, s->job.d_self, c0, nullptr, 1, groups_of(L, count));
 }
 
 extern "C" rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* offsets,

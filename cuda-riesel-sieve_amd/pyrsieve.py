@@ -74,6 +74,7 @@ _SIGS = {
     "rs_sieve_submit": (i32, [vp, u64, P(u32), i64]),
     "rs_sieve_submit_device": (i32, [vp, u64, vp, i64]),
     "rs_sieve_submit_range": (i32, [vp, u64, u64]),
+    "rs_sieve_submit_device_noalloc": (i32, [vp, u64, vp, i64, vp, i64, vp]),
     "rs_sieve_collect": (i32, [vp, P(Factor), i64, P(i64), P(i64)]),
     "rs_sieve_sync": (i32, [vp]),
     "rs_sieve_timing": (i32, [vp, P(ctypes.c_double), P(i64)]),
@@ -510,6 +511,16 @@ class Sieve:
         count = dbuf.count - start if count is None else count
         ptr = vp(dbuf.ptr.value + 4 * start)
         _chk("rs_sieve_submit_device", lib().rs_sieve_submit_device(self.h, low, ptr, count))
+
+    def submit_device_noalloc(self, low, dbuf, hits_buf, nhits_buf, start=0, count=None):
+        """Sieves device-resident offsets without allocating: hits into
+        hits_buf (a DeviceBuffer of 6 words per hit), their number into
+        nhits_buf (one word).  Asynchronous; read both after sync()."""
+        count = dbuf.count - start if count is None else count
+        ptr = vp(dbuf.ptr.value + 4 * start)
+        _chk("rs_sieve_submit_device_noalloc",
+             lib().rs_sieve_submit_device_noalloc(self.h, low, ptr, count, hits_buf.ptr,
+                                                  hits_buf.count // 6, nhits_buf.ptr))
 
     def submit_range(self, lo, hi):
         """Primes of [lo, hi) generated on the device, then sieved."""

@@ -237,6 +237,21 @@ rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* offsets,
  * persistent launch, asynchronous. */
 rs_status rs_sieve_submit_device(rs_sieve* s, uint64_t low,
                                  const uint32_t* offsets_dev, int64_t count);
+/* rs_sieve_submit_device that allocates nothing (no reference counterpart:
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture or tight serving loops).  The record scratch and the work
+ * counter live in a static 128 MiB device array of the library (one per
+ * device, held until the process ends; the wavefronts launched are limited
+ * to what it holds, RS_ERR_ARG if not even one workgroup's records fit);
+ * hits go to the caller's device buffer hits_dev[hit_cap] (job 0, in no
+ * particular order), their number to the device word *nhits_dev (cleared
+ * by the call; a value above hit_cap means hits were dropped).  Enqueued on
+ * the sieve's stream: rs_sieve_sync waits for it.  Calls that overlap on one
+ * device race on the array. */
+rs_status rs_sieve_submit_device_noalloc(rs_sieve* s, uint64_t low,
+                                         const uint32_t* offsets_dev, int64_t count,
+                                         rs_job_factor* hits_dev, int64_t hit_cap,
+                                         uint32_t* nhits_dev);
 /* Generates every prime of [lo, hi) on the device and sieves them: no host
  * prime generation and no PCIe copy (3 <= lo, hi <= 2^63; ranges wider than
  * 2^32 are split).  Asynchronous.  Replaces generateGPUPrimes (:1838-1987)
