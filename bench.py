@@ -732,15 +732,17 @@ def bench_cli(args):
 
 
 def bench_msweep(args, rs, p0):
-    """Table sizes m = 2^10 .. 2^14 for the R745-shaped job (tiles 4 and 4
-    with row-number lists) on the primes of [1e12, 1e12+2^28): kernel ns per
-    prime, or why a layout is refused."""
+    """Table sizes m = 2^10 .. 2^14 for the R745-shaped job on the primes of
+    [1e12, 1e12+2^26): one wavefront per prime with its own table (tiles 1,
+    2, 4) and one table shared by the 4 or 8 wavefronts of a workgroup
+    (tiles 1, 2); kernel ns per prime, or why a layout is refused."""
     plan = rs.Plan(rs.Abcd.synth(745, 22, 3, 10**7, 180000, 250000, 10**5, seed=args.seed))
-    gpg = rs.GpuPrimeGen(p0 + (1 << 28) + 1)
-    buf = gpg.device_range(p0, p0 + (1 << 28))
+    gpg = rs.GpuPrimeGen(p0 + (1 << 26) + 1)
+    buf = gpg.device_range(p0, p0 + (1 << 26))
     out = []
+    G4, G8 = rs.TILE_GROUP4, rs.TILE_GROUP8
     for m in (1024, 2048, 4096, 8192, 16384):
-        for tile in (4, 4 | rs.TILE_ROW_INDEX):
+        for tile in (1, 2, 4, 1 | G4, 2 | G4, 1 | G8, 2 | G8):
             rec = {"m": m, "tile": tile}
             try:
                 sv = rs.Sieve(plan, m=m, tile=tile, max_batch=1 << 10, hit_capacity=1 << 20)
@@ -754,12 +756,12 @@ def bench_msweep(args, rs, p0):
             sv.submit_device(p0, buf)
             sv.collect()
             tm = sv.timing()
-            rec.update({"T": sv.T, "kernel": sv.kernel, "lds_bytes_per_wave": sv.lds_bytes,
+            rec.update({"T": sv.T, "kernel": sv.kernel, "lds_bytes": sv.lds_bytes,
                         "ns_per_prime": tm["kernel_ms"] * 1e6 / buf.count})
             sv.close()
             out.append(rec)
-            log(f"msweep: m={m} tile={tile} {rec['ns_per_prime']:.3f} ns/prime "
-                f"({sv.kernel}, {rec['lds_bytes_per_wave']} B)")
+            log(f"msweep: m={m} tile={tile} T={rec['T']} {rec['ns_per_prime']:.3f} ns/prime "
+                f"({rec['kernel']}, {rec['lds_bytes']} B)")
     buf.free()
     return out
 

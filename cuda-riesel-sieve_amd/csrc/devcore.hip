@@ -98,7 +98,9 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     const bool row_index = (tw & RS_TILE_ROW_INDEX) != 0;
     const int nwv = (tw & RS_TILE_GROUP8) ? 8 : (tw & RS_TILE_GROUP4) ? 4 : 0;
     if ((tw & RS_TILE_GROUP8) && (tw & RS_TILE_GROUP4)) return RS_ERR_ARG;
-    if (nwv && (row_index || m % (64 * nwv) != 0)) return RS_ERR_ARG;
+    // shared tables: m in whole strides of the workgroup's lanes, and at
+    // most 16384 (bucket counts are 16-bit: at most 2m inserts per bucket)
+    if (nwv && (row_index || m % (64 * nwv) != 0 || m > 16384)) return RS_ERR_ARG;
     tw &= ~(RS_TILE_ROW_INDEX | RS_TILE_GROUP4 | RS_TILE_GROUP8);
     if (tw < 0) return RS_ERR_ARG;
     if (tw == 0) tw = 2;

@@ -137,7 +137,7 @@ static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W,
 // LDS words of one workgroup whose wavefronts share one prime's table
 // (nwv > 0): the table first, at LDS address 0 (so aligned to its size),
 //   4 NB                      NB buckets x 8 uint16 fingerprints (0 = empty)
-//   NB                        bucket fill counts
+//   NB / 2                    bucket fill counts (uint16)
 //   2 Q ys                    Y entries (uint64)
 //   list_words(nrows, false)  active rows of one slice
 //   2 W                       the prime's record (uint64)
@@ -145,7 +145,7 @@ static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W,
 enum { WG_MISC = 8 };
 static inline int lds_words_group(int logNB, int Q, int ys, int nrows, int W) {
     const int NB = 1 << logNB;
-    int w = 5 * NB + 2 * Q * ys + list_words(nrows, false) + 2 * W + WG_MISC;
+    int w = 4 * NB + NB / 2 + 2 * Q * ys + list_words(nrows, false) + 2 * W + WG_MISC;
     return (w + 3) & ~3;
 }
 // Global scratch of one shared-table workgroup, in uint64 words: the

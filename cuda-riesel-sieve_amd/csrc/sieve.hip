@@ -941,7 +941,7 @@ __device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __re
 template <int NV, int TW, bool CHAIN, bool EXACT>
 __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev* __restrict__ jobs,
                                                   int job, int A, int r0,
-                                                  const u32* __restrict__ cnt,
+                                                  const unsigned short* __restrict__ cnt,
                                                   const char* __restrict__ Yb,
                                                   const uint2* __restrict__ list, const u64* R,
                                                   u32 negp0, const u32* __restrict__ low,
@@ -1034,15 +1034,22 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
     const u32 NB = 1u << J.logNB;
     const u32 nbm = NB - 1;
     const int Q = J.Q, T = J.T, m = J.m, W = J.W;
+    // bucket fill counts as 16-bit halves of 32-bit words (at most 2m <=
+    // 32768 inserts reach one bucket, so a half never carries into the other)
     u32* const cnt = wbase;
-    u64* const Y = (u64*)(cnt + NB);
+    const unsigned short* const cnt16 = (const unsigned short*)cnt;
+    auto cnt_add = [&](u32 b) -> u32 {
+        const u32 sh = (b & 1) << 4;
+        return (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
+    };
+    u64* const Y = (u64*)(cnt + NB / 2);
     uint2* const list = (uint2*)(Y + Q * J.ys);
     u64* const R = (u64*)((u32*)list + list_words(J.nrows, false));
     u32* const misc = (u32*)(R + W);  // [0]: some insert overflowed its bucket
 
     for (int w = tid; w < W; w += 64 * NV) R[w] = Rg[w];
     for (u32 b = tid; b < NB; b += 64 * NV) tab[b] = make_uint4(0, 0, 0, 0);
-    for (u32 c = 4 * tid; c < NB; c += 256 * NV) *(uint4*)(cnt + c) = make_uint4(0, 0, 0, 0);
+    for (u32 c = tid; c < NB / 2; c += 64 * NV) cnt[c] = 0;
     if (tid == 0) misc[0] = 0;
     __syncthreads();
     const u64 p = R[REC_P], mp = R[REC_MP];
@@ -1069,7 +1076,7 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
             ovf = true;
             do {
                 b = (b + 1) & nbm;
-                c = atomicAdd(&cnt[b], 1u);
+                c = cnt_add(b);
             } while (c >= 8);
             ta = 16 * b;
         }
@@ -1080,7 +1087,7 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
     };
     auto insert = [&](u32 lo) {
         const u32 b = bucket_of(lo, (u32)J.logNB);
-        store_fp(place(b, 16 * b, atomicAdd(&cnt[b], 1u)), lo);
+        store_fp(place(b, 16 * b, cnt_add(b)), lo);
     };
     const u64 bound = (p >> 32) * J.kmax + J.kmax + 1;
     const u32 plo = lo32(p);
@@ -1099,7 +1106,7 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
                 bk[u] = bucket_of(lo[u], (u32)J.logNB);
                 ta[u] = 16 * bk[u];
                 asm volatile("" : "+v"(ta[u]));
-                c[u] = atomicAdd(&cnt[bk[u]], 1u);
+                c[u] = cnt_add(bk[u]);
             }
 #pragma unroll
             for (int u = 0; u < BB; ++u) store_fp(place(bk[u], ta[u], c[u]), lo[u]);
@@ -1171,7 +1178,7 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
         __syncthreads();
         const char* const Yb = (const char*)Y;
 #define RS_GIANT_G(TWV, CH, EX)                                                              \
-    giant_steps_group<NV, TWV, CH, EX>(J, jobs, job, A, r0, cnt, Yb, list, R, 0u - lo32(p), \
+    giant_steps_group<NV, TWV, CH, EX>(J, jobs, job, A, r0, cnt16, Yb, list, R, 0u - lo32(p), \
                                        low, hits, hit_count, hit_cap)
 #define RS_GIANT_G_TW(CH, EX)  \
     if (J.tw >= 4)             \

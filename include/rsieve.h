@@ -200,7 +200,8 @@ typedef struct rs_sieve rs_sieve;
  * together with one baby table in the workgroup's LDS, for tables larger
  * than one wavefront's share of LDS allows (jobs with many rows over wide n
  * windows, whose best m runs into the thousands).  m must be a multiple of
- * 64 * 4 or 64 * 8; rs_sieve_info then reports the workgroup's LDS bytes. */
+ * 64 * 4 or 64 * 8 and at most 16384; rs_sieve_info then reports the
+ * workgroup's LDS bytes. */
 #define RS_TILE_GROUP4 32
 #define RS_TILE_GROUP8 64
 /* Job-shape limits (rs_sieve_create, rs_batch_create, rs_autotune and the
