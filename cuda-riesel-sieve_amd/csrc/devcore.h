@@ -34,6 +34,8 @@ struct DevJob {
     uint4* d_kvals = nullptr;    // Jacobi constants of each distinct k
     uint32_t* d_kjt = nullptr;   // quadratic-character table offsets of each k
     uint32_t* d_jtab = nullptr;  // the tables
+    uint4* d_kfac = nullptr;     // per k: its prime-factor entries
+    uint4* d_fac = nullptr;
     uint32_t* d_kw0 = nullptr;   // row-mask words of each k
     uint64_t* d_kmask = nullptr;
     uint2* d_rowent = nullptr;

@@ -28,6 +28,9 @@ void free_job(DevJob& j) {
     (void)hipFree(j.d_kjt);
     (void)hipFree(j.d_jtab);
     j.d_kjt = j.d_jtab = nullptr;
+    (void)hipFree(j.d_kfac);
+    (void)hipFree(j.d_fac);
+    j.d_kfac = j.d_fac = nullptr;
     (void)hipFree(j.d_kw0);
     (void)hipFree(j.d_kmask);
     j.d_kw0 = nullptr;
@@ -41,22 +44,90 @@ void free_job(DevJob& j) {
     j.d_self = nullptr;
 }
 
-// Word offsets of the quadratic-character tables of the distinct k (~0u
-// for k without one) and their total size in words.
-static uint64_t jacobi_table_layout(const std::vector<uint32_t>& kvals,
-                                    std::vector<uint32_t>* offs) {
-    uint64_t total = 0;
-    offs->assign(kvals.size(), 0xffffffffu);
+// Quadratic-character tables of a job: direct tables of the odd parts n of
+// the distinct k (in order of k, n <= JTAB_MAX_N, within JTAB_MAX_WORDS),
+// then for the other k the tables of the primes q <= FTAB_MAX_Q with an odd
+// exponent in n (one per q, within FTAB_MAX_WORDS); what is left of n is its
+// cofactor, for the binary loop.
+struct JacobiLayout {
+    std::vector<uint32_t> direct;  // per k: word offset of n's table, or ~0u
+    std::vector<uint4> kfac;       // per k: first entry, count, largest prime factor, factored
+    std::vector<uint4> fac;        // entries: q, floor((2^64-1)/q) lo, hi, table offset or ~0u
+    std::vector<std::pair<uint32_t, uint32_t>> build;  // (n or q, word offset) to fill
+    uint64_t words = 0;
+};
+
+static uint4 barrett_entry(uint32_t q, uint32_t off) {
+    const uint64_t rn = ~0ull / q;
+    return make_uint4(q, (uint32_t)rn, (uint32_t)(rn >> 32), off);
+}
+
+static JacobiLayout jacobi_layout(const std::vector<uint32_t>& kvals) {
+    JacobiLayout L;
+    L.direct.assign(kvals.size(), 0xffffffffu);
+    L.kfac.assign(kvals.size(), make_uint4(0, 0, 0, 0));
     for (size_t i = 0; i < kvals.size(); ++i) {
         uint32_t n = kvals[i];
         while ((n & 1) == 0) n >>= 1;
         if (n < 3 || n > JTAB_MAX_N) continue;
         const uint64_t w = (n + 31) / 32;
-        if (total + w > JTAB_MAX_WORDS) continue;
-        (*offs)[i] = (uint32_t)total;
-        total += w;
+        if (L.words + w > JTAB_MAX_WORDS) continue;
+        L.direct[i] = (uint32_t)L.words;
+        L.build.push_back({n, (uint32_t)L.words});
+        L.words += w;
     }
-    return total;
+    // factor the odd parts left: the odd primes of odd exponent (a square
+    // factor's character is 1 for every p above it), the largest prime
+    std::vector<std::vector<uint32_t>> odd(kvals.size());
+    std::vector<uint32_t> small;  // odd primes up to sqrt(2^31)
+    for (size_t i = 0; i < kvals.size(); ++i) {
+        uint32_t x = kvals[i];
+        while ((x & 1) == 0) x >>= 1;
+        if (L.direct[i] != 0xffffffffu || x < 3) continue;
+        if (small.empty()) {
+            small = rsh::primes_upto(46341);
+            small.erase(small.begin());  // drop 2
+        }
+        uint32_t top = 1;
+        for (uint32_t q : small) {
+            if ((uint64_t)q * q > x) break;
+            int e = 0;
+            while (x % q == 0) x /= q, ++e;
+            if (e) top = q;
+            if (e & 1) odd[i].push_back(q);
+        }
+        if (x > 1) odd[i].push_back(x), top = x;
+        L.kfac[i] = make_uint4(0, 0, top, 1);
+    }
+    // tables for the smallest of those primes first (the most shared and
+    // the cheapest), while they fit
+    std::vector<uint32_t> qs;
+    for (const auto& f : odd) qs.insert(qs.end(), f.begin(), f.end());
+    std::sort(qs.begin(), qs.end());
+    qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+    std::map<uint32_t, uint32_t> qoff;
+    uint64_t fwords = 0;
+    for (uint32_t q : qs) {
+        const uint64_t w = (q + 31) / 32;
+        if (q > FTAB_MAX_Q || fwords + w > FTAB_MAX_WORDS) break;
+        qoff.emplace(q, (uint32_t)L.words);
+        L.build.push_back({q, (uint32_t)L.words});
+        L.words += w;
+        fwords += w;
+    }
+    for (size_t i = 0; i < kvals.size(); ++i) {
+        if (!L.kfac[i].w) continue;
+        L.kfac[i].x = (uint32_t)L.fac.size();
+        uint32_t cof = 1;
+        for (uint32_t q : odd[i]) {
+            const auto it = qoff.find(q);
+            if (it != qoff.end()) L.fac.push_back(barrett_entry(q, it->second));
+            else cof *= q;
+        }
+        if (cof > 1) L.fac.push_back(barrett_entry(cof, 0xffffffffu));
+        L.kfac[i].y = (uint32_t)L.fac.size() - L.kfac[i].x;
+    }
+    return L;
 }
 
 static std::vector<uint32_t> distinct_k(const rs_plan* pl) {
@@ -68,12 +139,12 @@ static std::vector<uint32_t> distinct_k(const rs_plan* pl) {
 
 size_t job_device_bytes(const rs_plan* pl, bool dat) {
     const size_t rows = pl->rows_k.size();
-    std::vector<uint32_t> offs;
-    const uint64_t tab = jacobi_table_layout(distinct_k(pl), &offs);
+    const JacobiLayout L = jacobi_layout(distinct_k(pl));
     // row tables, per k (at most one per row) Jacobi constants, masks and
-    // table offsets, the tables, and the n-list bitmaps with dat
-    return 8 * rows + 56 * rows + 4 * tab + (dat ? 4 * rows * (size_t)pl->words : 0) + 256 +
-           sizeof(JobDev);
+    // table offsets, factor entries, the tables, and the n-list bitmaps
+    // with dat
+    return 8 * rows + 72 * rows + 16 * L.fac.size() + 4 * L.words +
+           (dat ? 4 * rows * (size_t)pl->words : 0) + 256 + sizeof(JobDev);
 }
 
 // Jacobi constants of a 32-bit a > 0: odd part n, 2-adic valuation z and
@@ -101,7 +172,8 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     // shared tables: m in whole strides of the workgroup's lanes, and at
     // most 16384 (bucket counts are 16-bit: at most 2m inserts per bucket)
     if (nwv && (row_index || m % (64 * nwv) != 0 || m > 16384)) return RS_ERR_ARG;
-    tw &= ~(RS_TILE_ROW_INDEX | RS_TILE_GROUP4 | RS_TILE_GROUP8);
+    const bool sparse = (tw & RS_TILE_SPARSE) != 0;
+    tw &= ~(RS_TILE_ROW_INDEX | RS_TILE_GROUP4 | RS_TILE_GROUP8 | RS_TILE_SPARSE);
     if (tw < 0) return RS_ERR_ARG;
     if (tw == 0) tw = 2;
     if (tw > 4) return RS_ERR_ARG;
@@ -139,7 +211,8 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     const int entries = m + (int)(((uint64_t)m * kvals.back() >> 32) + 1);
     int logNB = 0;
     while ((1 << logNB) * 3 < entries) ++logNB;
-    J.logNB = std::max(logNB, 2);
+    J.logNB = std::max(logNB, 2) + (sparse ? 1 : 0);
+    J.sparse = sparse ? 1 : 0;
     J.ys = J.T + (J.T % 2 == 0 ? 1 : 0);
     J.invT = magic_div(J.T);
     J.tw = tw;
@@ -183,26 +256,33 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
                      hipMemcpyHostToDevice));
     J.kjac = dj->d_kvals;
     {
-        std::vector<uint32_t> toffs;
-        const uint64_t words = jacobi_table_layout(kvals, &toffs);
-        HIPCHK(hipMalloc(&dj->d_kjt, toffs.size() * 4));
-        HIPCHK(hipMemcpy(dj->d_kjt, toffs.data(), toffs.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMalloc(&dj->d_jtab, std::max<uint64_t>(words, 1) * 4));
+        const JacobiLayout TL = jacobi_layout(kvals);
+        HIPCHK(hipMalloc(&dj->d_kjt, TL.direct.size() * 4));
+        HIPCHK(hipMemcpy(dj->d_kjt, TL.direct.data(), TL.direct.size() * 4,
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&dj->d_kfac, TL.kfac.size() * sizeof(uint4)));
+        HIPCHK(hipMemcpy(dj->d_kfac, TL.kfac.data(), TL.kfac.size() * sizeof(uint4),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&dj->d_fac, std::max<size_t>(TL.fac.size(), 1) * sizeof(uint4)));
+        if (!TL.fac.empty())
+            HIPCHK(hipMemcpy(dj->d_fac, TL.fac.data(), TL.fac.size() * sizeof(uint4),
+                             hipMemcpyHostToDevice));
+        if (hipMalloc(&dj->d_jtab, std::max<uint64_t>(TL.words, 1) * 4) != hipSuccess)
+            return RS_ERR_NOMEM;
         hipStream_t bs = nullptr;
         HIPCHK(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
-        for (size_t i = 0; i < kvals.size(); ++i) {
-            if (toffs[i] == 0xffffffffu) continue;
-            uint32_t n = kvals[i];
-            while ((n & 1) == 0) n >>= 1;
-            const uint32_t w = (n + 31) / 32;
-            hipLaunchKernelGGL(rs_jacobi_table, dim3((w + 255) / 256), dim3(256), 0, bs, n, w,
-                               dj->d_jtab + toffs[i]);
+        for (const auto& t : TL.build) {
+            const uint32_t w = (t.first + 31) / 32;
+            hipLaunchKernelGGL(rs_jacobi_table, dim3((w + 255) / 256), dim3(256), 0, bs,
+                               t.first, w, dj->d_jtab + t.second);
         }
         const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(bs);
         (void)hipStreamDestroy(bs);
         if (e1 != hipSuccess || e2 != hipSuccess) return RS_ERR_HIP;
         J.kjt = dj->d_kjt;
         J.jtab = dj->d_jtab;
+        J.kfac = dj->d_kfac;
+        J.fac = dj->d_fac;
     }
     HIPCHK(hipMalloc(&dj->d_kw0, kw0.size() * 4));
     HIPCHK(hipMalloc(&dj->d_kmask, kmask.size() * 8));

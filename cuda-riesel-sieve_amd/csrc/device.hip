@@ -138,7 +138,8 @@ extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table
     if (!s) return RS_ERR_ARG;
     if (tile)
         *tile = s->job.J.tw | (s->job.J.list_idx ? RS_TILE_ROW_INDEX : 0) |
-                (s->job.J.nwv == 4 ? RS_TILE_GROUP4 : s->job.J.nwv == 8 ? RS_TILE_GROUP8 : 0);
+                (s->job.J.nwv == 4 ? RS_TILE_GROUP4 : s->job.J.nwv == 8 ? RS_TILE_GROUP8 : 0) |
+                (s->job.J.sparse ? RS_TILE_SPARSE : 0);
     if (m) *m = s->job.J.m;
     if (T) *T = s->job.J.T;
     if (table_slots) *table_slots = 8 << s->job.J.logNB;
@@ -474,7 +475,7 @@ struct Tuner {
     // small for the x-window: a larger m may fit).
     static const rs_status kSkip = 1;
     rs_status time(int m, int tile, double* ns) {
-        if (!rsh::shape_ok(pl, m, std::max(1, tile & ~RS_TILE_ROW_INDEX))) return kSkip;
+        if (!rsh::shape_ok(pl, m, std::max(1, tile & 7))) return kSkip;
         rs_sieve* s = nullptr;
         rs_status st = rs_sieve_create(device, pl, m, tile, 1, 1 << 16, &s);
         if (st != RS_OK) return st;
@@ -644,6 +645,13 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
     if (groups_worth(pl, tu.best_m)) {
         rs_status st = tune_groups(tu, effort);
         if (st != RS_OK) return st;
+    }
+    // the sparse table at the best pair (fewer bucket overflows for more LDS)
+    if (tu.best_m && !(tu.best_tile & RS_TILE_SPARSE)) {
+        double ns = 0;
+        rs_status st = tu.time(tu.best_m, tu.best_tile | RS_TILE_SPARSE, &ns);
+        if (st != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip && st != RS_ERR_NOMEM)
+            return st;
     }
     if (ncand) *ncand = tu.n;
     *best_m = tu.best_m;

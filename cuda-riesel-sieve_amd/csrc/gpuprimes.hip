@@ -5,7 +5,8 @@
 // Every prime p in [lo, hi) (hi - lo <= 2^32, 3 <= lo, hi <= 2^63) is
 // written as the uint32 offset p - lo, ascending.  The odd numbers of the
 // range are split into segments of SEG_BITS; one workgroup sieves a segment
-// in an LDS bitmap with the odd primes q <= S (S = min(sqrt(hi), 2^22)),
+// in an LDS bitmap with the odd primes q <= S (S = sqrt(hi) up to 2^25,
+// else 2^22),
 // primes below SMALL_Q marked by the whole block and larger ones by one
 // thread each; survivors at or above S^2 are proved by deterministic
 // Miller-Rabin (bases 2, 325, 9375, 28178, 450775, 9780504, 1795265022,
@@ -214,8 +215,12 @@ extern "C" rs_status rs_gpu_primegen_create(int device, uint64_t hi_max, rs_gpu_
     rs_gpu_primegen* g = new rs_gpu_primegen();
     g->device = device;
     g->hi_max = hi_max;
+    // sieving primes up to sqrt(hi_max) when that is at most 2^25 (then no
+    // survivor needs Miller-Rabin: below 2^50 a window of 2^30 costs more
+    // in Miller-Rabin than in sieving by the larger primes); above, MR is
+    // needed anyway and the primes up to 2^22 keep the sieve light
     uint64_t S = (uint64_t)std::sqrt((double)hi_max) + 2;
-    S = std::min<uint64_t>(S, 1ull << 22);
+    if (S > (1ull << 25)) S = 1ull << 22;
     std::vector<uint32_t> ps = rsh::primes_upto((uint32_t)S);
     std::vector<uint32_t> odd(ps.begin() + 1, ps.end());
     g->np = (int)odd.size();

@@ -68,7 +68,13 @@ struct JobDev {
     const uint32_t* kjt;      // per distinct k: word offset of its quadratic-
                               // character table in jtab, or ~0u (no table)
     const uint32_t* jtab;     // bit x of a k's table: (x/n) = +1, x < n, n the
-                              // odd part of k
+                              // odd part of k (and the prime-factor tables)
+    const uint4* kfac;        // per distinct k without a table of its own:
+                              // first factor entry, entry count, the largest
+                              // prime factor of n, 1 (0: not factored)
+    const uint4* fac;         // factor entries: prime q (or the cofactor),
+                              // floor((2^64-1)/q) lo, hi, word offset of q's
+                              // table in jtab (~0u: the cofactor, by the loop)
     const uint32_t* kw0;      // per distinct k: 64-row word holding its first row
     const uint64_t* kmask;    // per distinct k: its even-r rows in words w0 and
                               // w0 + 1, then its odd-r rows in the same words
@@ -81,6 +87,7 @@ struct JobDev {
                     // (RS_TILE_GROUP4 / RS_TILE_GROUP8)
     int off_x;      // record words of the shared-table layout (nwv > 0)
     int lds_words_wg;  // 32-bit LDS words of one workgroup (nwv > 0)
+    int sparse;        // 1: twice the buckets of the load rule (RS_TILE_SPARSE)
 };
 
 // A chunk of primes p = low + offs[i], i < count, of one job; group0 is
@@ -107,7 +114,16 @@ struct HitDev {
 
 // Quadratic-character tables: k whose odd part n is at most JTAB_MAX_N get
 // one, in order of k, while the job's tables stay within JTAB_MAX_WORDS.
-enum : uint32_t { JTAB_MAX_N = 1u << 28, JTAB_MAX_WORDS = 1u << 26 };
+// The odd parts of the other k are factored: the primes q <= FTAB_MAX_Q
+// with an odd exponent get a table of (x/q) each (one per q, shared), the
+// smallest first, while those tables stay within FTAB_MAX_WORDS; the rest
+// (the product of the primes left) goes through the binary Jacobi loop.
+enum : uint32_t {
+    JTAB_MAX_N = 1u << 28,
+    JTAB_MAX_WORDS = 1u << 26,
+    FTAB_MAX_Q = 1u << 28,
+    FTAB_MAX_WORDS = 1u << 27
+};
 
 static inline int rec_words(int Q, int T, int nrows, int nwv = 0) {
     return REC_BH + Q + 2 * T + (nrows + 63) / 64 + (nwv ? nwv + 2 : 0);

@@ -529,6 +529,27 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
             if (r >= kj.x) r -= kj.x;
             const u32 x = (u32)r;
             jkv = (global_ld(J.jtab + tabw + (x >> 5)) >> (x & 31)) & 1 ? t : -t;
+        } else if (const uint4 kf = const_ld(J.kfac + i); kf.w && p > kf.z) {
+            // (k/p) = (2/p)^z prod (q/p) over the primes q of odd exponent in
+            // n, (q/p) = (p mod q / q) (-1)^((q-1)/2 (p-1)/2) from q's table;
+            // the primes without one come as their product, by the loop
+            const u32 p8 = lo32(p) & 7;
+            int t = (kj.y & 1) && (p8 == 3 || p8 == 5) ? -1 : 1;
+            for (u32 f = 0; f < kf.y; ++f) {
+                const uint4 e = const_ld(J.fac + kf.x + f);
+                if (e.w == 0xffffffffu) {
+                    t *= jacobi_pre(e.x, 0, ((u64)e.z << 32) | e.y, p);
+                    continue;
+                }
+                if ((e.x & 3) == 3 && (p8 & 3) == 3) t = -t;
+                const u64 q = umulhi64(p, ((u64)e.z << 32) | e.y);
+                u64 r = p - mad32(lo32(q), e.x, (u64)(hi32(q) * e.x) << 32);
+                if (r >= e.x) r -= e.x;
+                if (r >= e.x) r -= e.x;
+                const u32 x = (u32)r;
+                if (!((global_ld(J.jtab + e.w + (x >> 5)) >> (x & 31)) & 1)) t = -t;
+            }
+            jkv = t;
         } else {
             jkv = jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
         }
@@ -808,7 +829,13 @@ __device__ __forceinline__ void sieve_body(
         const long long cnt =
             C.count_dev ? (long long)(u32)__builtin_amdgcn_readfirstlane(*C.count_dev) : C.count;
         const int n = (int)min(64ll, cnt - q0);
-        if (n <= 0) break;  // device-counted chunk: groups past the end are empty
+        // a device-counted chunk numbers its groups up to a bound and the
+        // groups past its count are empty: past the last chunk the loop ends,
+        // in a table of chunks the next group may belong to the next chunk
+        if (n <= 0) {
+            if (chunks) continue;
+            break;
+        }
 
         // set-up, one lane per prime
         if (lane < n) prep_record(J, C.low + global_ld(C.offs + q0 + lane), S + lane * J.W);

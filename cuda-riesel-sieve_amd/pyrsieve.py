@@ -121,6 +121,7 @@ _lib = None
 TILE_ROW_INDEX = 16
 TILE_GROUP4 = 32
 TILE_GROUP8 = 64
+TILE_SPARSE = 128
 
 class RsError(RuntimeError):
     def __init__(self, what, status):

@@ -156,7 +156,8 @@ rs_status rs_primes_batch(rs_primegen* g, uint64_t low, uint32_t* offsets,
 void rs_primegen_free(rs_primegen* g);
 
 /* Prime generation on the GPU (the step before the sieve): segmented
- * sieve in LDS with the odd primes up to min(sqrt(hi_max), 2^22), and
+ * sieve in LDS with the odd primes up to sqrt(hi_max) when that is at most
+ * 2^25 (no Miller-Rabin at all below 2^50), else up to 2^22 with
  * deterministic Miller-Rabin above that bound squared.  Offsets p - lo go to
  * DEVICE memory.  No reference counterpart (the reference generates on the
  * CPU, :1838-1987). */
@@ -204,6 +205,10 @@ typedef struct rs_sieve rs_sieve;
  * workgroup's LDS bytes. */
 #define RS_TILE_GROUP4 32
 #define RS_TILE_GROUP8 64
+/* Sparse table (tile flag, any layout): twice the buckets, so the average
+ * bucket load halves and fewer primes see a bucket overflow (whose giant
+ * steps then also follow overflow chains), for more LDS. */
+#define RS_TILE_SPARSE 128
 /* Job-shape limits (rs_sieve_create, rs_batch_create, rs_autotune and the
  * server return RS_ERR_ARG outside them; rs_plan_auto_m only picks shapes
  * inside them): with X = floor(maxN/Q) - floor(minN/Q) + 1 and T the giant
@@ -322,12 +327,15 @@ void rs_batch_destroy(rs_batch* b);
 /* ------------------------------------------------------------ serving */
 /* Continuous-batching server on one GPU (no reference counterpart).  Jobs
  * arrive at any time and queue FIFO; each rs_server_step admits queued jobs
- * while a slot is free and their device footprint (row tables, n-list
- * bitmaps with dat, one prime buffer of chunk_width) fits hbm_budget,
- * generates on the GPU the primes of every distinct p-window of the active
- * jobs (jobs at equal progress share one), sieves all active jobs (one
- * persistent launch per occupancy class of their LDS layouts), moves hits
- * to their jobs and retires finished jobs.  Job state stays resident on
+ * while a slot is free (max_active counts jobs with range left to launch)
+ * and their device footprint (row tables, n-list bitmaps with dat, one
+ * prime buffer of chunk_width) fits hbm_budget, queues the next step behind
+ * the one in flight (GPU generation of the primes of every distinct
+ * p-window of the running jobs, jobs at equal progress sharing one, then
+ * one persistent launch per occupancy class of their LDS layouts), then
+ * waits for the step before it, moves its hits to their jobs and retires
+ * finished jobs (freeing their budget).  So the device always holds the next
+ * step while the host admits, plans and drains.  Job state stays resident on
  * the device between steps.  lds_cap_bytes bounds one wavefront's LDS (a
  * job's m is lowered to fit).  A job larger than the whole budget is
  * rejected (state -1).
@@ -355,8 +363,10 @@ rs_status rs_server_job_bytes(const rs_server* s, const rs_plan* pl, int dat,
 rs_status rs_server_submit(rs_server* s, const rs_plan* pl, int m, int tile,
                            int dat, uint64_t p_lo, uint64_t p_hi,
                            double ns_per_prime, int64_t* job_id);
-/* One scheduling step (synchronous).  Reports active and queued jobs after
- * the step and the primes sieved in it. */
+/* One scheduling step: returns once the previous step is drained (the new
+ * one stays queued on the device).  Reports the admitted, unfinished jobs
+ * and the queued ones after the call, and the primes sieved (per job) in the
+ * step it drained. */
 rs_status rs_server_step(rs_server* s, int* active, int* queued,
                          int64_t* primes_step);
 /* state: 0 queued, 1 running, 2 done, -1 rejected; p_done: p-range sieved. */

@@ -50,15 +50,22 @@ def odd_part(k):
 
 
 # k on every path of the set-up's Jacobi code: odd part 1; small and large
-# odd parts with a character table (odd part <= 2^28); odd parts above 2^28
-# (binary Jacobi loop); 2-adic valuations 1, 2, 3, 5, 30; the reference's own
-# k values
+# odd parts with a character table (odd part <= 2^28); odd parts above 2^28,
+# factored: primes with a table of their own (3, 100000007), a prime above
+# 2^28 (binary Jacobi loop), a square factor (3^2 in 2^30 - 1 and
+# 9 * 100000007), a cube (3^3); 2-adic valuations 1, 2, 3, 5, 30; the
+# reference's own k values
 K_PATHS = [1, 3, 5, 7, 2, 4, 8, 1 << 30, 8 * 3**5 * 7, 1000003, 2 * 1000003, 4 * 1000003,
            8 * 1000003, 32 * 12345679, 4 * 268435399, 268435459, 999999937, 2 * 999999937,
-           2147483647, 2147483646, 18300, 29586]
+           2147483647, 2147483646, 18300, 29586, 3 * 100000007, 9 * 100000007,
+           2 * 3 * 100000007, 3 * 268435459, 3**3 * 5 * 7 * 11 * 13 * 17 * 19 * 23,
+           (1 << 30) - 1]
 # nine odd parts just below 2^28: their tables exceed the per-job table
 # budget, so the last ones fall back to the loop
 K_BUDGET = [(1 << 28) - 2 * j - 1 for j in range(9)]
+# three times seventeen primes just below 2^28: their factor tables exceed
+# that budget, so the largest primes go through the loop
+Q_BUDGET = [q for q in range((1 << 28) - 1, (1 << 28) - 4000, -2) if is_prime(q)][:17]
 
 
 def test_k_cases_cover_the_paths():
@@ -68,19 +75,28 @@ def test_k_cases_cover_the_paths():
     assert any(odd_part(k) > 1 << 28 for k in K_PATHS)
     vals = {(k & -k).bit_length() - 1 for k in K_PATHS}
     assert {0, 1, 2, 3, 5, 30} <= vals
-    # words of the character tables of K_BUDGET exceed 2^26
+    # words of the character tables of K_BUDGET exceed 2^26, of the factor
+    # tables of 3 Q_BUDGET 2^27
     assert sum((odd_part(k) + 31) // 32 for k in K_BUDGET) > 1 << 26
+    assert len(Q_BUDGET) == 17 and sum((q + 31) // 32 for q in Q_BUDGET) > 1 << 27
+    assert all(3 * q < 1 << 31 for q in Q_BUDGET)
+    # factored odd parts: with a prime of a table, a prime above 2^28, a
+    # square, a cube
+    big = [odd_part(k) for k in K_PATHS if odd_part(k) > 1 << 28]
+    assert any(n % 100000007 == 0 for n in big) and any(n % 268435459 == 0 for n in big)
+    assert any(n % 9 == 0 and n % 27 for n in big) and any(n % 27 == 0 for n in big)
     ps = sample_primes()
     assert {p % 8 for p in ps if p > 1 << 50} == {1, 3, 5, 7}
     assert any(p < 1 << 50 < q for p, q in zip(ps, ps[1:]))
     assert 1000003 in ps and 5 in ps and 149 in ps  # p | k, p | 745
+    assert 100000007 in ps and 268435459 in ps  # the prime factor of k itself
 
 
 def sample_primes():
     rng = random.Random(99)
     ps = [q for q in range(3, 3000, 2) if is_prime(q)]
     # around the odd parts: p = n (p | k), just below and above
-    for n in (1000003, 12345679, 268435399, 268435459, 999999937):
+    for n in (1000003, 12345679, 100000007, 268435399, 268435459, 999999937):
         ps += [q for q in range(n - 200, n + 200) if q % 2 and is_prime(q)]
     ps.append(1000003)
     for bits in (20, 29, 31, 33, 40, 49, 50, 51, 56, 62, 63):
@@ -148,6 +164,20 @@ def test_row_masks_table_budget_fallback(rs, gpu, c):
     primes = [q for q in sample_primes() if q > 1 << 28][:200] + [3, 5, 7, 11, 13, 1000003]
     plan = plan_of(rs, 745, K_BUDGET, c)
     assert np.array_equal(rs.dev_row_masks(plan, primes), expected_masks(plan, primes))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c", [-1, 1])
+def test_row_masks_factor_table_budget(rs, gpu, c):
+    """Factor tables for the smallest primes of the factored odd parts until
+    their budget runs out, the loop for the rest: the masks equal Euler's
+    criterion either way, for p on both sides of the largest factor."""
+    primes = ([q for q in sample_primes() if q > 1 << 28][:200] + [3, 5, 7, 11, 13]
+              + Q_BUDGET[::4] + [q for q in range(Q_BUDGET[-1] - 100, Q_BUDGET[0] + 100)
+                                 if q % 2 and is_prime(q)][:40])
+    plan = plan_of(rs, 745, [3 * q for q in Q_BUDGET], c)
+    assert np.array_equal(rs.dev_row_masks(plan, sorted(set(primes))),
+                          expected_masks(plan, sorted(set(primes))))
 
 
 def run_oracle(orc, plan, m, low, offs):
