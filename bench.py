@@ -280,7 +280,7 @@ def profile_summary(kernel, workload):
     return out
 
 
-def roofline(rs, plan, sv, tm, launches, nprimes):
+def roofline(rs, plan, sv, tm, launches, nprimes, workload="r745", probes=True):
     """The dominant kernel's roofline line.  The kernel is bound by VALU
     instruction issue (exact 32-bit integer and FP64 arithmetic), neither
     HBM (4 bytes in per prime) nor MFMA.  achieved = algorithmic 32-bit
@@ -297,22 +297,24 @@ def roofline(rs, plan, sv, tm, launches, nprimes):
     launch_s = tm["kernel_ms"] / 1e3 / launches
     achieved = muls * per_launch / launch_s
     published = 78.6e12 / 2
-    mad = rs.dev_mad_rate()
-    copy_bytes = 1 << 30
-    hbm = rs.dev_copy_rate(copy_bytes)
-    prof = profile_summary(sv.kernel, "r745")
+    prof = profile_summary(sv.kernel, workload)
     traffic = prof.get("traffic_bytes_per_prime")
-    return {"bound": "valu", "achieved": achieved / 1e9, "peak": published / 1e9,
-            "unit": "Gmul/s", "frac": achieved / published,
-            "traffic": traffic * per_launch if traffic else None,
-            "traffic_bytes_per_prime": traffic, "profile": prof,
-            "kernel": sv.kernel, "m": sv.m, "T": sv.T, "tile": sv.tile,
-            "muls_per_prime": muls, "primes_per_launch": per_launch,
-            "mean_launch_ms": launch_s * 1e3,
-            "published_valu_peak": published / 1e9,
-            "measured_mad_rate": mad / 1e9, "frac_of_measured_mad_rate": achieved / mad,
-            "hbm_copy_GBps": hbm / 1e9, "copy_probe_bytes": copy_bytes,
-            "hbm_frac": (traffic * per_launch / launch_s / 8e12) if traffic else None}
+    out = {"bound": "valu", "achieved": achieved / 1e9, "peak": published / 1e9,
+           "unit": "Gmul/s", "frac": achieved / published,
+           "traffic": traffic * per_launch if traffic else None,
+           "traffic_bytes_per_prime": traffic, "profile": prof,
+           "kernel": sv.kernel, "m": sv.m, "T": sv.T, "tile": sv.tile,
+           "muls_per_prime": muls, "primes_per_launch": per_launch,
+           "mean_launch_ms": launch_s * 1e3,
+           "published_valu_peak": published / 1e9,
+           "hbm_frac": (traffic * per_launch / launch_s / 8e12) if traffic else None}
+    if probes:
+        mad = rs.dev_mad_rate()
+        copy_bytes = 1 << 30
+        out.update({"measured_mad_rate": mad / 1e9, "frac_of_measured_mad_rate": achieved / mad,
+                    "hbm_copy_GBps": rs.dev_copy_rate(copy_bytes) / 1e9,
+                    "copy_probe_bytes": copy_bytes})
+    return out
 
 
 def cpu_model():
@@ -471,15 +473,19 @@ def bench_edge(args, rs, chunk):
                                      effort=1)
     sv = rs.Sieve(plan, m=m, tile=tile, max_batch=1 << 10, hit_capacity=1 << 20)
     hits, dt, tm = timed_device_chunks(rs, sv, lows, bufs)
+    roof = roofline(rs, plan, sv, tm, K, nprimes, workload="edge", probes=False)
     out = {"workload": f"base 2, 500 odd k in [3,1e9] (seed 3), n in [1e6,2e6] presieved by "
                        f"odd primes < 1e6 ({plan.nrows} rows, Q={plan.Q}), "
                        f"p in [2^62 - {K}*2^30, 2^62)",
            "value": K * chunk / dt, "unit": "p/sec", "primes_per_sec": nprimes / dt,
            "primes": nprimes, "m": sv.m, "tile": sv.tile, "T": sv.T,
-           "lds_bytes_per_wave": sv.lds_bytes, "kernel": sv.kernel, "factors": len(hits),
+           # LDS of one wavefront's layout, of one workgroup's for the
+           # shared-table kernels
+           "lds_bytes": sv.lds_bytes, "kernel": sv.kernel, "factors": len(hits),
            "kernel_ms": tm["kernel_ms"], "gpu_primegen_s": gen_s, "synth_s": synth_s,
-           "autotune": [[cm, ct, round(ns, 3)] for cm, ct, ns in cands]}
-    log(f"edge: m={sv.m} tile={sv.tile} T={sv.T} lds/wave={sv.lds_bytes}B "
+           "autotune": [[cm, ct, round(ns, 3)] for cm, ct, ns in cands],
+           "roofline": roof}
+    log(f"edge: m={sv.m} tile={sv.tile} T={sv.T} lds={sv.lds_bytes}B "
         f"{out['value']:.3e} p/s")
     sv.close()
     for b in bufs:

@@ -30,6 +30,7 @@
 #include <condition_variable>
 #include <ctime>
 #include <deque>
+#include <future>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -67,9 +68,10 @@ struct Batch {
 };
 
 // CPU-generated batches from a producer thread: windows of about 8 batches
-// are sieved at once by rs_primes_range (all host threads) and cut into
-// batches of `size` primes as the reference cuts its prime arrays (:1561),
-// at most 4 batches ahead of the sieve.
+// (at least 2^29 wide) are sieved at once by rs_primes_range (all host
+// threads), one window ahead of the one being cut into batches of `size`
+// primes as the reference cuts its prime arrays (:1561), at most 4 batches
+// ahead of the sieve.
 class BatchFeed {
    public:
     BatchFeed(rs_primegen* g, uint64_t low, uint64_t high, long long size)
@@ -110,43 +112,85 @@ class BatchFeed {
         finished_ = true;
         cv_.notify_all();
     }
+    // Primes of the window [lo, min(lo + width, P1 + 1)) as offsets from lo.
+    struct Window {
+        uint64_t lo = 0, hi = 0;
+        std::vector<uint32_t> offs;
+        rs_status st = RS_OK;
+    };
+    Window sieve_window(uint64_t wlo) {
+        Window w;
+        w.lo = wlo;
+        const double lnp = std::log((double)std::max<uint64_t>(wlo, 16));
+        // at least 2^29 wide, so that each generator thread sieves several
+        // segments per block
+        uint64_t width = (uint64_t)(8.0 * size_ * lnp);
+        width = std::min<uint64_t>(std::max<uint64_t>(width, 1ull << 29), 1ull << 32);
+        w.hi = std::min<uint64_t>(wlo + width, high_ + 1);
+        w.offs.resize((size_t)((w.hi - wlo) / std::max(1.0, lnp - 1.5) * 1.2) + 4096);
+        int64_t got = 0;
+        w.st = rs_primes_range(g_, wlo, w.hi, w.offs.data(), (int64_t)w.offs.size(), &got);
+        if (w.st == RS_ERR_OVERFLOW) {
+            w.offs.resize((size_t)got);
+            w.st = rs_primes_range(g_, wlo, w.hi, w.offs.data(), got, &got);
+        }
+        w.offs.resize(w.st == RS_OK ? (size_t)got : 0);
+        return w;
+    }
     void run() {
-        uint64_t cur = low_, wlo = low_;
-        std::vector<uint64_t> pend;  // generated primes not yet in a batch
-        size_t head = 0;
+        uint64_t cur = low_;   // the next batch's low
+        uint64_t wbase = low_; // the current window's low: its primes are wbase + win[i]
         std::vector<uint32_t> win;
+        size_t head = 0, cnt = 0;
+        // the next window is sieved while the current one is cut and sent
+        std::future<Window> ahead;
+        if (low_ <= high_)
+            ahead = std::async(std::launch::async, [this] { return sieve_window(low_); });
         while (cur <= high_) {
             {
                 std::lock_guard<std::mutex> l(mu_);
                 if (stop_) break;
             }
-            while (pend.size() - head < (size_t)size_ && wlo <= high_) {
-                const double lnp = std::log((double)std::max<uint64_t>(wlo, 16));
-                uint64_t width = (uint64_t)(8.0 * size_ * lnp);
-                width = std::min<uint64_t>(std::max<uint64_t>(width, 1ull << 20), 1ull << 32);
-                const uint64_t whi = std::min<uint64_t>(wlo + width, high_ + 1);
-                win.resize((size_t)((whi - wlo) / std::max(1.0, lnp - 1.5) * 1.2) + 4096);
-                int64_t cnt = 0;
-                rs_status st = rs_primes_range(g_, wlo, whi, win.data(), (int64_t)win.size(), &cnt);
-                if (st == RS_ERR_OVERFLOW) {
-                    win.resize((size_t)cnt);
-                    st = rs_primes_range(g_, wlo, whi, win.data(), cnt, &cnt);
-                }
-                if (st != RS_OK) return finish(st);
-                pend.erase(pend.begin(), pend.begin() + head);
-                head = 0;
-                for (int64_t i = 0; i < cnt; ++i) pend.push_back(wlo + win[i]);
-                wlo = whi;
-            }
             Batch b;
             b.low = cur;
             b.done = false;
-            b.offs.reserve((size_t)size_);
-            while (head < pend.size() && (long long)b.offs.size() < size_ &&
-                   pend[head] - cur < (1ull << 32))
-                b.offs.push_back((uint32_t)(pend[head++] - cur));
-            if (b.offs.empty()) break;  // no prime left in [cur, P1]
-            b.n = (long long)b.offs.size();
+            b.offs.resize((size_t)size_);
+            size_t n = 0;
+            while (n < (size_t)size_) {
+                if (head == cnt) {  // take the next window
+                    if (!ahead.valid()) break;
+                    Window w = ahead.get();
+                    if (w.st != RS_OK) return finish(w.st);
+                    if (w.hi <= high_) {
+                        const uint64_t nlo = w.hi;
+                        ahead = std::async(std::launch::async, [this, nlo] { return sieve_window(nlo); });
+                    }
+                    wbase = w.lo;
+                    win = std::move(w.offs);
+                    head = 0;
+                    cnt = win.size();
+                    continue;
+                }
+                // the window's next primes, while their offsets from the
+                // batch's low fit 32 bits (wbase + win[i] - cur < 2^32)
+                size_t take = std::min((size_t)size_ - n, cnt - head);
+                const uint64_t room = (1ull << 32) + cur - wbase;  // win[i] < room
+                if (room <= 0xffffffffull)
+                    take = std::min<size_t>(
+                        take, std::lower_bound(win.begin() + head, win.begin() + head + take,
+                                               (uint32_t)room) - (win.begin() + head));
+                if (take == 0) break;
+                const uint32_t adj = (uint32_t)(wbase - cur);  // mod 2^32
+                const uint32_t* src = win.data() + head;
+                uint32_t* dst = b.offs.data() + n;
+                for (size_t i = 0; i < take; ++i) dst[i] = src[i] + adj;
+                head += take;
+                n += take;
+                if (head < cnt && n < (size_t)size_) break;  // stopped by the 32-bit bound
+            }
+            if (n == 0) break;  // no prime left in [cur, P1]
+            b.offs.resize(n);
+            b.n = (long long)n;
             b.next = cur + b.offs.back() + 2;
             cur = b.next;
             push(std::move(b));

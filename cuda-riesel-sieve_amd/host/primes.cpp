@@ -10,6 +10,13 @@
 // squared with deterministic Miller-Rabin, and splits the window over
 // threads.  Output keeps the reference's layout: uint32 offsets from an odd
 // uint64 low (:1960-1966).
+//
+// Sieve: odd numbers only, one bit each, in segments of kSegBits; a thread
+// takes blocks of kBlockSegs consecutive segments and carries every sieving
+// prime's next multiple from segment to segment (one remainder per prime
+// per block, by a precomputed reciprocal, not a division per segment).  The multiples of 3, 5, 7, 11 and 13
+// come from a precomputed pattern of period 15015 bits, copied word by word
+// at the segment's phase.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -18,15 +25,29 @@
 
 #include "host.h"
 
+namespace {
+
+const uint32_t kWheel[] = {3, 5, 7, 11, 13};
+const uint32_t kPeriod = 3 * 5 * 7 * 11 * 13;  // 15015 odd numbers
+
+}  // namespace
+
 struct rs_primegen {
-    std::vector<uint32_t> sp;  // odd sieving primes, ascending
-    uint64_t bound2 = 0;       // survivors below this are prime without MR
+    std::vector<uint32_t> sp;   // odd sieving primes above 13, ascending
+    std::vector<uint64_t> pat;  // bit j: 2j + 1 has a factor in kWheel, for
+                                // j < kPeriod + 128 (wrapped)
+    std::vector<uint64_t> rq;   // floor((2^64 - 1) / q) per sieving prime
+    size_t nsmall = 0;          // sieving primes below kSmallQ
+    uint64_t bound2 = 0;        // survivors below this are prime without MR
     int threads = 1;
 };
 
 namespace {
 
 const uint64_t kSegBits = 1u << 21;  // odd numbers per segment (256 KiB of bits)
+const uint64_t kBlockSegs = 16;      // segments per thread block
+const uint64_t kSliceBits = 1u << 18;  // 32 KiB slices for the small primes
+const uint32_t kSmallQ = 1u << 15;
 
 uint64_t isqrt_ceil(uint64_t n) {
     uint64_t r = (uint64_t)std::sqrt((double)n);
@@ -35,36 +56,82 @@ uint64_t isqrt_ceil(uint64_t n) {
     return r;
 }
 
-// Sieves the odd numbers base + 2i, i in [0, nbits), appending primes as
-// offsets from `origin`.
-void sieve_segment(const rs_primegen* g, uint64_t base, uint64_t nbits, uint64_t origin,
-                   std::vector<uint64_t>& bits, std::vector<uint32_t>& out) {
-    const size_t words = (nbits + 63) / 64;
-    bits.assign(words, 0);
-    const uint64_t top = base + 2 * (nbits - 1);
-    for (uint32_t q : g->sp) {
-        const uint64_t qq = (uint64_t)q * q;
-        if (qq > top) break;
-        // first i with q | base + 2i and base + 2i >= q^2
-        uint64_t i;
+// 64 pattern bits starting at bit `pos` (pos < kPeriod + 64).
+inline uint64_t pattern_word(const std::vector<uint64_t>& pat, uint64_t pos) {
+    const uint64_t w = pos >> 6, sh = pos & 63;
+    return sh ? (pat[w] >> sh) | (pat[w + 1] << (64 - sh)) : pat[w];
+}
+
+// Sieves nseg segments of odd numbers from the odd `base` (nbits_total
+// bits in all), appending primes as offsets from `origin`.
+void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbits_total, uint64_t origin,
+                 std::vector<uint64_t>& bits, std::vector<uint64_t>& next,
+                 std::vector<uint32_t>& out) {
+    const uint64_t top = base + 2 * (nbits_total - 1);
+    // sieving primes that reach the block, and the bit index of the first
+    // multiple of each at or above both base and q^2
+    size_t np = 0;
+    while (np < g->sp.size() && (uint64_t)g->sp[np] * g->sp[np] <= top) ++np;
+    next.resize(np);
+    for (size_t i = 0; i < np; ++i) {
+        const uint64_t q = g->sp[i], qq = q * q;
         if (qq >= base) {
-            i = (qq - base) / 2;
+            next[i] = (qq - base) / 2;
         } else {
-            const uint64_t r = base % q;
-            i = ((q - r) % q) * (uint64_t)((q + 1) / 2) % q;
+            // r = base mod q by the reciprocal (the estimated quotient is
+            // short by at most one); base + d == 0 mod q for d = q - r (or
+            // 0), and base + 2j for j = d / 2, or (d + q) / 2 with d odd
+            uint64_t r = base - (uint64_t)(((rsh::u128)base * g->rq[i]) >> 64) * q;
+            if (r >= q) r -= q;
+            const uint64_t d = r ? q - r : 0;
+            next[i] = (d & 1) ? (d + q) / 2 : d / 2;
         }
-        for (; i < nbits; i += q) bits[i >> 6] |= 1ull << (i & 63);
     }
-    for (size_t w = 0; w < words; ++w) {
-        uint64_t free = ~bits[w];
-        if (w == words - 1 && (nbits & 63)) free &= (1ull << (nbits & 63)) - 1;
-        while (free) {
-            const int b = __builtin_ctzll(free);
-            free &= free - 1;
-            const uint64_t v = base + 2 * (w * 64 + b);
-            if (v < 3) continue;  // 1 is not prime (2 is never in an odd window)
-            if (v >= g->bound2 && !rsh::is_prime_u64(v)) continue;
-            out.push_back((uint32_t)(v - origin));
+    for (uint64_t done = 0; done < nbits_total; done += kSegBits) {
+        const uint64_t nbits = std::min(kSegBits, nbits_total - done);
+        const uint64_t b0 = base + 2 * done;
+        const size_t words = (nbits + 63) / 64;
+        bits.resize(words);
+        // wheel primes from the pattern: bit i is 2j + 1 = b0 + 2i, j = (b0 - 1) / 2 + i
+        uint64_t pos = ((b0 - 1) / 2) % kPeriod;
+        for (size_t w = 0; w < words; ++w) {
+            bits[w] = pattern_word(g->pat, pos);
+            pos += 64;
+            if (pos >= kPeriod) pos -= kPeriod;
+        }
+        if (b0 <= 13)  // the wheel primes themselves
+            for (uint32_t q : kWheel)
+                if (q >= b0 && (q - b0) / 2 < nbits) bits[(q - b0) / 2 >> 6] &= ~(1ull << ((q - b0) / 2 & 63));
+        // primes below kSmallQ (many marks each) a cache-sized slice at a
+        // time, the rest (at most a few marks each) over the whole segment
+        const size_t ns = std::min(np, g->nsmall);
+        for (uint64_t s0 = 0; s0 < nbits; s0 += kSliceBits) {
+            const uint64_t s1 = std::min(nbits, s0 + kSliceBits);
+            for (size_t i = 0; i < ns; ++i) {
+                const uint64_t q = g->sp[i];
+                uint64_t j = next[i];
+                for (; j < s1; j += q) bits[j >> 6] |= 1ull << (j & 63);
+                next[i] = j;
+            }
+        }
+        for (size_t i = 0; i < ns; ++i) next[i] -= nbits;
+        for (size_t i = ns; i < np; ++i) {
+            const uint64_t q = g->sp[i];
+            uint64_t j = next[i];
+            for (; j < nbits; j += q) bits[j >> 6] |= 1ull << (j & 63);
+            next[i] = j - nbits;
+        }
+        for (size_t w = 0; w < words; ++w) {
+            uint64_t free = ~bits[w];
+            if (w == words - 1 && (nbits & 63)) free &= (1ull << (nbits & 63)) - 1;
+            while (free) {
+                const int b = __builtin_ctzll(free);
+                free &= free - 1;
+                const uint64_t v = b0 + 2 * (w * 64 + b);
+                if (v < 3) continue;  // 1 is not prime (2 is never in an odd window)
+                if (v >= g->bound2 && !rsh::is_prime_u64(v)) continue;
+                out.push_back((uint32_t)(v - origin));
+            }
         }
     }
 }
@@ -89,8 +156,19 @@ extern "C" rs_status rs_primegen_create(int small_prime_scale, uint64_t hi, int 
         }
         lim *= 2;
     }
-    const uint64_t last = g->sp.back();
-    g->bound2 = last * last;
+    // every prime <= lim sieves, so a survivor below (lim + 1)^2 is prime
+    g->bound2 = (lim + 1) * (lim + 1);
+    // the wheel primes are marked by the pattern
+    g->sp.erase(g->sp.begin(), std::upper_bound(g->sp.begin(), g->sp.end(), 13u));
+    g->nsmall = std::lower_bound(g->sp.begin(), g->sp.end(), kSmallQ) - g->sp.begin();
+    g->rq.resize(g->sp.size());
+    for (size_t i = 0; i < g->sp.size(); ++i) g->rq[i] = ~0ull / g->sp[i];
+    g->pat.assign((kPeriod + 128 + 63) / 64 + 1, 0);
+    for (uint32_t j = 0; j < kPeriod + 128; ++j) {
+        const uint32_t v = 2 * (j % kPeriod) + 1;
+        for (uint32_t q : kWheel)
+            if (v % q == 0) g->pat[j >> 6] |= 1ull << (j & 63);
+    }
     *out = g;
     return RS_OK;
 }
@@ -112,20 +190,24 @@ extern "C" rs_status rs_primes_range(rs_primegen* g, uint64_t lo, uint64_t hi,
     }
     const uint64_t nodd = (hi - base + 1) / 2;
     const uint64_t nseg = (nodd + kSegBits - 1) / kSegBits;
-    std::vector<std::vector<uint32_t>> parts(nseg);
+    // blocks of kBlockSegs segments, fewer when that leaves threads idle
+    const uint64_t bsegs =
+        std::max<uint64_t>(1, std::min<uint64_t>(kBlockSegs, nseg / std::max(1, g->threads)));
+    const uint64_t nblk = (nseg + bsegs - 1) / bsegs;
+    std::vector<std::vector<uint32_t>> parts(nblk);
     std::atomic<uint64_t> next(0);
     std::vector<std::thread> th;
-    const int nt = (int)std::min<uint64_t>(g->threads, nseg);
-    for (int t = 0; t < nt; ++t) {
-        th.emplace_back([&]() {
-            std::vector<uint64_t> bits;
-            for (uint64_t s; (s = next.fetch_add(1)) < nseg;) {
-                const uint64_t b0 = base + 2 * s * kSegBits;
-                const uint64_t nb = std::min<uint64_t>(kSegBits, nodd - s * kSegBits);
-                sieve_segment(g, b0, nb, lo, bits, parts[s]);
-            }
-        });
-    }
+    const int nt = (int)std::min<uint64_t>(g->threads, nblk);
+    auto work = [&]() {
+        std::vector<uint64_t> bits, carry;
+        for (uint64_t k; (k = next.fetch_add(1)) < nblk;) {
+            const uint64_t b0 = base + 2 * k * bsegs * kSegBits;
+            const uint64_t nb = std::min<uint64_t>(bsegs * kSegBits, nodd - k * bsegs * kSegBits);
+            sieve_block(g, b0, nb, lo, bits, carry, parts[k]);
+        }
+    };
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();  // the calling thread takes blocks too
     for (auto& x : th) x.join();
     int64_t total = (int64_t)two.size();
     for (auto& p : parts) total += (int64_t)p.size();

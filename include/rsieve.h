@@ -219,13 +219,16 @@ typedef struct rs_sieve rs_sieve;
  *   min(rows, 512) * T + 4096 < 65536 (a row slice's giant steps are
  *     numbered in 16 bits),
  *   at most 65535 distinct k,
- * and the layout of one wavefront (table, Y table, row list, record; see
- * rs_sieve_info) must fit the device's LDS.  A small m raises T: a wide
- * n window with many rows needs a larger m. */
+ * and the layout of one wavefront (table, Y table, row list, record; with
+ * RS_TILE_GROUP4/8 that of one workgroup; see rs_sieve_info) must fit the
+ * device's LDS.  A small m raises T: a wide n window with many rows needs a
+ * larger m. */
 
 rs_status rs_sieve_create(int device, const rs_plan* pl, int m, int tile,
                           int64_t max_batch, int64_t hit_capacity,
                           rs_sieve** out);
+/* lds_bytes_per_wave: the LDS bytes of one wavefront's layout, or of one
+ * workgroup's for the shared-table layouts (RS_TILE_GROUP4/8). */
 rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table_slots,
                         int* lds_bytes_per_wave, int* tile);
 /* Name of the kernel build the job's launches run (static string; "" for

@@ -745,6 +745,38 @@ def test_cli_gpu_generated_primes(tmp_path, gpu):
     assert outs[0] == outs[1] and f"{KNOWN_P} | 18300*745^242670-1" in outs[1]
 
 
+def test_cli_batches_across_windows(rs, tmp_path, gpu):
+    """The CLI's CPU-generated batches (16384 or 65536 primes, cut from
+    windows of about eight batches, so many batches span two windows) cover
+    every prime of [P0, P1] once: a job with thousands of factors gives the
+    same lines as the GPU-generated run and as the library's own sieve,
+    verified and in the n-lists by default, every datless hit with -v (the
+    reference's "don't verify")."""
+    abcd = rs.Abcd.synth(5, 10, 3, 10**5, 2000, 9000, 200, seed=4, odd_k=2)
+    path = tmp_path / "in.abcd"
+    abcd.write(path)
+    lo, hi = 10**6 + 1, 2 * 10**7
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+    outs = []
+    runs = (["-b", "1"], ["-b", "3"], ["-g"], ["-b", "1", "-v"])
+    for extra in runs:
+        out = tmp_path / f"f{len(outs)}.txt"
+        r = subprocess.run([exe, "-i", str(path), "-p", str(lo), "-P", str(hi), "-m", "8",
+                            "-O", str(out)] + extra, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, (extra, r.stderr)
+        outs.append(sorted(out.read_text().splitlines()))
+    plan = rs.Plan(abcd)
+    offs = rs.PrimeGen(hi + 1).range(lo, hi + 1)
+    hits = run_device(rs, plan, lo, offs, m=512)
+    every = sorted(rs.format_factor(p, k, 5, n).strip() for p, k, n in hits)
+    listed = sorted(rs.format_factor(p, k, 5, n).strip() for p, k, n in hits
+                    if plan.contains(k, n))
+    assert len(listed) > 1000 and len(every) > len(listed)
+    for extra, got in zip(runs, outs):
+        want = every if "-v" in extra else listed
+        assert got == want, (extra, len(got), len(want), sorted(set(got) ^ set(want))[:5])
+
+
 @pytest.mark.parametrize("quantum_ms", [0.0, 0.05])
 def test_server_results_alone_and_in_the_mix(rs, orc, gpu, quantum_ms):
     """Jobs arriving between steps are admitted as slots and budget free up;

@@ -136,6 +136,23 @@ def test_primes_range_matches_oracle(rs, orc):
         assert np.array_equal(got, want), (lo, hi)
 
 
+@pytest.mark.parametrize("threads", [1, 4])
+def test_primes_range_across_segments_and_blocks(rs, orc, threads):
+    """A window of 32 segments of 2^21 odd numbers (blocks of 16 or 8 of
+    them, each prime's next multiple carried from segment to segment), with
+    sieving primes to 1.01e7 and an odd, ragged end, equals the oracle; so
+    do pieces of it cut at arbitrary points."""
+    lo = 102297149770000 + 1
+    hi = lo + (1 << 27) - 77
+    g = rs.PrimeGen(hi, threads=threads)
+    want = orc.true_primes(lo, hi)
+    got = g.range(lo, hi)
+    assert np.array_equal(got, want)
+    cuts = [lo, lo + 12345, lo + (1 << 22) + 1, lo + (5 << 22) - 3, hi]
+    parts = [c0 - lo + g.range(c0, c1).astype(np.int64) for c0, c1 in zip(cuts, cuts[1:])]
+    assert np.array_equal(np.concatenate(parts), want.astype(np.int64))
+
+
 def test_primes_batch_is_reference_layout(rs, orc):
     g = rs.PrimeGen(10**12 + 2**24, threads=4)
     low = 10**12 + 1
