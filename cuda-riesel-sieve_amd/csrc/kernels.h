@@ -155,13 +155,18 @@ static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W,
 //   4 NB                      NB buckets x 8 uint16 fingerprints (0 = empty)
 //   NB / 2                    bucket fill counts (uint16)
 //   2 Q ys                    Y entries (uint64)
-//   list_words(nrows, false)  active rows of one slice
+//   group_list_words(nrows)   active rows of one slice of GROUP_SLICE rows,
+//                             as uint16 row numbers from the slice's start
 //   2 W                       the prime's record (uint64)
 //   WG_MISC                   the group number and the chain flag
-enum { WG_MISC = 8 };
+enum { WG_MISC = 8, GROUP_SLICE = 4096 };
+__host__ __device__ inline int group_list_words(int nrows) {
+    const int n = nrows < GROUP_SLICE ? nrows : GROUP_SLICE;
+    return (n + 3) / 4 * 2;  // even: the record is 8-aligned
+}
 static inline int lds_words_group(int logNB, int Q, int ys, int nrows, int W) {
     const int NB = 1 << logNB;
-    int w = 4 * NB + NB / 2 + 2 * Q * ys + list_words(nrows, false) + 2 * W + WG_MISC;
+    int w = 4 * NB + NB / 2 + 2 * Q * ys + group_list_words(nrows) + 2 * W + WG_MISC;
     return (w + 3) & ~3;
 }
 // Global scratch of one shared-table workgroup, in uint64 words: the

@@ -901,13 +901,13 @@ __device__ __forceinline__ void prep_group_powers(const JobDev& J, u64* __restri
 }
 
 // Emits every baby step j whose value equals the exact product of a giant
-// step item (entry (k, yent) of the slice starting at row r0, step tt, Y
-// entry yv): the lanes of one wavefront compare the low words `low` (lane l
-// reads j = l + 64 v) and confirm a low-word match exactly, by g^j from the
-// record (square-and-multiply, in the form of the baby values).
+// step item (entry (k, yent) of row `row`, step tt, Y entry yv): the lanes
+// of one wavefront compare the low words `low` (lane l reads j = l + 64 v)
+// and confirm a low-word match exactly, by g^j from the record
+// (square-and-multiply, in the form of the baby values).
 __device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __restrict__ jobs,
                                              int job, const u64* R, u64 yv, u32 k, u32 yent,
-                                             int r0, int tt, const u32* __restrict__ low,
+                                             int row, int tt, const u32* __restrict__ low,
                                              HitDev* __restrict__ hits,
                                              u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
@@ -935,7 +935,6 @@ __device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __re
             }
             if (gj != target) continue;
             const u32 r = ((yent & 0x7ffffu) >> 3) / (u32)J.ys;
-            const u32 row = (u32)r0 + (yent >> 19);
             const JobDev* const jp = jobs + job;
             const int x0 = const_ld(&jp->x0);
             const long long xx = (long long)x0 + (long long)tt * J.m + (J.m - 1 - j);
@@ -964,14 +963,18 @@ __device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __re
 }
 
 // Giant steps of one prime over the lanes of the whole workgroup: as
-// giant_steps, with 64 NV runs per round and the table at LDS address 0.
+// giant_steps, with 64 NV runs per round, the table at LDS address 0, and
+// the active rows of the slice starting at row r0 given as uint16 numbers
+// (row - r0) in list16; each lane reads its row's entry from the job's row
+// table in global memory (a few KB, cache-resident), one round ahead.
 template <int NV, int TW, bool CHAIN, bool EXACT>
 __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev* __restrict__ jobs,
                                                   int job, int A, int r0,
                                                   const unsigned short* __restrict__ cnt,
                                                   const char* __restrict__ Yb,
-                                                  const uint2* __restrict__ list, const u64* R,
-                                                  u32 negp0, const u32* __restrict__ low,
+                                                  const uint16_t* __restrict__ list16,
+                                                  const u64* R, u32 negp0,
+                                                  const u32* __restrict__ low,
                                                   HitDev* __restrict__ hits,
                                                   u32* __restrict__ hit_count, u32 hit_cap) {
     const int lid = threadIdx.x;  // lane of the workgroup
@@ -983,11 +986,26 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
     int a = div_T((u32)(TW * lid), J.invT);
     int t0 = TW * lid - a * T;
     const int rstep = div_T(64u * NV * TW, J.invT), tstep = 64 * NV * TW - rstep * T;
+    const uint2* const rowent = J.rowent + r0;
+    int rnum = list16[min(a, A - 1)];
+    uint2 rown = global_ld(rowent + rnum);
     for (int base = 0; base < total; base += 64 * NV * TW) {
         const bool ok = a < A;
         const u64 okm = lanes(ok);
-        const uint2 row = list[min(a, A - 1)];
-        const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * t0);
+        const uint2 row = rown;
+        const int rcur = rnum;
+        const int tcur = t0;
+        // the next round's run, and its row entry fetched now
+        a += rstep;
+        if (!EXACT) {
+            t0 += tstep;
+            if (t0 >= T) t0 -= T, a += 1;
+        }
+        if (base + 64 * NV * TW < total) {
+            rnum = list16[min(a, A - 1)];
+            rown = global_ld(rowent + rnum);
+        }
+        const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * tcur);
         uint4 e[TW];
         u32 fp2[TW], ba[TW];
 #pragma unroll
@@ -1032,16 +1050,12 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
                 mb &= mb - 1;
                 const u32 k = __builtin_amdgcn_readlane(row.x, src);
                 const u32 yent = __builtin_amdgcn_readlane(row.y, src);
-                const int tt = __builtin_amdgcn_readlane(t0, src) + u;
+                const int tt = __builtin_amdgcn_readlane(tcur, src) + u;
+                const int rw = r0 + __builtin_amdgcn_readlane(rcur, src);
                 const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
-                rescan_group(J, jobs, job, R, yv, k, yent, r0, tt, low, hits, hit_count,
+                rescan_group(J, jobs, job, R, yv, k, yent, rw, tt, low, hits, hit_count,
                              hit_cap);
             }
-        }
-        a += rstep;
-        if (!EXACT) {
-            t0 += tstep;
-            if (t0 >= T) t0 -= T, a += 1;
         }
     }
 }
@@ -1070,8 +1084,8 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
         return (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
     };
     u64* const Y = (u64*)(cnt + NB / 2);
-    uint2* const list = (uint2*)(Y + Q * J.ys);
-    u64* const R = (u64*)((u32*)list + list_words(J.nrows, false));
+    uint16_t* const list16 = (uint16_t*)(Y + Q * J.ys);
+    u64* const R = (u64*)((u32*)list16 + group_list_words(J.nrows));
     u32* const misc = (u32*)(R + W);  // [0]: some insert overflowed its bucket
 
     for (int w = tid; w < W; w += 64 * NV) R[w] = Rg[w];
@@ -1185,11 +1199,11 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
     __syncthreads();  // table, low words and Y complete
     const bool chain = misc[0] != 0;
 
-    for (int r0 = 0; r0 < J.nrows; r0 += ROW_SLICE) {
-        const int r1 = min(J.nrows, r0 + ROW_SLICE);
+    for (int r0 = 0; r0 < J.nrows; r0 += GROUP_SLICE) {
+        const int r1 = min(J.nrows, r0 + GROUP_SLICE);
         const int nw = (r1 - r0 + 63) >> 6;
         // mask words of the slice; wavefront w compacts words w, w + NV, ...
-        // after the rows of the words before them
+        // after the rows of the words before them, as row numbers from r0
         int A = 0;
         for (int w = 0; w < nw; ++w) {
             const u64 mask = R[J.off_j + (r0 >> 6) + w];
@@ -1197,15 +1211,16 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
                 if (in_mask(mask)) {
                     const u32 below = __builtin_amdgcn_mbcnt_hi(
                         hi32(mask), __builtin_amdgcn_mbcnt_lo(lo32(mask), 0));
-                    list[A + below] = global_ld(J.rowent + r0 + 64 * w + lane);
+                    list16[A + below] = (uint16_t)(64 * w + lane);
                 }
             }
             A += __popcll(mask);
         }
         __syncthreads();
+        if (A == 0) continue;  // no barrier needed: the list was not read
         const char* const Yb = (const char*)Y;
-#define RS_GIANT_G(TWV, CH, EX)                                                              \
-    giant_steps_group<NV, TWV, CH, EX>(J, jobs, job, A, r0, cnt16, Yb, list, R, 0u - lo32(p), \
+#define RS_GIANT_G(TWV, CH, EX)                                                                \
+    giant_steps_group<NV, TWV, CH, EX>(J, jobs, job, A, r0, cnt16, Yb, list16, R, 0u - lo32(p), \
                                        low, hits, hit_count, hit_cap)
 #define RS_GIANT_G_TW(CH, EX)  \
     if (J.tw >= 4)             \
