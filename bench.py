@@ -473,7 +473,7 @@ def bench_edge(args, rs, chunk):
                                      effort=1)
     sv = rs.Sieve(plan, m=m, tile=tile, max_batch=1 << 10, hit_capacity=1 << 20)
     hits, dt, tm = timed_device_chunks(rs, sv, lows, bufs)
-    roof = roofline(rs, plan, sv, tm, K, nprimes, workload="edge", probes=False)
+    roof = roofline(rs, plan, sv, tm, K, nprimes, workload="edge")
     out = {"workload": f"base 2, 500 odd k in [3,1e9] (seed 3), n in [1e6,2e6] presieved by "
                        f"odd primes < 1e6 ({plan.nrows} rows, Q={plan.Q}), "
                        f"p in [2^62 - {K}*2^30, 2^62)",
