@@ -225,7 +225,7 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.off_x = J.off_j + (nrows + 63) / 64;
     J.lds_words = lds_words_per_wave(J.logNB, J.Q, J.ys, nrows, J.W, false);
     J.lds_words_idx = lds_words_per_wave(J.logNB, J.Q, J.ys, nrows, J.W, true);
-    J.lds_words_wg = nwv ? lds_words_group(J.logNB, J.Q, J.ys, nrows, J.W) : 0;
+    J.lds_words_wg = nwv ? lds_words_group(J.logNB, J.Q, J.ys, J.W) : 0;
     J.list_idx = row_index && nrows <= ROW_SLICE ? 1 : 0;
     J.tab_words = 4 << J.logNB;
     // per k: the 64-row word of its first row and its even-r / odd-r rows
@@ -408,7 +408,7 @@ rs_status launch_geometry_group(int device, const JobDev& J, Launcher* L) {
     L->shared_words = 0;
     L->lds_bytes = bytes;
     L->grid = cus * per_cu;
-    L->scratch_words = (int)scratch_words_group(J.nwv, J.W, J.m);
+    L->scratch_words = (int)scratch_words_group(J.nwv, J.W, J.m, J.nrows);
     return RS_OK;
 }
 

@@ -155,23 +155,24 @@ static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W,
 //   4 NB                      NB buckets x 8 uint16 fingerprints (0 = empty)
 //   NB / 2                    bucket fill counts (uint16)
 //   2 Q ys                    Y entries (uint64)
-//   group_list_words(nrows)   active rows of one slice of GROUP_SLICE rows,
-//                             as uint16 row numbers from the slice's start
 //   2 W                       the prime's record (uint64)
 //   WG_MISC                   the group number and the chain flag
 enum { WG_MISC = 8, GROUP_SLICE = 4096 };
-__host__ __device__ inline int group_list_words(int nrows) {
-    const int n = nrows < GROUP_SLICE ? nrows : GROUP_SLICE;
-    return (n + 3) / 4 * 2;  // even: the record is 8-aligned
-}
-static inline int lds_words_group(int logNB, int Q, int ys, int nrows, int W) {
+static inline int lds_words_group(int logNB, int Q, int ys, int W) {
     const int NB = 1 << logNB;
-    int w = 4 * NB + NB / 2 + 2 * Q * ys + group_list_words(nrows) + 2 * W + WG_MISC;
+    int w = 4 * NB + NB / 2 + 2 * Q * ys + 2 * W + WG_MISC;
     return (w + 3) & ~3;
 }
+// uint64 words of a shared-table workgroup's list of active rows: one slice
+// of GROUP_SLICE rows, as uint16 row numbers from the slice's start.
+__host__ __device__ inline int group_list_words(int nrows) {
+    const int n = nrows < GROUP_SLICE ? nrows : GROUP_SLICE;
+    return (n + 3) / 4;
+}
 // Global scratch of one shared-table workgroup, in uint64 words: the
-// records of its 64 nwv primes, then the low words of the current prime's m
-// baby-step values (uint32, in order of j) for exact rescans.
-static inline long long scratch_words_group(int nwv, int W, int m) {
-    return 64ll * nwv * W + (m + 1) / 2;
+// records of its 64 nwv primes, the low words of the current prime's m
+// baby-step values (uint32, in order of j) for exact rescans, then its
+// list of active rows.
+static inline long long scratch_words_group(int nwv, int W, int m, int nrows) {
+    return 64ll * nwv * W + (m + 1) / 2 + group_list_words(nrows);
 }

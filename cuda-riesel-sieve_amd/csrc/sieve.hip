@@ -57,11 +57,11 @@ __device__ __forceinline__ u64 readlane64(u64 x, int l) {
     return ((u64)hi << 32) | lo;
 }
 
-// Loads from the job's tables, which no kernel writes: through the
-// constant address space for wave-uniform addresses (scalar loads), and
-// the global one for per-lane addresses (instead of flat loads, which
-// also count as LDS operations in the wait counters).  Vector types go
-// through plain integer loads.
+// Loads from the job's tables, which no kernel writes, through the
+// constant address space for wave-uniform addresses (scalar loads); and
+// per-lane loads through the global address space (instead of flat loads,
+// which also count as LDS operations in the wait counters).  Vector types
+// go through plain integer loads.
 template <typename T>
 __device__ __forceinline__ T const_ld(const T* p) {
     return *(const __attribute__((address_space(4))) T*)p;
@@ -963,16 +963,17 @@ __device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __re
 }
 
 // Giant steps of one prime over the lanes of the whole workgroup: as
-// giant_steps, with 64 NV runs per round, the table at LDS address 0, and
-// the active rows of the slice starting at row r0 given as uint16 numbers
-// (row - r0) in list16; each lane reads its row's entry from the job's row
-// table in global memory (a few KB, cache-resident), one round ahead.
+// giant_steps, with 64 NV runs per round and the table at LDS address 0.
+// The active rows of the slice starting at row r0 are uint16 numbers
+// (row - r0) in list16, in the workgroup's global scratch; each lane reads
+// its row number two rounds ahead and its row's entry from the job's row
+// table (a few KB, cache-resident) one round ahead.
 template <int NV, int TW, bool CHAIN, bool EXACT>
 __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev* __restrict__ jobs,
                                                   int job, int A, int r0,
                                                   const unsigned short* __restrict__ cnt,
                                                   const char* __restrict__ Yb,
-                                                  const uint16_t* __restrict__ list16,
+                                                  const uint16_t* list16,
                                                   const u64* R, u32 negp0,
                                                   const u32* __restrict__ low,
                                                   HitDev* __restrict__ hits,
@@ -983,28 +984,39 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
     asm volatile("" : "+v"(bmask));
     const int T = J.T;
     const int total = A * T;
+    const int per = 64 * NV * TW;  // items per round
+    const int rstep = div_T((u32)per, J.invT), tstep = per - rstep * T;
+    auto advance = [&](int& aa, int& tt) {
+        aa += rstep;
+        if (!EXACT) {
+            tt += tstep;
+            if (tt >= T) tt -= T, aa += 1;
+        }
+    };
+    const uint2* const rowent = J.rowent + r0;
+    // runs of this round (a, t0) and the next (a1, t1)
     int a = div_T((u32)(TW * lid), J.invT);
     int t0 = TW * lid - a * T;
-    const int rstep = div_T(64u * NV * TW, J.invT), tstep = 64 * NV * TW - rstep * T;
-    const uint2* const rowent = J.rowent + r0;
-    int rnum = list16[min(a, A - 1)];
+    int a1 = a, t1 = t0;
+    advance(a1, t1);
+    int rnum = global_ld(list16 + min(a, A - 1));
+    int rnum1 = per < total ? global_ld(list16 + min(a1, A - 1)) : 0;
     uint2 rown = global_ld(rowent + rnum);
-    for (int base = 0; base < total; base += 64 * NV * TW) {
+    for (int base = 0; base < total; base += per) {
         const bool ok = a < A;
         const u64 okm = lanes(ok);
         const uint2 row = rown;
         const int rcur = rnum;
         const int tcur = t0;
-        // the next round's run, and its row entry fetched now
-        a += rstep;
-        if (!EXACT) {
-            t0 += tstep;
-            if (t0 >= T) t0 -= T, a += 1;
-        }
-        if (base + 64 * NV * TW < total) {
-            rnum = list16[min(a, A - 1)];
-            rown = global_ld(rowent + rnum);
-        }
+        // the row number two rounds ahead, the row entry one round ahead
+        int a2 = a1, t2 = t1;
+        advance(a2, t2);
+        const int rnum2 = base + 2 * per < total ? global_ld(list16 + min(a2, A - 1)) : 0;
+        if (base + per < total) rown = global_ld(rowent + rnum1);
+        rnum = rnum1;
+        rnum1 = rnum2;
+        a = a1, t0 = t1;
+        a1 = a2, t1 = t2;
         const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * tcur);
         uint4 e[TW];
         u32 fp2[TW], ba[TW];
@@ -1063,12 +1075,12 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
 // BSGS of one prime by the NV wavefronts of the workgroup.  Rg: the
 // prime's record in global scratch; the table at LDS address 0, the rest of
 // the layout (lds_words_group) from wbase; low: the workgroup's m low
-// words in global scratch.
+// words and list16 its row list, both in its global scratch.
 template <int NV, int BB>
 __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* __restrict__ jobs,
                                                  int job, const u64* __restrict__ Rg,
                                                  uint4* __restrict__ tab, u32* __restrict__ wbase,
-                                                 u32* __restrict__ low,
+                                                 u32* __restrict__ low, uint16_t* list16,
                                                  HitDev* __restrict__ hits,
                                                  u32* __restrict__ hit_count, u32 hit_cap) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1084,8 +1096,7 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
         return (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
     };
     u64* const Y = (u64*)(cnt + NB / 2);
-    uint16_t* const list16 = (uint16_t*)(Y + Q * J.ys);
-    u64* const R = (u64*)((u32*)list16 + group_list_words(J.nrows));
+    u64* const R = Y + Q * J.ys;
     u32* const misc = (u32*)(R + W);  // [0]: some insert overflowed its bucket
 
     for (int w = tid; w < W; w += 64 * NV) R[w] = Rg[w];
@@ -1211,7 +1222,8 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
                 if (in_mask(mask)) {
                     const u32 below = __builtin_amdgcn_mbcnt_hi(
                         hi32(mask), __builtin_amdgcn_mbcnt_lo(lo32(mask), 0));
-                    list16[A + below] = (uint16_t)(64 * w + lane);
+                    *(__attribute__((address_space(1))) uint16_t*)(list16 + A + below) =
+                        (uint16_t)(64 * w + lane);
                 }
             }
             A += __popcll(mask);
@@ -1259,6 +1271,7 @@ __device__ __forceinline__ void sieve_body_group(
     u32* const gmisc = lds + lds_words - 4;  // the group number (WG_MISC words at the end)
     u64* const S = scratch + (long long)blockIdx.x * scratch_words;
     u32* const low = (u32*)(S + 64ll * NV * J.W);
+    uint16_t* const list16 = (uint16_t*)(S + 64ll * NV * J.W + (J.m + 1) / 2);
     while (true) {
         if (tid == 0) {
             const unsigned long long g = atomicAdd(work, 1ull);
@@ -1287,7 +1300,7 @@ __device__ __forceinline__ void sieve_body_group(
         __syncthreads();
         for (int k = 0; k < n; ++k)
             bsgs_prime_group<NV, BB>(J, jobs, c0.job, S + (long long)k * J.W, tab, wbase, low,
-                                     hits, hit_count, hit_cap);
+                                     list16, hits, hit_count, hit_cap);
     }
 }
 
