@@ -19,6 +19,7 @@
 // (:1833; "+1" for a Sierpinski file) and a progress line every 20
 // launches (:1759).
 #include <getopt.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <chrono>
@@ -68,7 +69,7 @@ struct Batch {
 };
 
 // CPU-generated batches from a producer thread: windows of about 8 batches
-// (at least 2^29 wide) are sieved at once by rs_primes_range (all host
+// (at least 2^31 wide) are sieved at once by rs_primes_range (all host
 // threads), one window ahead of the one being cut into batches of `size`
 // primes as the reference cuts its prime arrays (:1561), at most 4 batches
 // ahead of the sieve.
@@ -122,10 +123,10 @@ class BatchFeed {
         Window w;
         w.lo = wlo;
         const double lnp = std::log((double)std::max<uint64_t>(wlo, 16));
-        // at least 2^29 wide, so that each generator thread sieves several
+        // at least 2^31 wide, so that each generator thread sieves many
         // segments per block
         uint64_t width = (uint64_t)(8.0 * size_ * lnp);
-        width = std::min<uint64_t>(std::max<uint64_t>(width, 1ull << 29), 1ull << 32);
+        width = std::min<uint64_t>(std::max<uint64_t>(width, 1ull << 31), 1ull << 32);
         w.hi = std::min<uint64_t>(wlo + width, high_ + 1);
         w.offs.resize((size_t)((w.hi - wlo) / std::max(1.0, lnp - 1.5) * 1.2) + 4096);
         int64_t got = 0;
@@ -325,8 +326,16 @@ int main(int argc, char* argv[]) {
     rs_sieve_info(sv, &mUsed, &T, &slots, &ldsw, &tile);
     printf("Each prime uses %d baby steps, %d giant steps per row, a %d-slot LDS table "
            "(%d bytes of LDS per wavefront)\n\n", mUsed, T, slots, ldsw);
+    // the generator's threads leave two of the process's CPUs to the batch
+    // cutter and this submitting thread
+    int cpus = (int)std::thread::hardware_concurrency();
+    cpu_set_t cset;
+    if (sched_getaffinity(0, sizeof cset, &cset) == 0) cpus = CPU_COUNT(&cset);
+    if (const char* e = getenv("OMP_NUM_THREADS"))
+        if (atoi(e) > 0) cpus = std::min(cpus, atoi(e));
     rs_primegen* gen = nullptr;
-    if ((st = rs_primegen_create(smallPrimeScale, high + 1, 0, &gen)) != RS_OK)
+    if ((st = rs_primegen_create(smallPrimeScale, high + 1, std::max(1, cpus - 2), &gen)) !=
+        RS_OK)
         return fail("prime generator set-up", st);
 
     const unsigned long long startLow = low;

@@ -11,12 +11,13 @@
 // threads.  Output keeps the reference's layout: uint32 offsets from an odd
 // uint64 low (:1960-1966).
 //
-// Sieve: odd numbers only, one bit each, in segments of kSegBits; a thread
-// takes blocks of kBlockSegs consecutive segments and carries every sieving
-// prime's next multiple from segment to segment (one remainder per prime
-// per block, by a precomputed reciprocal, not a division per segment).  The multiples of 3, 5, 7, 11 and 13
-// come from a precomputed pattern of period 15015 bits, copied word by word
-// at the segment's phase.
+// Sieve: a mod-30 wheel, one byte per 30 numbers (bit i: 30 j + kRes[i]), in
+// segments of kSegBytes; a thread takes blocks of up to kBlockSegs
+// consecutive segments and carries every sieving prime's next multiple
+// (byte position and wheel index) from segment to segment: one remainder
+// per prime per block, by a precomputed reciprocal.  The multiples of 7, 11
+// and 13 come from a precomputed pattern of period 1001 bytes; primes below
+// kSmallQ mark 32 KiB slices at a time, eight wheel steps per loop turn.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -27,26 +28,54 @@
 
 namespace {
 
-const uint32_t kWheel[] = {3, 5, 7, 11, 13};
-const uint32_t kPeriod = 3 * 5 * 7 * 11 * 13;  // 15015 odd numbers
+// the residues coprime to 30, the gaps between them, and the index of each
+const uint32_t kRes[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+const uint32_t kStep[8] = {6, 4, 2, 4, 2, 4, 6, 2};
+const int kPatQ[3] = {7, 11, 13};
+const uint32_t kPeriod = 7 * 11 * 13;  // pattern bytes
+
+struct WheelTables {
+    int8_t idx[30];         // index of a residue in kRes, or -1
+    uint8_t up[30];         // distance from a residue to the next one prime to 30
+    uint8_t mask[8][8];     // bit of q k, for q = kRes[qi] and k = kRes[wi] (mod 30)
+    uint8_t add[8][8];      // extra bytes of the step from q k to q (k + kStep[wi])
+    WheelTables() {
+        for (int r = 0; r < 30; ++r) idx[r] = -1;
+        for (int i = 0; i < 8; ++i) idx[kRes[i]] = (int8_t)i;
+        for (int r = 0; r < 30; ++r) {
+            int d = 0;
+            while (idx[(r + d) % 30] < 0) ++d;
+            up[r] = (uint8_t)d;
+        }
+        for (int qi = 0; qi < 8; ++qi)
+            for (int wi = 0; wi < 8; ++wi) {
+                const uint32_t rv = kRes[qi] * kRes[wi] % 30;
+                mask[qi][wi] = (uint8_t)(1u << idx[rv]);
+                add[qi][wi] = (uint8_t)((rv + kRes[qi] * kStep[wi]) / 30);
+            }
+    }
+};
+const WheelTables kW;
 
 }  // namespace
 
 struct rs_primegen {
-    std::vector<uint32_t> sp;   // odd sieving primes above 13, ascending
-    std::vector<uint64_t> pat;  // bit j: 2j + 1 has a factor in kWheel, for
-                                // j < kPeriod + 128 (wrapped)
-    std::vector<uint64_t> rq;   // floor((2^64 - 1) / q) per sieving prime
-    size_t nsmall = 0;          // sieving primes below kSmallQ
-    uint64_t bound2 = 0;        // survivors below this are prime without MR
+    std::vector<uint32_t> sp;  // sieving primes above 13, ascending
+    std::vector<uint64_t> rq;  // floor((2^64 - 1) / q) per sieving prime
+    std::vector<uint32_t> q30;  // q / 30 per sieving prime
+    std::vector<uint8_t> qi;    // the index of q mod 30 per sieving prime
+    std::vector<uint8_t> pat;  // byte j: bit i set when 30 j + kRes[i] has a
+                               // factor 7, 11 or 13, j < 2 kPeriod
+    size_t nsmall = 0;         // sieving primes below kSmallQ
+    uint64_t bound2 = 0;       // survivors below this are prime without MR
     int threads = 1;
 };
 
 namespace {
 
-const uint64_t kSegBits = 1u << 21;  // odd numbers per segment (256 KiB of bits)
-const uint64_t kBlockSegs = 16;      // segments per thread block
-const uint64_t kSliceBits = 1u << 18;  // 32 KiB slices for the small primes
+const uint64_t kSegBytes = 1u << 18;   // 256 KiB: 7.86M numbers per segment
+const uint64_t kBlockSegs = 16;        // segments per thread block
+const uint64_t kSliceBytes = 1u << 15;  // 32 KiB slices for the small primes
 const uint32_t kSmallQ = 1u << 15;
 
 uint64_t isqrt_ceil(uint64_t n) {
@@ -56,83 +85,149 @@ uint64_t isqrt_ceil(uint64_t n) {
     return r;
 }
 
-// 64 pattern bits starting at bit `pos` (pos < kPeriod + 64).
-inline uint64_t pattern_word(const std::vector<uint64_t>& pat, uint64_t pos) {
-    const uint64_t w = pos >> 6, sh = pos & 63;
-    return sh ? (pat[w] >> sh) | (pat[w + 1] << (64 - sh)) : pat[w];
+// Marks the multiples q k of one prime in seg[0, end) from byte `pos`,
+// wheel index `wi` (k = kRes[wi] mod 30) on; leaves pos, wi at the first
+// multiple at or past `end`.
+inline void mark_prime(uint8_t* seg, uint64_t end, uint32_t q, uint64_t& pos, int& wi) {
+    const uint32_t q30 = q / 30;
+    const int qi = kW.idx[q % 30];
+    const uint8_t* const mk = kW.mask[qi];
+    const uint8_t* const ad = kW.add[qi];
+    uint64_t p = pos;
+    int w = wi;
+    // eight wheel steps (one turn: q bytes) per loop while a whole turn fits
+    if (p + q < end) {
+        uint32_t inc[8];
+        for (int i = 0; i < 8; ++i) inc[i] = q30 * kStep[(w + i) & 7] + ad[(w + i) & 7];
+        uint8_t m[8];
+        for (int i = 0; i < 8; ++i) m[i] = mk[(w + i) & 7];
+        while (p + q < end) {
+            seg[p] |= m[0]; p += inc[0];
+            seg[p] |= m[1]; p += inc[1];
+            seg[p] |= m[2]; p += inc[2];
+            seg[p] |= m[3]; p += inc[3];
+            seg[p] |= m[4]; p += inc[4];
+            seg[p] |= m[5]; p += inc[5];
+            seg[p] |= m[6]; p += inc[6];
+            seg[p] |= m[7]; p += inc[7];
+        }
+    }
+    while (p < end) {
+        seg[p] |= mk[w];
+        p += q30 * kStep[w] + ad[w];
+        w = (w + 1) & 7;
+    }
+    pos = p;
+    wi = w;
 }
 
-// Sieves nseg segments of odd numbers from the odd `base` (nbits_total
-// bits in all), appending primes as offsets from `origin`.
-void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbits_total, uint64_t origin,
-                 std::vector<uint64_t>& bits, std::vector<uint64_t>& next,
+// Sieves the numbers [base, base + 30 nbytes_total) (base a multiple of 30)
+// segment by segment, appending the primes of [lo, hi) as offsets from
+// `origin`.
+void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbytes_total, uint64_t lo,
+                 uint64_t hi, uint64_t origin, std::vector<uint8_t>& seg,
+                 std::vector<uint64_t>& next, std::vector<uint8_t>& wheel,
                  std::vector<uint32_t>& out) {
-    const uint64_t top = base + 2 * (nbits_total - 1);
-    // sieving primes that reach the block, and the bit index of the first
-    // multiple of each at or above both base and q^2
+    const uint64_t top = std::min<uint64_t>(base + 30 * nbytes_total, hi) - 1;
+    // sieving primes that reach the block, and for each the first multiple
+    // q k >= max(q^2, base) with k prime to 30, as a byte position and the
+    // wheel index of k
     size_t np = 0;
     while (np < g->sp.size() && (uint64_t)g->sp[np] * g->sp[np] <= top) ++np;
     next.resize(np);
+    wheel.resize(np);
     for (size_t i = 0; i < np; ++i) {
-        const uint64_t q = g->sp[i], qq = q * q;
-        if (qq >= base) {
-            next[i] = (qq - base) / 2;
+        const uint64_t q = g->sp[i];
+        uint64_t k;
+        if (q * q >= base) {
+            k = q;
         } else {
-            // r = base mod q by the reciprocal (the estimated quotient is
-            // short by at most one); base + d == 0 mod q for d = q - r (or
-            // 0), and base + 2j for j = d / 2, or (d + q) / 2 with d odd
-            uint64_t r = base - (uint64_t)(((rsh::u128)base * g->rq[i]) >> 64) * q;
-            if (r >= q) r -= q;
-            const uint64_t d = r ? q - r : 0;
-            next[i] = (d & 1) ? (d + q) / 2 : d / 2;
+            // k = ceil(base / q) by the reciprocal (the estimate is short by
+            // at most one)
+            uint64_t d = (uint64_t)(((rsh::u128)base * g->rq[i]) >> 64);
+            if (base - d * q >= q) ++d;
+            k = d + (base - d * q != 0);
         }
+        const uint32_t kr0 = (uint32_t)(k % 30);
+        k += kW.up[kr0];
+        const uint32_t kr = (kr0 + kW.up[kr0]) % 30;
+        next[i] = (q * k - base) / 30;
+        wheel[i] = (uint8_t)kW.idx[kr];
     }
-    for (uint64_t done = 0; done < nbits_total; done += kSegBits) {
-        const uint64_t nbits = std::min(kSegBits, nbits_total - done);
-        const uint64_t b0 = base + 2 * done;
-        const size_t words = (nbits + 63) / 64;
-        bits.resize(words);
-        // wheel primes from the pattern: bit i is 2j + 1 = b0 + 2i, j = (b0 - 1) / 2 + i
-        uint64_t pos = ((b0 - 1) / 2) % kPeriod;
-        for (size_t w = 0; w < words; ++w) {
-            bits[w] = pattern_word(g->pat, pos);
-            pos += 64;
-            if (pos >= kPeriod) pos -= kPeriod;
+    const size_t ns = std::min(np, g->nsmall);
+    for (uint64_t done = 0; done < nbytes_total; done += kSegBytes) {
+        const uint64_t nb = std::min(kSegBytes, nbytes_total - done);
+        const uint64_t b0 = base + 30 * done;
+        seg.resize(nb + 8);
+        // 7, 11, 13 from the pattern, at the segment's phase
+        uint64_t ph = (b0 / 30) % kPeriod;
+        for (uint64_t at = 0; at < nb;) {
+            const uint64_t n = std::min<uint64_t>(nb - at, kPeriod);
+            memcpy(seg.data() + at, g->pat.data() + ph, n);
+            at += n;
+            ph = (ph + n) % kPeriod;
         }
-        if (b0 <= 13)  // the wheel primes themselves
-            for (uint32_t q : kWheel)
-                if (q >= b0 && (q - b0) / 2 < nbits) bits[(q - b0) / 2 >> 6] &= ~(1ull << ((q - b0) / 2 & 63));
-        // primes below kSmallQ (many marks each) a cache-sized slice at a
-        // time, the rest (at most a few marks each) over the whole segment
-        const size_t ns = std::min(np, g->nsmall);
-        for (uint64_t s0 = 0; s0 < nbits; s0 += kSliceBits) {
-            const uint64_t s1 = std::min(nbits, s0 + kSliceBits);
+        memset(seg.data() + nb, 0xff, 8);  // past the end: never a prime
+        if (b0 == 0) seg[0] = (uint8_t)((seg[0] & ~0x0e) | 1);  // 7, 11, 13 prime; 1 not
+        for (uint64_t s0 = 0; s0 < nb; s0 += kSliceBytes) {
+            const uint64_t s1 = std::min(nb, s0 + kSliceBytes);
             for (size_t i = 0; i < ns; ++i) {
-                const uint64_t q = g->sp[i];
-                uint64_t j = next[i];
-                for (; j < s1; j += q) bits[j >> 6] |= 1ull << (j & 63);
-                next[i] = j;
+                int w = wheel[i];
+                mark_prime(seg.data(), s1, g->sp[i], next[i], w);
+                wheel[i] = (uint8_t)w;
             }
         }
-        for (size_t i = 0; i < ns; ++i) next[i] -= nbits;
+        for (size_t i = 0; i < ns; ++i) next[i] -= nb;
+        // the rest: a few marks per segment each, most none
+        uint8_t* const sg = seg.data();
         for (size_t i = ns; i < np; ++i) {
-            const uint64_t q = g->sp[i];
-            uint64_t j = next[i];
-            for (; j < nbits; j += q) bits[j >> 6] |= 1ull << (j & 63);
-            next[i] = j - nbits;
+            uint64_t p = next[i];
+            if (p < nb) {
+                const uint32_t q30 = g->q30[i];
+                const uint8_t* const mk = kW.mask[g->qi[i]];
+                const uint8_t* const ad = kW.add[g->qi[i]];
+                int w = wheel[i];
+                do {
+                    sg[p] |= mk[w];
+                    p += q30 * kStep[w] + ad[w];
+                    w = (w + 1) & 7;
+                } while (p < nb);
+                wheel[i] = (uint8_t)w;
+            }
+            next[i] = p - nb;
         }
-        for (size_t w = 0; w < words; ++w) {
-            uint64_t free = ~bits[w];
-            if (w == words - 1 && (nbits & 63)) free &= (1ull << (nbits & 63)) - 1;
-            while (free) {
-                const int b = __builtin_ctzll(free);
-                free &= free - 1;
-                const uint64_t v = b0 + 2 * (w * 64 + b);
-                if (v < 3) continue;  // 1 is not prime (2 is never in an odd window)
-                if (v >= g->bound2 && !rsh::is_prime_u64(v)) continue;
-                out.push_back((uint32_t)(v - origin));
+        // survivors, in order: the words of 8 bytes (240 numbers) at both
+        // ends of the window check the bounds, the test above bound2 only
+        // where the segment reaches it
+        const size_t at = out.size();
+        out.resize(at + (size_t)(nb * 8) + 8);
+        uint32_t* o = out.data() + at;
+        const bool mr = b0 + 30 * nb > g->bound2;
+        const uint64_t off = b0 - origin;
+        for (uint64_t w = 0; w < nb; w += 8) {
+            uint64_t word;
+            memcpy(&word, seg.data() + w, 8);
+            uint64_t free = ~word;
+            const uint64_t vw = b0 + 30 * w;
+            if (vw < lo || vw + 240 > hi || mr) {
+                while (free) {
+                    const int b = __builtin_ctzll(free);
+                    free &= free - 1;
+                    const uint64_t v = vw + 30 * (uint64_t)(b >> 3) + kRes[b & 7];
+                    if (v < lo || v >= hi) continue;
+                    if (v >= g->bound2 && !rsh::is_prime_u64(v)) continue;
+                    *o++ = (uint32_t)(v - origin);
+                }
+            } else {
+                const uint32_t ow = (uint32_t)(off + 30 * w);
+                while (free) {
+                    const int b = __builtin_ctzll(free);
+                    free &= free - 1;
+                    *o++ = ow + 30 * (uint32_t)(b >> 3) + kRes[b & 7];
+                }
             }
         }
+        out.resize((size_t)(o - out.data()));
     }
 }
 
@@ -158,17 +253,24 @@ extern "C" rs_status rs_primegen_create(int small_prime_scale, uint64_t hi, int 
     }
     // every prime <= lim sieves, so a survivor below (lim + 1)^2 is prime
     g->bound2 = (lim + 1) * (lim + 1);
-    // the wheel primes are marked by the pattern
+    // 2, 3 and 5 are outside the wheel, 7, 11 and 13 come from the pattern
     g->sp.erase(g->sp.begin(), std::upper_bound(g->sp.begin(), g->sp.end(), 13u));
     g->nsmall = std::lower_bound(g->sp.begin(), g->sp.end(), kSmallQ) - g->sp.begin();
     g->rq.resize(g->sp.size());
-    for (size_t i = 0; i < g->sp.size(); ++i) g->rq[i] = ~0ull / g->sp[i];
-    g->pat.assign((kPeriod + 128 + 63) / 64 + 1, 0);
-    for (uint32_t j = 0; j < kPeriod + 128; ++j) {
-        const uint32_t v = 2 * (j % kPeriod) + 1;
-        for (uint32_t q : kWheel)
-            if (v % q == 0) g->pat[j >> 6] |= 1ull << (j & 63);
+    g->q30.resize(g->sp.size());
+    g->qi.resize(g->sp.size());
+    for (size_t i = 0; i < g->sp.size(); ++i) {
+        g->rq[i] = ~0ull / g->sp[i];
+        g->q30[i] = g->sp[i] / 30;
+        g->qi[i] = (uint8_t)kW.idx[g->sp[i] % 30];
     }
+    g->pat.assign(2 * kPeriod, 0);
+    for (uint32_t j = 0; j < 2 * kPeriod; ++j)
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t v = 30ull * j + kRes[i];
+            for (int q : kPatQ)
+                if (v % q == 0) g->pat[j] |= (uint8_t)(1u << i);
+        }
     *out = g;
     return RS_OK;
 }
@@ -179,42 +281,44 @@ extern "C" rs_status rs_primes_range(rs_primegen* g, uint64_t lo, uint64_t hi,
         return RS_ERR_ARG;
     *count = 0;
     if (hi == lo) return RS_OK;
-    std::vector<uint32_t> two;
-    if (lo <= 2 && hi > 2) two.push_back((uint32_t)(2 - lo));
-    const uint64_t base = (lo % 2) ? lo : lo + 1;  // first odd >= lo
-    if (base >= hi) {
-        *count = (int64_t)two.size();
-        if ((int64_t)two.size() > cap) return RS_ERR_OVERFLOW;
-        if (!two.empty()) offsets[0] = two[0];
-        return RS_OK;
-    }
-    const uint64_t nodd = (hi - base + 1) / 2;
-    const uint64_t nseg = (nodd + kSegBits - 1) / kSegBits;
-    // blocks of kBlockSegs segments, fewer when that leaves threads idle
-    const uint64_t bsegs =
-        std::max<uint64_t>(1, std::min<uint64_t>(kBlockSegs, nseg / std::max(1, g->threads)));
-    const uint64_t nblk = (nseg + bsegs - 1) / bsegs;
+    std::vector<uint32_t> small;  // 2, 3, 5 (outside the wheel)
+    for (uint32_t q : {2u, 3u, 5u})
+        if (q >= lo && q < hi) small.push_back((uint32_t)(q - lo));
+    const uint64_t base = lo / 30 * 30;
+    const uint64_t nbytes = (hi - base + 29) / 30;
+    const uint64_t nseg = (nbytes + kSegBytes - 1) / kSegBytes;
+    // equal blocks of at most about kBlockSegs segments, a whole number of
+    // them per thread (multiples of 8 bytes, the last one takes the rest)
+    const uint64_t nt0 = (uint64_t)std::max(1, g->threads);
+    const uint64_t rounds = std::max<uint64_t>(1, (nseg + nt0 * kBlockSegs - 1) / (nt0 * kBlockSegs));
+    // (each block computes every sieving prime's first multiple: blocks of
+    // at least two segments, or one block)
+    const uint64_t nblk =
+        std::max<uint64_t>(1, std::min<uint64_t>(nt0 * rounds, nbytes / (2 * kSegBytes)));
+    const uint64_t bbytes = (nbytes / nblk + 7) & ~7ull;
     std::vector<std::vector<uint32_t>> parts(nblk);
     std::atomic<uint64_t> next(0);
     std::vector<std::thread> th;
-    const int nt = (int)std::min<uint64_t>(g->threads, nblk);
+    const int nt = (int)std::min<uint64_t>(nt0, nblk);
     auto work = [&]() {
-        std::vector<uint64_t> bits, carry;
+        std::vector<uint8_t> seg, wheel;
+        std::vector<uint64_t> carry;
         for (uint64_t k; (k = next.fetch_add(1)) < nblk;) {
-            const uint64_t b0 = base + 2 * k * bsegs * kSegBits;
-            const uint64_t nb = std::min<uint64_t>(bsegs * kSegBits, nodd - k * bsegs * kSegBits);
-            sieve_block(g, b0, nb, lo, bits, carry, parts[k]);
+            const uint64_t first = k * bbytes;
+            if (first >= nbytes) continue;
+            const uint64_t nb = k + 1 == nblk ? nbytes - first : std::min(bbytes, nbytes - first);
+            sieve_block(g, base + 30 * first, nb, lo, hi, lo, seg, carry, wheel, parts[k]);
         }
     };
     for (int t = 1; t < nt; ++t) th.emplace_back(work);
     work();  // the calling thread takes blocks too
     for (auto& x : th) x.join();
-    int64_t total = (int64_t)two.size();
+    int64_t total = (int64_t)small.size();
     for (auto& p : parts) total += (int64_t)p.size();
     *count = total;
     if (total > cap) return RS_ERR_OVERFLOW;
     int64_t c = 0;
-    for (uint32_t v : two) offsets[c++] = v;
+    for (uint32_t v : small) offsets[c++] = v;
     for (auto& p : parts) {
         memcpy(offsets + c, p.data(), p.size() * sizeof(uint32_t));
         c += (int64_t)p.size();
