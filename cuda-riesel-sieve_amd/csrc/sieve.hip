@@ -915,13 +915,18 @@ __device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __re
     const u64 target = shoup32_full(k, yv, p);
     const u32 tlo = lo32(target);
     const int L = J.m >> 6;
-    for (int v = 0; v < L; v += 4) {
-        u32 w[4];
+    // sixteen loads in flight per lane, then the (rare) matches
+    for (int v = 0; v < L; v += 16) {
+        u32 w[16];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) w[u] = v + u < L ? low[lane + 64 * (v + u)] : ~tlo;
+        for (int u = 0; u < 16; ++u)
+            w[u] = v + u < L ? global_ld(low + lane + 64 * (v + u)) : ~tlo;
+        u32 found = 0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (w[u] != tlo) continue;
+        for (int u = 0; u < 16; ++u) found |= (u32)(w[u] == tlo) << u;
+        while (found) {
+            const int u = __builtin_ctz(found);
+            found &= found - 1;
             const int j = lane + 64 * (v + u);
             u64 gj;
             if (p < kFpMaxP) {
@@ -962,13 +967,16 @@ __device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __re
     }
 }
 
-// Giant steps of one prime over the lanes of the whole workgroup: as
-// giant_steps, with 64 NV runs per round and the table at LDS address 0.
+// Giant steps of one prime over the lanes of the whole workgroup: runs of TW
+// consecutive t of one row as in giant_steps, but each lane keeps its run's
+// t for the whole prime: a round covers RR = floor(64 NV TW / T) whole rows
+// (the lanes past RR T / TW rows' worth of runs idle, less than T items of
+// 64 NV TW) and every lane moves RR rows on.  The table is at LDS address 0.
 // The active rows of the slice starting at row r0 are uint16 numbers
 // (row - r0) in list16, in the workgroup's global scratch; each lane reads
 // its row number two rounds ahead and its row's entry from the job's row
 // table (a few KB, cache-resident) one round ahead.
-template <int NV, int TW, bool CHAIN, bool EXACT>
+template <int NV, int TW, bool CHAIN>
 __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev* __restrict__ jobs,
                                                   int job, int A, int r0,
                                                   const unsigned short* __restrict__ cnt,
@@ -983,40 +991,26 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
     u32 bmask = nbm << 4;
     asm volatile("" : "+v"(bmask));
     const int T = J.T;
-    const int total = A * T;
-    const int per = 64 * NV * TW;  // items per round
-    const int rstep = div_T((u32)per, J.invT), tstep = per - rstep * T;
-    auto advance = [&](int& aa, int& tt) {
-        aa += rstep;
-        if (!EXACT) {
-            tt += tstep;
-            if (tt >= T) tt -= T, aa += 1;
-        }
-    };
+    const int RR = div_T(64u * NV * TW, J.invT);  // rows per round (T <= 64 NV TW)
+    int a = div_T((u32)(TW * lid), J.invT);       // >= RR: an idle lane
+    const int tcur = TW * lid - a * T;
+    const bool on = a < RR;
+    if (!on) a = A;
     const uint2* const rowent = J.rowent + r0;
-    // runs of this round (a, t0) and the next (a1, t1)
-    int a = div_T((u32)(TW * lid), J.invT);
-    int t0 = TW * lid - a * T;
-    int a1 = a, t1 = t0;
-    advance(a1, t1);
     int rnum = global_ld(list16 + min(a, A - 1));
-    int rnum1 = per < total ? global_ld(list16 + min(a1, A - 1)) : 0;
+    int rnum1 = RR < A ? global_ld(list16 + min(a + RR, A - 1)) : 0;
     uint2 rown = global_ld(rowent + rnum);
-    for (int base = 0; base < total; base += per) {
+    for (int base = 0; base < A; base += RR) {
         const bool ok = a < A;
         const u64 okm = lanes(ok);
         const uint2 row = rown;
         const int rcur = rnum;
-        const int tcur = t0;
         // the row number two rounds ahead, the row entry one round ahead
-        int a2 = a1, t2 = t1;
-        advance(a2, t2);
-        const int rnum2 = base + 2 * per < total ? global_ld(list16 + min(a2, A - 1)) : 0;
-        if (base + per < total) rown = global_ld(rowent + rnum1);
+        const int rnum2 = base + 2 * RR < A ? global_ld(list16 + min(a + 2 * RR, A - 1)) : 0;
+        if (base + RR < A) rown = global_ld(rowent + rnum1);
         rnum = rnum1;
         rnum1 = rnum2;
-        a = a1, t0 = t1;
-        a1 = a2, t1 = t2;
+        if (on) a += RR;
         const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * tcur);
         uint4 e[TW];
         u32 fp2[TW], ba[TW];
@@ -1073,14 +1067,16 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
 }
 
 // BSGS of one prime by the NV wavefronts of the workgroup.  Rg: the
-// prime's record in global scratch; the table at LDS address 0, the rest of
-// the layout (lds_words_group) from wbase; low: the workgroup's m low
-// words and list16 its row list, both in its global scratch.
+// prime's record in global scratch (word tid already in pre); the table at
+// LDS address 0, the rest of the layout (lds_words_group) from wbase; low:
+// the workgroup's m low words and list16 its row list, both in its global
+// scratch.
 template <int NV, int BB>
 __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* __restrict__ jobs,
                                                  int job, const u64* __restrict__ Rg,
                                                  uint4* __restrict__ tab, u32* __restrict__ wbase,
                                                  u32* __restrict__ low, uint16_t* list16,
+                                                 u64 pre,
                                                  HitDev* __restrict__ hits,
                                                  u32* __restrict__ hit_count, u32 hit_cap) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1099,7 +1095,9 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
     u64* const R = Y + Q * J.ys;
     u32* const misc = (u32*)(R + W);  // [0]: some insert overflowed its bucket
 
-    for (int w = tid; w < W; w += 64 * NV) R[w] = Rg[w];
+    // the record: word tid came in `pre`, loaded one prime ahead
+    if (tid < W) R[tid] = pre;
+    for (int w = tid + 64 * NV; w < W; w += 64 * NV) R[w] = Rg[w];
     for (u32 b = tid; b < NB; b += 64 * NV) tab[b] = make_uint4(0, 0, 0, 0);
     for (u32 c = tid; c < NB / 2; c += 64 * NV) cnt[c] = 0;
     if (tid == 0) misc[0] = 0;
@@ -1231,22 +1229,20 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
         __syncthreads();
         if (A == 0) continue;  // no barrier needed: the list was not read
         const char* const Yb = (const char*)Y;
-#define RS_GIANT_G(TWV, CH, EX)                                                                \
-    giant_steps_group<NV, TWV, CH, EX>(J, jobs, job, A, r0, cnt16, Yb, list16, R, 0u - lo32(p), \
-                                       low, hits, hit_count, hit_cap)
-#define RS_GIANT_G_TW(CH, EX)  \
-    if (J.tw >= 4)             \
-        RS_GIANT_G(4, CH, EX); \
-    else if (J.tw == 2)        \
-        RS_GIANT_G(2, CH, EX); \
-    else                       \
-        RS_GIANT_G(1, CH, EX);
+#define RS_GIANT_G(TWV, CH)                                                                 \
+    giant_steps_group<NV, TWV, CH>(J, jobs, job, A, r0, cnt16, Yb, list16, R, 0u - lo32(p), \
+                                   low, hits, hit_count, hit_cap)
+#define RS_GIANT_G_TW(CH)  \
+    if (J.tw >= 4)         \
+        RS_GIANT_G(4, CH); \
+    else if (J.tw == 2)    \
+        RS_GIANT_G(2, CH); \
+    else                   \
+        RS_GIANT_G(1, CH);
         if (chain) {
-            RS_GIANT_G_TW(true, false)
-        } else if ((64 * NV * J.tw) % T == 0) {
-            RS_GIANT_G_TW(false, true)
+            RS_GIANT_G_TW(true)
         } else {
-            RS_GIANT_G_TW(false, false)
+            RS_GIANT_G_TW(false)
         }
 #undef RS_GIANT_G_TW
 #undef RS_GIANT_G
@@ -1298,9 +1294,13 @@ __device__ __forceinline__ void sieve_body_group(
             prep_group_powers<NV>(J, R);
         }
         __syncthreads();
-        for (int k = 0; k < n; ++k)
+        u64 pre = tid < J.W ? S[tid] : 0;
+        for (int k = 0; k < n; ++k) {
+            const u64 cur = pre;
+            if (k + 1 < n) pre = tid < J.W ? S[(long long)(k + 1) * J.W + tid] : 0;
             bsgs_prime_group<NV, BB>(J, jobs, c0.job, S + (long long)k * J.W, tab, wbase, low,
-                                     list16, hits, hit_count, hit_cap);
+                                     list16, cur, hits, hit_count, hit_cap);
+        }
     }
 }
 
