@@ -472,17 +472,17 @@ extern "C" rs_status rs_server_step(rs_server* s, int* active, int* queued,
         if (s->jobs.at(id).p_next < s->jobs.at(id).p_hi) run.push_back(id);
     bool launched = false;
     if (!run.empty()) {
-        int free_slot = !s->slots[0].busy ? 0 : !s->slots[1].busy ? 1 : -1;
-        if (free_slot < 0) {  // both in flight: drain the older first
+        int idle = !s->slots[0].busy ? 0 : !s->slots[1].busy ? 1 : -1;
+        if (idle < 0) {  // both in flight: drain the older first
             const int old = s->inflight.front();
             s->inflight.pop_front();
             rs_status st = drain(s, s->slots[old], &drained);
             if (st != RS_OK) return st;
-            free_slot = old;
+            idle = old;
         }
-        rs_status st = launch_step(s, s->slots[free_slot], run);
+        rs_status st = launch_step(s, s->slots[idle], run);
         if (st != RS_OK) return st;
-        s->inflight.push_back(free_slot);
+        s->inflight.push_back(idle);
         launched = true;
     }
     // 3. drain the previous step (or, with nothing launched, the last one)
