@@ -997,8 +997,11 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
     const bool on = a < RR;
     if (!on) a = A;
     const uint2* const rowent = J.rowent + r0;
+    // (the loads past the last round re-read the last row: unconditional
+    // loads keep the loop free of branches, so the wait for each load is the
+    // round after its issue, not the next merge point)
     int rnum = global_ld(list16 + min(a, A - 1));
-    int rnum1 = RR < A ? global_ld(list16 + min(a + RR, A - 1)) : 0;
+    int rnum1 = global_ld(list16 + min(a + RR, A - 1));
     uint2 rown = global_ld(rowent + rnum);
     for (int base = 0; base < A; base += RR) {
         const bool ok = a < A;
@@ -1006,8 +1009,8 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
         const uint2 row = rown;
         const int rcur = rnum;
         // the row number two rounds ahead, the row entry one round ahead
-        const int rnum2 = base + 2 * RR < A ? global_ld(list16 + min(a + 2 * RR, A - 1)) : 0;
-        if (base + RR < A) rown = global_ld(rowent + rnum1);
+        const int rnum2 = global_ld(list16 + min(a + 2 * RR, A - 1));
+        rown = global_ld(rowent + rnum1);
         rnum = rnum1;
         rnum1 = rnum2;
         if (on) a += RR;
