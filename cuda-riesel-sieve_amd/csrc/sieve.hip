@@ -215,7 +215,7 @@ __device__ __forceinline__ bool in_mask(u64 m) { return __builtin_amdgcn_inverse
 // fingerprint match is resolved by an exact rescan() of
 // the reduced product.  The loop is wave-uniform because rescan() needs
 // every lane.
-template <int TW, bool CHAIN, bool EXACT>
+template <int TW, bool CHAIN, bool EXACT, bool PF>
 __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __restrict__ jobs,
                                             int job, int A, int r0,
                                             u32 tab_addr,
@@ -240,13 +240,32 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
     int a = div_T((u32)(TW * lane), J.invT);
     int t0 = TW * lane - a * T;
     const int rstep = div_T(64u * TW, J.invT), tstep = 64 * TW - rstep * T;
+    auto advance = [&](int& aa, int& tt) {
+        aa += rstep;
+        if (!EXACT) {
+            tt += tstep;
+            if (tt >= T) tt -= T, aa += 1;
+        }
+    };
+    // PF (full row entries only): the next round's entry is read during
+    // this round, unconditionally (clamped), so its wait is a round later
+    uint2 rown = make_uint2(0, 0);
+    if (PF && A > 0) rown = list[min(a, A - 1)];
     for (int base = 0; base < total; base += 64 * TW) {
         const bool ok = a < A;
         const u64 okm = lanes(ok);
         // the row's entry: in the list, or (idx) behind its row number
-        const int ac = min(a, A - 1);
-        const uint2 row = idx ? rows_lds[list16[ac]] : list[ac];
-        const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * t0);
+        uint2 row;
+        const int tcur = t0;
+        if (PF) {
+            row = rown;
+            advance(a, t0);
+            rown = list[min(a, A - 1)];
+        } else {
+            const int ac = min(a, A - 1);
+            row = idx ? rows_lds[list16[ac]] : list[ac];
+        }
+        const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * tcur);
         uint4 e[TW];
         u32 fp2[TW], ba[TW];
 #pragma unroll
@@ -292,16 +311,12 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
                 mb &= mb - 1;
                 const u32 k = __builtin_amdgcn_readlane(row.x, src);
                 const u32 yent = __builtin_amdgcn_readlane(row.y, src);
-                const int tt = __builtin_amdgcn_readlane(t0, src) + u;
+                const int tt = __builtin_amdgcn_readlane(tcur, src) + u;
                 const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
                 rescan(J, jobs, job, R, yv, k, yent, r0, tt, vstart, hits, hit_count, hit_cap);
             }
         }
-        a += rstep;
-        if (!EXACT) {
-            t0 += tstep;
-            if (t0 >= T) t0 -= T, a += 1;
-        }
+        if (!PF) advance(a, t0);
     }
 }
 
@@ -568,7 +583,7 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
 // tab, LDS byte address tab_addr, a multiple of its size; the rest of the
 // wavefront's LDS (counts, Y, row list, record) at wbase.  With idx (only
 // with rows_in_lds) the row list holds row numbers into rows_lds.
-template <int BB>
+template <int BB, bool PF>
 __device__ __forceinline__ void bsgs_prime(const JobDev& J, const JobDev* __restrict__ jobs,
                                            int job, u64 pre,
                                            const u64* __restrict__ Rg, uint4* __restrict__ tab,
@@ -736,7 +751,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, const JobDev* __rest
         wave_sync();
         const char* const Yb = (const char*)Y;
 #define RS_GIANT(TWV, CH, EX)                                                                \
-    giant_steps<TWV, CH, EX>(J, jobs, job, A, r0, tab_addr, cnt, Yb, list, rows_lds, idx,           \
+    giant_steps<TWV, CH, EX, PF>(J, jobs, job, A, r0, tab_addr, cnt, Yb, list, rows_lds, idx,       \
                              R, 0u - lo32(p), vstart, hits, hit_count, hit_cap)
 #define RS_GIANT_TW(CH, EX)      \
     if (J.tw >= 4)               \
@@ -764,7 +779,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, const JobDev* __rest
 // The persistent kernel.  IDX: single-job launches whose wavefronts list
 // active rows by row number (RS_TILE_ROW_INDEX), a smaller LDS layout.  BB:
 // baby steps per block of fill-counter atomics in flight.
-template <bool IDX, int BB>
+template <bool IDX, int BB, bool PF>
 __device__ __forceinline__ void sieve_body(
     ChunkDev c0, const JobDev* __restrict__ jobs, const ChunkDev* __restrict__ chunks,
     int nchunks, long long ngroups, int lds_words, int tab_words, int shared_words,
@@ -847,7 +862,7 @@ __device__ __forceinline__ void sieve_body(
         for (int i = 0; i < n; ++i) {
             const u64 cur = pre;
             if (i + 1 < n) pre = lane < J.W ? S[(i + 1) * J.W + lane] : 0;
-            bsgs_prime<BB>(J, jobs, C.job, cur, S + i * J.W, tab, tab_addr, wbase, rows_lds, rows_in_lds,
+            bsgs_prime<BB, PF>(J, jobs, C.job, cur, S + i * J.W, tab, tab_addr, wbase, rows_lds, rows_in_lds,
                        IDX, hits, hit_count, hit_cap);
         }
         wave_sync_global();  // records are rewritten by the next group
@@ -1316,18 +1331,21 @@ __device__ __forceinline__ void sieve_body_group(
 // baby-step blocks of the 128- and the 96-VGPR builds
 constexpr int BB4 = RS_BB4, BB5 = RS_BB5;
 
-// At least 4 wavefronts per SIMD (128 VGPRs): full row entries; the LDS
-// layouts of typical jobs allow 4, and fewer leave the VALU idle on LDS
-// waits.  The host picks this or rs_sieve_kernel5, whichever keeps more
-// wavefronts resident for the launch's layout.
+// At least 4 wavefronts per SIMD (128 VGPRs): full row entries, each giant-
+// step round's entry read a round ahead (the registers for it fit this
+// build's budget, not the 96 of the 5-wavefront builds); the LDS layouts of
+// typical jobs allow 4, and fewer leave the VALU idle on LDS waits.  The
+// host picks this or rs_sieve_kernel5, whichever keeps more wavefronts
+// resident for the launch's layout.
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 rs_sieve_kernel(ChunkDev c0, const JobDev* __restrict__ jobs,
                 const ChunkDev* __restrict__ chunks, int nchunks, long long ngroups,
                 int lds_words, int tab_words, int shared_words, int scratch_words,
                 uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
                 HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
-    sieve_body<false, BB4>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
-                      scratch_words, scratch, work, hits, hit_count, hit_cap);
+    sieve_body<false, BB4, true>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words,
+                                 shared_words, scratch_words, scratch, work, hits, hit_count,
+                                 hit_cap);
 }
 
 
@@ -1340,8 +1358,9 @@ rs_sieve_kernel5(ChunkDev c0, const JobDev* __restrict__ jobs,
                  int lds_words, int tab_words, int shared_words, int scratch_words,
                  uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
                  HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count, uint32_t hit_cap) {
-    sieve_body<false, BB5>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
-                      scratch_words, scratch, work, hits, hit_count, hit_cap);
+    sieve_body<false, BB5, false>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words,
+                                  shared_words, scratch_words, scratch, work, hits, hit_count,
+                                  hit_cap);
 }
 
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
@@ -1351,8 +1370,9 @@ rs_sieve_kernel5_rix(ChunkDev c0, const JobDev* __restrict__ jobs,
                      uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
                      HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count,
                      uint32_t hit_cap) {
-    sieve_body<true, BB5>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words, shared_words,
-                     scratch_words, scratch, work, hits, hit_count, hit_cap);
+    sieve_body<true, BB5, false>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words,
+                                 shared_words, scratch_words, scratch, work, hits, hit_count,
+                                 hit_cap);
 }
 
 // One workgroup of 4 or 8 wavefronts per prime, one shared table
