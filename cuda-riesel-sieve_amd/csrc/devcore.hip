@@ -243,12 +243,10 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     for (size_t i = 0; i < kvals.size(); ++i) kjac[i] = jacobi_consts(kvals[i]);
     J.bjac = jacobi_consts(pl->base);
     HIPCHK(hipMalloc(&dj->d_kvals, kjac.size() * sizeof(uint4)));
-    // giant-step entries of the rows (the Y offset of r and the row within
-    // its slice)
+    // giant-step entries of the rows: k and the byte offset of r's Y run
     std::vector<uint2> rowent(nrows);
     for (int i = 0; i < nrows; ++i)
-        rowent[i] = make_uint2(rowk[i], ((rowinfo[i] >> 16) * (uint32_t)J.ys * 8u) |
-                                            ((uint32_t)(i % ROW_SLICE) << 19));
+        rowent[i] = make_uint2(rowk[i], (rowinfo[i] >> 16) * (uint32_t)J.ys * 8u);
     HIPCHK(hipMalloc(&dj->d_rowent, rowent.size() * sizeof(uint2)));
     HIPCHK(hipMemcpy(dj->d_kvals, kjac.data(), kjac.size() * sizeof(uint4),
                      hipMemcpyHostToDevice));

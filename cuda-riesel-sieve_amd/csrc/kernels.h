@@ -78,8 +78,8 @@ struct JobDev {
     const uint32_t* kw0;      // per distinct k: 64-row word holding its first row
     const uint64_t* kmask;    // per distinct k: its even-r rows in words w0 and
                               // w0 + 1, then its odd-r rows in the same words
-    const uint2* rowent;      // per row: its giant-step entry (k, 8 r ys |
-                              // (row mod ROW_SLICE) << 19)
+    const uint2* rowent;      // per row: its giant-step entry (k, 8 r ys),
+                              // in row order (ascending k, then r)
     const uint32_t* bitmap;   // optional n-list bitmaps (rows x words, MSB first)
     int words;                // 32-bit words per row bitmap
     int nwv;        // 0: one wavefront per prime with its own table; 4 or 8:
@@ -131,12 +131,13 @@ static inline int rec_words(int Q, int T, int nrows, int nwv = 0) {
 
 // LDS words of one wavefront: its table, placed apart from the rest so
 // that it is aligned to its size,
-//   4 NB                      NB buckets x 8 uint16 fingerprints (0 = empty)
+//   4 NB                      NB buckets x 8 uint16 fingerprints (slots past
+//                             a bucket's fill count hold stale values)
 // and, from the base of the rest,
 //   NB                        bucket fill counts, one uint32 per bucket
 //   then 2 Q ys               Y entries (uint64), rows ys apart
 //   then list_words           active rows of one slice: their entries (k,
-//                             8 r ys | row << 19), or with idx (single-job
+//                             8 r ys), or with idx (single-job
 //                             launches, entries in block-shared LDS) their
 //                             uint16 row numbers
 //   then 2 W                  the prime's record (uint64)

@@ -85,9 +85,15 @@ __device__ __forceinline__ uint4 const_ld(const uint4* p) {
 __device__ __forceinline__ int div_T(u32 idx, u32 invT) { return (int)__umulhi(idx, invT); }
 
 // Table hash of a residue: the bucket is taken from bits 4 and up of its
-// low word and the 16-bit fingerprint is its high half.  Empty slots hold
-// 0, so a probe whose fingerprint is 0 matches them and is resolved by the
-// exact rescan (a 2^-16 chance per probe); slot occupancy is read from the
+// low word and the 16-bit fingerprint is its high half.  The per-wavefront
+// kernels clear only the fill counts for each prime, not the slots: a slot
+// past its bucket's count holds a fingerprint of another prime's baby step
+// (or whatever the LDS held at launch), which equals a probe's fingerprint
+// with chance 2^-16 like any unrelated value, and such a match is resolved
+// by the exact rescan.  (With about a thousand probes per prime, clearing
+// the table costs more LDS write cycles than the extra rescans cost.)  The
+// shared-table kernels clear their slots to zero, so a probe whose
+// fingerprint is 0 matches the empty ones.  Slot occupancy is read from the
 // fill counts, never from the slots.  The bucket's byte offset in the table
 // is lo & (nbm << 4), which a table aligned to its size adds by OR.  (Above
 // 4096 buckets the bucket bits reach into the fingerprint, whose distinct
@@ -96,10 +102,6 @@ __device__ __forceinline__ u32 bucket_of(u32 lo, u32 logNB) {
     return __builtin_amdgcn_ubfe(lo, 4, logNB);
 }
 __device__ __forceinline__ unsigned short fingerprint(u32 lo) { return (unsigned short)(lo >> 16); }
-// The fingerprint of a low word in both halves (one byte permute).
-__device__ __forceinline__ u32 fingerprint2(u32 lo) {
-    return __builtin_amdgcn_perm(lo, lo, 0x03020302u);
-}
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -111,38 +113,64 @@ __device__ __forceinline__ uint4 lds_bucket(u32 a) {
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
-// True iff one of the 8 16-bit slots of e equals the fingerprint in both
-// halves of fp2: the packed minimum of the XORs, folded with its halves
-// swapped, has a zero low half.
-__device__ __forceinline__ bool bucket_match(uint4 e, u32 fp2) {
-    const u16x2 a = __builtin_bit_cast(u16x2, e.x ^ fp2), b = __builtin_bit_cast(u16x2, e.y ^ fp2);
-    const u16x2 c = __builtin_bit_cast(u16x2, e.z ^ fp2), d = __builtin_bit_cast(u16x2, e.w ^ fp2);
+// The LDS byte address of a pointer into LDS, and a uint2 at one.
+template <typename T>
+__device__ __forceinline__ u32 lds_addr(const T* p) {
+    return (u32)(size_t)(const __attribute__((address_space(3))) T*)p;
+}
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint2 lds_ld2(u32 a) {
+    const u32x2 v = *(const __attribute__((address_space(3))) u32x2*)(size_t)a;
+    return make_uint2(v.x, v.y);
+}
+
+// True iff one of the 8 16-bit slots of e equals the fingerprint of the
+// low word lo (its high half): the slots minus the fingerprint (packed
+// 16-bit subtractions whose second operand takes the high half of lo for
+// both halves, so the fingerprint needs no instruction of its own), their
+// packed minimum, folded with its halves swapped, has a zero low half.
+__device__ __forceinline__ bool bucket_match(uint4 e, u32 lo) {
+    const u16x2 l = __builtin_bit_cast(u16x2, lo);
+    const u16x2 f = __builtin_shufflevector(l, l, 1, 1);
+    const u16x2 a = __builtin_bit_cast(u16x2, e.x) - f, b = __builtin_bit_cast(u16x2, e.y) - f;
+    const u16x2 c = __builtin_bit_cast(u16x2, e.z) - f, d = __builtin_bit_cast(u16x2, e.w) - f;
     const u16x2 mn = __builtin_elementwise_min(__builtin_elementwise_min(a, b),
                                                __builtin_elementwise_min(c, d));
-    const u16x2 f = __builtin_elementwise_min(mn, __builtin_shufflevector(mn, mn, 1, 0));
-    return f.x == 0;
+    const u16x2 g = __builtin_elementwise_min(mn, __builtin_shufflevector(mn, mn, 1, 0));
+    return g.x == 0;
 }
 
 // Active-row entries of the giant steps (built on the host, JobDev.rowent):
-// .x = k, .y = 8*r*ys (byte offset of the row's Y run, ys the Y row stride;
-// < 2^19) | (row - slice start) << 19.
+// .x = k, .y = 8*r*ys (byte offset of the row's Y run, ys the Y row stride).
 
-
+// The job row of the entry (k, yoff): rows are ordered by k, then r (so by
+// Y offset), and no two are equal, so a binary search of the job's row
+// table finds it (only a hit in a job with n-list bitmaps needs it).
+__device__ __forceinline__ int find_row(const uint2* __restrict__ rowent, int nrows, u32 k,
+                                        u32 yoff) {
+    int lo = 0, hi = nrows - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint2 e = global_ld(rowent + mid);
+        if (e.x < k || (e.x == k && e.y < yoff)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // Emits every baby step j with g^j == target (all 64 lanes take part:
 // lane l checks j = l + 64 v) as a hit of giant step tt of the row whose
-// entry is (k, yent) in the slice starting at row r0.  With n-list bitmaps
-// only n in the row's list is emitted.  p, mp and g^64 come from the
+// entry is (k, yent).  With n-list bitmaps only n in the row's list is
+// emitted.  p, mp and g^64 come from the
 // prime's record R in LDS (a hit is rare: they need not stay in registers
 // through the giant steps); g^64 there is a Montgomery form, so the chain
 // keeps vstart's form: plain for p < 2^50, else Montgomery.
 __device__ __forceinline__ void rescan(const JobDev& J, const JobDev* __restrict__ jobs, int job,
-                                       const u64* R, u64 yv, u32 k, u32 yent, int r0, int tt,
+                                       const u64* R, u64 yv, u32 k, u32 yent, int tt,
                                        u64 vstart, HitDev* __restrict__ hits,
                                        u32* __restrict__ hit_count, u32 hit_cap) {
     const int lane = threadIdx.x & 63;
-    const u32 r = ((yent & 0x7ffffu) >> 3) / (u32)J.ys;
-    const u32 row = (u32)r0 + (yent >> 19);
+    const u32 r = (yent >> 3) / (u32)J.ys;
     const u64 p = R[REC_P], mp = R[REC_MP], g64 = R[REC_G64];
     const u64 target = shoup32_full(k, yv, p);  // the exact product of the item
     u64 w = vstart;
@@ -159,6 +187,7 @@ __device__ __forceinline__ void rescan(const JobDev& J, const JobDev* __restrict
             const uint32_t* const bitmap = const_ld(&jp->bitmap);
             if (keep && bitmap) {
                 const long long xp = xx - x0;  // x' of the bitmap, < 32 words
+                const int row = find_row(const_ld(&jp->rowent), J.nrows, k, yent);
                 keep = (global_ld(bitmap + (long long)row * const_ld(&jp->words) + (xp >> 5)) >>
                         (31 - (xp & 31))) & 1;
             }
@@ -247,40 +276,52 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
             if (tt >= T) tt -= T, aa += 1;
         }
     };
-    // PF (full row entries only): the next round's entry is read during
-    // this round, unconditionally (clamped), so its wait is a round later
+    // PF (full row entries only): the lane's entry is kept as its LDS byte
+    // address la (the lanes past the end read the last entry, llast), and
+    // the next round's entry is read during this round, unconditionally
+    // (after the round's products, into the registers they free), so its
+    // wait is a round later; a rescan reads its entry again
+    const u32 llast = lds_addr(list + (A - 1));  // A >= 1 when the loop runs
+    const u32 lstep = 8 * rstep;
+    u32 la = lds_addr(list + a);
     uint2 rown = make_uint2(0, 0);
-    if (PF && A > 0) rown = list[min(a, A - 1)];
+    if (PF && A > 0) rown = lds_ld2(min(la, llast));
     for (int base = 0; base < total; base += 64 * TW) {
-        const bool ok = a < A;
-        const u64 okm = lanes(ok);
         // the row's entry: in the list, or (idx) behind its row number
         uint2 row;
         const int tcur = t0;
+        const u32 lcur = la;
         if (PF) {
             row = rown;
-            advance(a, t0);
-            rown = list[min(a, A - 1)];
+            la += lstep;
+            if (!EXACT) {
+                t0 += tstep;
+                if (t0 >= T) t0 -= T, la += 8;
+            }
         } else {
             const int ac = min(a, A - 1);
             row = idx ? rows_lds[list16[ac]] : list[ac];
         }
-        const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * tcur);
+        // (EXACT: tested on the advanced address, so lcur need not be kept)
+        const bool ok = PF ? (EXACT ? la <= llast + lstep : lcur <= llast) : a < A;
+        const u64 okm = lanes(ok);
+        const u64* const yp = (const u64*)(Yb + row.y + 8 * tcur);
         uint4 e[TW];
-        u32 fp2[TW], ba[TW];
+        u32 lw[TW], ba[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             const u32 lo = shoup32_lo(row.x, lo32(yp[u]), hi32(yp[u]), negp0);
             ba[u] = (lo & bmask) | tab_addr;  // the bucket's LDS byte address
-            fp2[u] = fingerprint2(lo);
+            lw[u] = lo;
         }
+        if (PF) rown = lds_ld2(min(la, llast));
 #pragma unroll
         for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
         u64 mm[TW], full[TW];
         u64 anyf = 0;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            mm[u] = lanes(bucket_match(e[u], fp2[u])) & okm;
+            mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
             if (CHAIN) {
                 full[u] = lanes(cnt[(ba[u] - tab_addr) >> 4] >= 8) & okm;
                 anyf |= full[u];
@@ -296,7 +337,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
                     while (true) {
                         b = (b + 1) & nbm;
                         const uint4 f = lds_bucket(tab_addr + 16 * b);
-                        if (bucket_match(f, fp2[u])) hit = true;
+                        if (bucket_match(f, lw[u])) hit = true;
                         if (cnt[b] < 8) break;
                     }
                 }
@@ -309,11 +350,20 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
             while (mb) {
                 const int src = __builtin_ctzll(mb);
                 mb &= mb - 1;
-                const u32 k = __builtin_amdgcn_readlane(row.x, src);
-                const u32 yent = __builtin_amdgcn_readlane(row.y, src);
+                uint2 re;
+                if (PF) {
+                    const u32 at = EXACT ? __builtin_amdgcn_readlane(la, src) - lstep
+                                         : __builtin_amdgcn_readlane(lcur, src);
+                    re = lds_ld2(at);
+                } else {
+                    re = make_uint2(__builtin_amdgcn_readlane(row.x, src),
+                                    __builtin_amdgcn_readlane(row.y, src));
+                }
+                const u32 k = __builtin_amdgcn_readfirstlane(re.x);
+                const u32 yent = __builtin_amdgcn_readfirstlane(re.y);
                 const int tt = __builtin_amdgcn_readlane(tcur, src) + u;
-                const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
-                rescan(J, jobs, job, R, yv, k, yent, r0, tt, vstart, hits, hit_count, hit_cap);
+                const u64 yv = *(const u64*)(Yb + yent + 8 * tt);
+                rescan(J, jobs, job, R, yv, k, yent, tt, vstart, hits, hit_count, hit_cap);
             }
         }
         if (!PF) advance(a, t0);
@@ -607,8 +657,7 @@ __device__ __forceinline__ void bsgs_prime(const JobDev& J, const JobDev* __rest
     const u64 p = readlane64(pre, REC_P), mp = readlane64(pre, REC_MP);
     const u64 gw = readlane64(pre, REC_GW), gws = readlane64(pre, REC_GWS);
 
-    // clear table and counts
-    for (u32 b = lane; b < NB; b += 64) tab[b] = make_uint4(0, 0, 0, 0);
+    // clear the fill counts (the slots are not cleared: see bucket_of)
     for (u32 c = 4 * lane; c < NB; c += 256) *(uint4*)(cnt + c) = make_uint4(0, 0, 0, 0);
     wave_sync();
 
@@ -954,7 +1003,7 @@ __device__ __forceinline__ void rescan_group(const JobDev& J, const JobDev* __re
                 gj = mont_pow(R[REC_GP + 1], (u32)j, R[REC_GP], p, R[REC_MP]);
             }
             if (gj != target) continue;
-            const u32 r = ((yent & 0x7ffffu) >> 3) / (u32)J.ys;
+            const u32 r = (yent >> 3) / (u32)J.ys;
             const JobDev* const jp = jobs + job;
             const int x0 = const_ld(&jp->x0);
             const long long xx = (long long)x0 + (long long)tt * J.m + (J.m - 1 - j);
@@ -1029,14 +1078,14 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
         rnum = rnum1;
         rnum1 = rnum2;
         if (on) a += RR;
-        const u64* const yp = (const u64*)(Yb + (row.y & 0x7ffffu) + 8 * tcur);
+        const u64* const yp = (const u64*)(Yb + row.y + 8 * tcur);
         uint4 e[TW];
-        u32 fp2[TW], ba[TW];
+        u32 lw[TW], ba[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             const u32 lo = shoup32_lo(row.x, lo32(yp[u]), hi32(yp[u]), negp0);
             ba[u] = lo & bmask;  // the bucket's LDS byte address
-            fp2[u] = fingerprint2(lo);
+            lw[u] = lo;
         }
 #pragma unroll
         for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
@@ -1044,7 +1093,7 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
         u64 anyf = 0;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            mm[u] = lanes(bucket_match(e[u], fp2[u])) & okm;
+            mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
             if (CHAIN) {
                 full[u] = lanes(cnt[ba[u] >> 4] >= 8) & okm;
                 anyf |= full[u];
@@ -1059,7 +1108,7 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
                     while (true) {
                         b = (b + 1) & nbm;
                         const uint4 f = lds_bucket(16 * b);
-                        if (bucket_match(f, fp2[u])) hit = true;
+                        if (bucket_match(f, lw[u])) hit = true;
                         if (cnt[b] < 8) break;
                     }
                 }
@@ -1076,7 +1125,7 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
                 const u32 yent = __builtin_amdgcn_readlane(row.y, src);
                 const int tt = __builtin_amdgcn_readlane(tcur, src) + u;
                 const int rw = r0 + __builtin_amdgcn_readlane(rcur, src);
-                const u64 yv = *(const u64*)(Yb + (yent & 0x7ffffu) + 8 * tt);
+                const u64 yv = *(const u64*)(Yb + yent + 8 * tt);
                 rescan_group(J, jobs, job, R, yv, k, yent, rw, tt, low, hits, hit_count,
                              hit_cap);
             }
@@ -1116,6 +1165,9 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
     // the record: word tid came in `pre`, loaded one prime ahead
     if (tid < W) R[tid] = pre;
     for (int w = tid + 64 * NV; w < W; w += 64 * NV) R[w] = Rg[w];
+    // the slots are cleared too: with tens of thousands of probes per
+    // prime, the stale slots' false matches would cost more rescans (each
+    // over m stored low words) than the clear
     for (u32 b = tid; b < NB; b += 64 * NV) tab[b] = make_uint4(0, 0, 0, 0);
     for (u32 c = tid; c < NB / 2; c += 64 * NV) cnt[c] = 0;
     if (tid == 0) misc[0] = 0;
