@@ -153,7 +153,8 @@ static inline int lds_words_per_wave(int logNB, int Q, int ys, int nrows, int W,
 
 // LDS words of one workgroup whose wavefronts share one prime's table
 // (nwv > 0): the table first, at LDS address 0 (so aligned to its size),
-//   4 NB                      NB buckets x 8 uint16 fingerprints (0 = empty)
+//   4 NB                      NB buckets x 8 uint16 fingerprints (slots past
+//                             a bucket's fill count hold stale values)
 //   NB / 2                    bucket fill counts (uint16)
 //   2 Q ys                    Y entries (uint64)
 //   2 W                       the prime's record (uint64)

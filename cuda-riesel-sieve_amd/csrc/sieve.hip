@@ -85,16 +85,15 @@ __device__ __forceinline__ uint4 const_ld(const uint4* p) {
 __device__ __forceinline__ int div_T(u32 idx, u32 invT) { return (int)__umulhi(idx, invT); }
 
 // Table hash of a residue: the bucket is taken from bits 4 and up of its
-// low word and the 16-bit fingerprint is its high half.  The per-wavefront
-// kernels clear only the fill counts for each prime, not the slots: a slot
-// past its bucket's count holds a fingerprint of another prime's baby step
-// (or whatever the LDS held at launch), which equals a probe's fingerprint
-// with chance 2^-16 like any unrelated value, and such a match is resolved
-// by the exact rescan.  (With about a thousand probes per prime, clearing
-// the table costs more LDS write cycles than the extra rescans cost.)  The
-// shared-table kernels clear their slots to zero, so a probe whose
-// fingerprint is 0 matches the empty ones.  Slot occupancy is read from the
-// fill counts, never from the slots.  The bucket's byte offset in the table
+// low word and the 16-bit fingerprint is its high half.  Only the fill
+// counts are cleared for each prime, not the slots: a slot past its
+// bucket's count holds a fingerprint of another prime's baby step (or
+// whatever the LDS held at launch), which equals a probe's fingerprint with
+// chance 2^-16 like any unrelated value.  The probe compares all eight
+// slots at once; only when one matches does the bucket's count decide
+// whether a filled slot matched (bucket_match_live), and only then does the
+// exact rescan run.  (Clearing the table would cost more LDS write cycles
+// than that rare test.)  The bucket's byte offset in the table
 // is lo & (nbm << 4), which a table aligned to its size adds by OR.  (Above
 // 4096 buckets the bucket bits reach into the fingerprint, whose distinct
 // bits within a bucket drop below 16: only the rescan rate grows.)
@@ -138,6 +137,19 @@ __device__ __forceinline__ bool bucket_match(uint4 e, u32 lo) {
                                                __builtin_elementwise_min(c, d));
     const u16x2 g = __builtin_elementwise_min(mn, __builtin_shufflevector(mn, mn, 1, 0));
     return g.x == 0;
+}
+
+// Whether one of the first c slots of bucket e (the slots its inserts have
+// filled) holds the fingerprint of lo: the exact test behind a
+// bucket_match, run only when one matched (a match in a slot past the fill
+// count is a stale or empty slot, which no baby step of this prime wrote).
+__device__ __forceinline__ bool bucket_match_live(uint4 e, u32 lo, u32 c) {
+    const u32 f = lo >> 16;
+    const u32 w[4] = {e.x, e.y, e.z, e.w};
+    bool m = false;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) m |= (u32)s < c && ((w[s >> 1] >> (16 * (s & 1))) & 0xffffu) == f;
+    return m;
 }
 
 // Active-row entries of the giant steps (built on the host, JobDev.rowent):
@@ -318,13 +330,23 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
 #pragma unroll
         for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
         u64 mm[TW], full[TW];
-        u64 anyf = 0;
+        u64 anyf = 0, anym = 0;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
+            anym |= mm[u];
             if (CHAIN) {
                 full[u] = lanes(cnt[(ba[u] - tab_addr) >> 4] >= 8) & okm;
                 anyf |= full[u];
+            }
+        }
+        // rare: keep only the matches in filled slots
+        if (anym) {
+#pragma unroll
+            for (int u = 0; u < TW; ++u) {
+                u32 c = 0;
+                if (in_mask(mm[u])) c = cnt[(ba[u] - tab_addr) >> 4];
+                mm[u] &= lanes(bucket_match_live(e[u], lw[u], c));
             }
         }
         // full buckets: follow the chain to the first bucket that is not full
@@ -337,8 +359,9 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
                     while (true) {
                         b = (b + 1) & nbm;
                         const uint4 f = lds_bucket(tab_addr + 16 * b);
-                        if (bucket_match(f, lw[u])) hit = true;
-                        if (cnt[b] < 8) break;
+                        const u32 cb = cnt[b];
+                        if (bucket_match(f, lw[u]) && bucket_match_live(f, lw[u], cb)) hit = true;
+                        if (cb < 8) break;
                     }
                 }
                 mm[u] |= lanes(hit);
@@ -1090,13 +1113,23 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
 #pragma unroll
         for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
         u64 mm[TW], full[TW];
-        u64 anyf = 0;
+        u64 anyf = 0, anym = 0;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
+            anym |= mm[u];
             if (CHAIN) {
                 full[u] = lanes(cnt[ba[u] >> 4] >= 8) & okm;
                 anyf |= full[u];
+            }
+        }
+        // rare: keep only the matches in filled slots
+        if (anym) {
+#pragma unroll
+            for (int u = 0; u < TW; ++u) {
+                u32 c = 0;
+                if (in_mask(mm[u])) c = cnt[ba[u] >> 4];
+                mm[u] &= lanes(bucket_match_live(e[u], lw[u], c));
             }
         }
         if (CHAIN && anyf) {
@@ -1108,8 +1141,9 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
                     while (true) {
                         b = (b + 1) & nbm;
                         const uint4 f = lds_bucket(16 * b);
-                        if (bucket_match(f, lw[u])) hit = true;
-                        if (cnt[b] < 8) break;
+                        const u32 cb = cnt[b];
+                        if (bucket_match(f, lw[u]) && bucket_match_live(f, lw[u], cb)) hit = true;
+                        if (cb < 8) break;
                     }
                 }
                 mm[u] |= lanes(hit);
@@ -1165,11 +1199,7 @@ __device__ __forceinline__ void bsgs_prime_group(const JobDev& J, const JobDev* 
     // the record: word tid came in `pre`, loaded one prime ahead
     if (tid < W) R[tid] = pre;
     for (int w = tid + 64 * NV; w < W; w += 64 * NV) R[w] = Rg[w];
-    // the slots are cleared too: with tens of thousands of probes per
-    // prime, the stale slots' false matches would cost more rescans (each
-    // over m stored low words) than the clear
-    for (u32 b = tid; b < NB; b += 64 * NV) tab[b] = make_uint4(0, 0, 0, 0);
-    for (u32 c = tid; c < NB / 2; c += 64 * NV) cnt[c] = 0;
+    for (u32 c = tid; c < NB / 2; c += 64 * NV) cnt[c] = 0;  // the slots are not cleared
     if (tid == 0) misc[0] = 0;
     __syncthreads();
     const u64 p = R[REC_P], mp = R[REC_MP];
