@@ -288,16 +288,23 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
             if (tt >= T) tt -= T, aa += 1;
         }
     };
-    // PF (full row entries only): the lane's entry is kept as its LDS byte
-    // address la (the lanes past the end read the last entry, llast), and
-    // the next round's entry is read during this round, unconditionally
-    // (after the round's products, into the registers they free), so its
-    // wait is a round later; a rescan reads its entry again
-    const u32 llast = lds_addr(list + (A - 1));  // A >= 1 when the loop runs
-    const u32 lstep = 8 * rstep;
-    u32 la = lds_addr(list + a);
+    // PF: the lane's list element is kept as its LDS byte address la (the
+    // lanes past the end read the last element, llast), and the next
+    // round's entry is read during this round, unconditionally (after the
+    // round's products, into the registers they free), so its wait is a
+    // round later; with idx the next round's row number is read with the
+    // round's bucket reads and its entry after the match tests.  A rescan
+    // reads its entry again.
+    const u32 es = idx ? 2u : 8u;  // bytes per list element
+    const u32 llast = lds_addr(list) + es * (A - 1);  // A >= 1 when the loop runs
+    const u32 lstep = es * rstep;
+    u32 la = lds_addr(list) + es * a;
+    auto lds_u16 = [](u32 x) -> u32 {
+        return *(const __attribute__((address_space(3))) uint16_t*)(size_t)x;
+    };
+    auto entry_at = [&](u32 x) { return idx ? rows_lds[lds_u16(x)] : lds_ld2(x); };
     uint2 rown = make_uint2(0, 0);
-    if (PF && A > 0) rown = lds_ld2(min(la, llast));
+    if (PF && A > 0) rown = entry_at(min(la, llast));
     for (int base = 0; base < total; base += 64 * TW) {
         // the row's entry: in the list, or (idx) behind its row number
         uint2 row;
@@ -308,7 +315,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
             la += lstep;
             if (!EXACT) {
                 t0 += tstep;
-                if (t0 >= T) t0 -= T, la += 8;
+                if (t0 >= T) t0 -= T, la += es;
             }
         } else {
             const int ac = min(a, A - 1);
@@ -326,7 +333,11 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
             ba[u] = (lo & bmask) | tab_addr;  // the bucket's LDS byte address
             lw[u] = lo;
         }
-        if (PF) rown = lds_ld2(min(la, llast));
+        u32 rnum = 0;
+        if (PF) {
+            if (idx) rnum = lds_u16(min(la, llast));
+            else rown = lds_ld2(min(la, llast));
+        }
 #pragma unroll
         for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
         u64 mm[TW], full[TW];
@@ -340,6 +351,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
                 anyf |= full[u];
             }
         }
+        if (PF && idx) rown = rows_lds[rnum];
         // rare: keep only the matches in filled slots
         if (anym) {
 #pragma unroll
@@ -377,7 +389,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
                 if (PF) {
                     const u32 at = EXACT ? __builtin_amdgcn_readlane(la, src) - lstep
                                          : __builtin_amdgcn_readlane(lcur, src);
-                    re = lds_ld2(at);
+                    re = entry_at(at);
                 } else {
                     re = make_uint2(__builtin_amdgcn_readlane(row.x, src),
                                     __builtin_amdgcn_readlane(row.y, src));
@@ -1410,6 +1422,9 @@ __device__ __forceinline__ void sieve_body_group(
 #ifndef RS_BB5
 #define RS_BB5 2
 #endif
+#ifndef RS_RIX_PF
+#define RS_RIX_PF true
+#endif
 // baby-step blocks of the 128- and the 96-VGPR builds
 constexpr int BB4 = RS_BB4, BB5 = RS_BB5;
 
@@ -1452,7 +1467,7 @@ rs_sieve_kernel5_rix(ChunkDev c0, const JobDev* __restrict__ jobs,
                      uint64_t* __restrict__ scratch, unsigned long long* __restrict__ work,
                      HitDev* __restrict__ hits, uint32_t* __restrict__ hit_count,
                      uint32_t hit_cap) {
-    sieve_body<true, BB5, false>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words,
+    sieve_body<true, BB5, RS_RIX_PF>(c0, jobs, chunks, nchunks, ngroups, lds_words, tab_words,
                                  shared_words, scratch_words, scratch, work, hits, hit_count,
                                  hit_cap);
 }
