@@ -38,6 +38,8 @@ struct DevJob {
     uint4* d_fac = nullptr;
     uint32_t* d_kw0 = nullptr;   // row-mask words of each k
     uint64_t* d_kmask = nullptr;
+    uint32_t* d_kval = nullptr;   // each distinct k
+    uint64_t* d_kmask3 = nullptr; // its rows by r mod 3
     uint2* d_rowent = nullptr;
     uint32_t* d_bitmap = nullptr;
     JobDev* d_self = nullptr;    // J itself in device memory

@@ -33,8 +33,12 @@ void free_job(DevJob& j) {
     j.d_kfac = j.d_fac = nullptr;
     (void)hipFree(j.d_kw0);
     (void)hipFree(j.d_kmask);
+    (void)hipFree(j.d_kval);
+    (void)hipFree(j.d_kmask3);
     j.d_kw0 = nullptr;
     j.d_kmask = nullptr;
+    j.d_kval = nullptr;
+    j.d_kmask3 = nullptr;
     (void)hipFree(j.d_rowent);
     (void)hipFree(j.d_bitmap);
     j.d_kvals = nullptr;
@@ -173,7 +177,9 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     // most 16384 (bucket counts are 16-bit: at most 2m inserts per bucket)
     if (nwv && (row_index || m % (64 * nwv) != 0 || m > 16384)) return RS_ERR_ARG;
     const bool sparse = (tw & RS_TILE_SPARSE) != 0;
-    tw &= ~(RS_TILE_ROW_INDEX | RS_TILE_GROUP4 | RS_TILE_GROUP8 | RS_TILE_SPARSE);
+    const bool want_cubic = (tw & RS_TILE_CUBIC) != 0;
+    tw &= ~(RS_TILE_ROW_INDEX | RS_TILE_GROUP4 | RS_TILE_GROUP8 | RS_TILE_SPARSE |
+            RS_TILE_CUBIC);
     if (tw < 0) return RS_ERR_ARG;
     if (tw == 0) tw = 2;
     if (tw > 4) return RS_ERR_ARG;
@@ -230,15 +236,24 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     J.tab_words = 4 << J.logNB;
     // per k: the 64-row word of its first row and its even-r / odd-r rows
     // in that word and the next (a k has at most Q < 64 rows)
+    // (and its rows by r mod 3, for the cubic-residue test)
     std::vector<uint32_t> kw0(kvals.size(), 0xffffffffu);
-    std::vector<uint64_t> kmask(4 * kvals.size(), 0);
+    std::vector<uint64_t> kmask(4 * kvals.size(), 0), kmask3(6 * kvals.size(), 0);
     for (int i = 0; i < nrows; ++i) {
         const uint32_t kidx = rowinfo[i] & 0xffffu, r = rowinfo[i] >> 16;
         if (kw0[kidx] == 0xffffffffu) kw0[kidx] = (uint32_t)(i >> 6);
         const uint32_t dw = (uint32_t)(i >> 6) - kw0[kidx];
         if (dw > 1) return RS_ERR_ARG;  // rows of one k are contiguous and < 64
         kmask[4 * kidx + 2 * (r & 1) + dw] |= 1ull << (i & 63);
+        kmask3[6 * kidx + 2 * (r % 3) + dw] |= 1ull << (i & 63);
     }
+    // the cubic-residue test: n = r (mod 3) when 3 | Q; its work area must
+    // fit the wavefront's table slot during the set-up
+    J.cubic = want_cubic && J.Q % 3 == 0 && nwv == 0 && J.nk <= CUBIC_MAX_K &&
+                      cubic_area_bytes(J.nk) <= 4 * J.tab_words
+                  ? 1
+                  : 0;
+    J.invnk = J.nk > 1 ? magic_div(J.nk) : 0;
     std::vector<uint4> kjac(kvals.size());
     for (size_t i = 0; i < kvals.size(); ++i) kjac[i] = jacobi_consts(kvals[i]);
     J.bjac = jacobi_consts(pl->base);
@@ -284,10 +299,16 @@ rs_status make_job(const rs_plan* pl, int m, int tw, bool dat, DevJob* dj) {
     }
     HIPCHK(hipMalloc(&dj->d_kw0, kw0.size() * 4));
     HIPCHK(hipMalloc(&dj->d_kmask, kmask.size() * 8));
+    HIPCHK(hipMalloc(&dj->d_kval, kvals.size() * 4));
+    HIPCHK(hipMalloc(&dj->d_kmask3, kmask3.size() * 8));
     HIPCHK(hipMemcpy(dj->d_kw0, kw0.data(), kw0.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dj->d_kmask, kmask.data(), kmask.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dj->d_kval, kvals.data(), kvals.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dj->d_kmask3, kmask3.data(), kmask3.size() * 8, hipMemcpyHostToDevice));
     J.kw0 = dj->d_kw0;
     J.kmask = dj->d_kmask;
+    J.kval = dj->d_kval;
+    J.kmask3 = dj->d_kmask3;
     J.rowent = dj->d_rowent;
     J.words = pl->words;
     J.bitmap = nullptr;

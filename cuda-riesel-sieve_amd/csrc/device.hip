@@ -139,7 +139,7 @@ extern "C" rs_status rs_sieve_info(const rs_sieve* s, int* m, int* T, int* table
     if (tile)
         *tile = s->job.J.tw | (s->job.J.list_idx ? RS_TILE_ROW_INDEX : 0) |
                 (s->job.J.nwv == 4 ? RS_TILE_GROUP4 : s->job.J.nwv == 8 ? RS_TILE_GROUP8 : 0) |
-                (s->job.J.sparse ? RS_TILE_SPARSE : 0);
+                (s->job.J.sparse ? RS_TILE_SPARSE : 0) | (s->job.J.cubic ? RS_TILE_CUBIC : 0);
     if (m) *m = s->job.J.m;
     if (T) *T = s->job.J.T;
     if (table_slots) *table_slots = 8 << s->job.J.logNB;
@@ -428,6 +428,13 @@ extern "C" rs_status rs_dev_row_masks(int device, const rs_plan* pl, const uint6
     if (m < 64) return RS_ERR_ARG;
     DevJob dj;
     rs_status st = make_job(pl, m, 0, false, &dj);
+    // the cubic test wherever it applies (the test kernel brings its own
+    // work area, whatever the table size)
+    if (st == RS_OK && dj.J.Q % 3 == 0 && dj.J.nk <= CUBIC_MAX_K && !dj.J.cubic) {
+        dj.J.cubic = 1;
+        if (hipMemcpy(dj.d_self, &dj.J, sizeof(JobDev), hipMemcpyHostToDevice) != hipSuccess)
+            st = RS_ERR_HIP;
+    }
     const int nw = (dj.J.nrows + 63) / 64;
     uint64_t *dp = nullptr, *drec = nullptr, *dout = nullptr;
     if (st == RS_OK &&
@@ -438,8 +445,9 @@ extern "C" rs_status rs_dev_row_masks(int device, const rs_plan* pl, const uint6
     if (st == RS_OK && hipMemcpy(dp, p, (size_t)count * 8, hipMemcpyHostToDevice) != hipSuccess)
         st = RS_ERR_HIP;
     if (st == RS_OK) {
-        hipLaunchKernelGGL(rs_test_row_masks, dim3((unsigned)((count + 63) / 64)), dim3(64), 0,
-                           0, dj.d_self, dp, drec, dout, (long long)count);
+        hipLaunchKernelGGL(rs_test_row_masks, dim3((unsigned)((count + 63) / 64)), dim3(64),
+                           dj.J.cubic ? cubic_area_bytes(dj.J.nk) : 0, 0, dj.d_self, dp, drec,
+                           dout, (long long)count);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpy(masks, dout, (size_t)count * nw * 8, hipMemcpyDeviceToHost) != hipSuccess)
             st = RS_ERR_HIP;
@@ -652,6 +660,19 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
         rs_status st = tu.time(tu.best_m, tu.best_tile | RS_TILE_SPARSE, &ns);
         if (st != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip && st != RS_ERR_NOMEM)
             return st;
+    }
+    // the cubic-residue row test at the best per-wavefront pair, where it
+    // applies (Q a multiple of 3, few enough distinct k)
+    if (tu.best_m && pl->Q % 3 == 0 && !(tu.best_tile & (RS_TILE_GROUP4 | RS_TILE_GROUP8))) {
+        size_t nk = 0;
+        for (size_t i = 0; i < pl->rows_k.size(); ++i)
+            nk += i == 0 || pl->rows_k[i] != pl->rows_k[i - 1];
+        if (nk <= CUBIC_MAX_K) {
+            double ns = 0;
+            rs_status st = tu.time(tu.best_m, tu.best_tile | RS_TILE_CUBIC, &ns);
+            if (st != RS_OK && st != RS_ERR_ARG && st != Tuner::kSkip && st != RS_ERR_NOMEM)
+                return st;
+        }
     }
     if (ncand) *ncand = tu.n;
     *best_m = tu.best_m;

@@ -88,7 +88,21 @@ struct JobDev {
     int off_x;      // record words of the shared-table layout (nwv > 0)
     int lds_words_wg;  // 32-bit LDS words of one workgroup (nwv > 0)
     int sparse;        // 1: twice the buckets of the load rule (RS_TILE_SPARSE)
+    int cubic;         // 1 (RS_TILE_CUBIC): for primes p = 1 (mod 3) below
+                       // 2^50 the rows must also pass the cubic-residue test
+                       // (Q a multiple of 3; per-wavefront layouts whose table
+                       // holds the set-up's work area, cubic_area_bytes)
+    uint32_t invnk;    // ceil(2^32 / nk) for nk > 1: j / nk for j < 2^16
+    const uint32_t* kval;     // per distinct k (ascending): k
+    const uint64_t* kmask3;   // per distinct k: its rows with r = 0, 1, 2 (mod 3)
+                              // in words kw0 and kw0 + 1 (6 words)
 };
+
+// The set-up's cubic work area in a wavefront's table slot (bytes): 64 slot
+// records of CUBIC_SLOT_BYTES (p, 1/p and the exponent (p-1)/3 of one prime,
+// b^e and b^2e), then one class byte per (prime slot, k).
+enum { CUBIC_SLOT_BYTES = 40, CUBIC_MAX_K = 1024 };
+__host__ __device__ inline int cubic_area_bytes(int nk) { return 64 * CUBIC_SLOT_BYTES + 64 * nk; }
 
 // A chunk of primes p = low + offs[i], i < count, of one job; group0 is
 // its first 64-prime group in the launch's global numbering.  When

@@ -580,8 +580,81 @@ __device__ __forceinline__ bool prep_powers_fp(const JobDev& J, u64 p, u64* __re
     return bd == 0.0;
 }
 
+// Cubic-residue classes of the job's k for the primes of one wavefront's
+// set-up (lane l holds prime p; valid lanes only).  For p = 1 (mod 3), the
+// cubic character x^e, e = (p-1)/3, is 1, w or w^2 for a cube root of
+// unity w; k b^n = 1 (mod p) needs k^e b^(n e) = 1, and n = r (mod 3) when
+// 3 | Q.  With u = b^e: if u = 1 a row passes iff k^e = 1 (every r or
+// none); else, taking w = u, k^e = u^c passes exactly the r = -c (mod 3).
+// Each such prime below 2^50 gets a slot: its lane computes u and u^2 (in
+// double precision, fp_mulmod) into the slot record at LDS byte address
+// `area`; then the (slot, k) pairs are spread over the 64 lanes, each
+// computing k^e and writing one class byte, the set of residues r mod 3 its
+// rows may have (bit rho: r = rho), after the slot records.  pmax bounds the
+// wavefront's primes (it fixes the exponent's bit length).  Returns the
+// lane's slot, or -1 (no cubic test for its prime).
+__device__ __forceinline__ int cubic_classes(const JobDev& J, u64 p, bool valid, u64 pmax,
+                                             u32 area) {
+    const int lane = threadIdx.x & 63;
+    const bool c3 = valid && p < kFpMaxP && (hi32(p) % 3u + lo32(p) % 3u) % 3u == 1;
+    const u64 cm = lanes(c3);
+    if (cm == 0) return -1;
+    const int nc = __popcll(cm);
+    const int slot =
+        (int)__builtin_amdgcn_mbcnt_hi(hi32(cm), __builtin_amdgcn_mbcnt_lo(lo32(cm), 0));
+    const u64 emax = (min(pmax, kFpMaxP) - 1) / 3;
+    const int top = 63 - __builtin_clzll(emax | 1);
+    // x^e mod p, canonical, for x in [0, p) (left to right over the bits)
+    auto powe = [&](double x, u64 e, double pd, double pinv) {
+        double r = 1.0;
+        for (int i = top; i >= 0; --i) {
+            r = fp_mulmod(r, r, pd, pinv);
+            if ((e >> i) & 1) r = fp_mulmod(r, x, pd, pinv);
+        }
+        return r < 0.0 ? r + pd : r;
+    };
+    // x mod p for a 32-bit x (p <= x only for p below 2^32)
+    auto reduce = [](u32 x, u64 pp) { return pp > x ? (double)x : (double)(x % (u32)pp); };
+    typedef __attribute__((address_space(3))) double lds_f64;
+    typedef __attribute__((address_space(3))) u64 lds_u64;
+    if (c3) {
+        const u32 a = area + CUBIC_SLOT_BYTES * slot;
+        const double pd = (double)p, pinv = 1.0 / pd;
+        const u64 e = (p - 1) * 0xAAAAAAAAAAAAAAABull;  // (p - 1) / 3, exact
+        const double u = powe(reduce(J.base, p), e, pd, pinv);
+        double u2 = fp_mulmod(u, u, pd, pinv);
+        u2 = u2 < 0.0 ? u2 + pd : u2;
+        *(lds_f64*)(size_t)a = pd;
+        *(lds_f64*)(size_t)(a + 8) = pinv;
+        *(lds_u64*)(size_t)(a + 16) = e;
+        *(lds_f64*)(size_t)(a + 24) = u;
+        *(lds_f64*)(size_t)(a + 32) = u2;
+    }
+    wave_sync();
+    const int nk = J.nk, total = nc * nk;  // total < 2^16 (nk <= CUBIC_MAX_K)
+    for (int j = lane; j < total; j += 64) {
+        const int s = nk > 1 ? div_T((u32)j, J.invnk) : j;
+        const u32 a = area + CUBIC_SLOT_BYTES * s;
+        const double pd = *(lds_f64*)(size_t)a, pinv = *(lds_f64*)(size_t)(a + 8);
+        const u64 e = *(lds_u64*)(size_t)(a + 16);
+        const double u = *(lds_f64*)(size_t)(a + 24), u2 = *(lds_f64*)(size_t)(a + 32);
+        const double y = powe(reduce(global_ld(J.kval + (j - s * nk)), (u64)pd), e, pd, pinv);
+        u32 m3 = 0;
+        if (y == 1.0) m3 = u == 1.0 ? 7u : 1u;  // k a cube: r = 0 (mod 3), or any r
+        else if (u != 1.0 && y == u) m3 = 4u;   // c = 1: r = 2
+        else if (u != 1.0 && y == u2) m3 = 2u;  // c = 2: r = 1
+        *(__attribute__((address_space(3))) unsigned char*)(size_t)(area + 64 * CUBIC_SLOT_BYTES +
+                                                                    j) = (unsigned char)m3;
+    }
+    wave_sync();
+    return c3 ? slot : -1;
+}
+
 // Set-up of one prime (one lane): writes the record described in kernels.h.
-__device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restrict__ R) {
+// cslot >= 0: the prime's slot in the cubic work area at LDS byte address
+// carea (cubic_classes), whose class bytes narrow each k's rows.
+__device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restrict__ R,
+                                            int cslot = -1, u32 carea = 0) {
     const u64 mp = mont_mprime(p);
     const bool bzero = p < kFpMaxP ? prep_powers_fp(J, p, R) : prep_powers_int(J, p, mp, R);
     // Jacobi bits; a prime dividing b gets none, so no row is searched
@@ -654,8 +727,22 @@ __device__ __forceinline__ void prep_record(const JobDev& J, u64 p, u64* __restr
             jkv = jacobi_pre(kj.x, kj.y, ((u64)kj.w << 32) | kj.z, p);
         }
         const u64* const km = J.kmask + 4 * i;
-        if (jkv == want) lo |= const_ld(km), hi |= const_ld(km + 1);
-        if (jkv * jb == want) lo |= const_ld(km + 2), hi |= const_ld(km + 3);
+        u64 qlo = 0, qhi = 0;
+        if (jkv == want) qlo |= const_ld(km), qhi |= const_ld(km + 1);
+        if (jkv * jb == want) qlo |= const_ld(km + 2), qhi |= const_ld(km + 3);
+        if (cslot >= 0) {  // the cubic test: the rows of the residues r mod 3 it passes
+            const u32 m3 = *(__attribute__((address_space(3))) const unsigned char*)(size_t)(
+                carea + 64 * CUBIC_SLOT_BYTES + cslot * J.nk + i);
+            const u64* const k3 = J.kmask3 + 6 * i;
+            u64 clo = 0, chi = 0;
+            if (m3 & 1) clo |= const_ld(k3), chi |= const_ld(k3 + 1);
+            if (m3 & 2) clo |= const_ld(k3 + 2), chi |= const_ld(k3 + 3);
+            if (m3 & 4) clo |= const_ld(k3 + 4), chi |= const_ld(k3 + 5);
+            qlo &= clo;
+            qhi &= chi;
+        }
+        lo |= qlo;
+        hi |= qhi;
     }
     RM[cur] = lo;
     if (cur + 1 < nw) RM[cur + 1] = hi;
@@ -936,8 +1023,13 @@ __device__ __forceinline__ void sieve_body(
             break;
         }
 
-        // set-up, one lane per prime
-        if (lane < n) prep_record(J, C.low + global_ld(C.offs + q0 + lane), S + lane * J.W);
+        // set-up, one lane per prime (the cubic classes first, with the
+        // wavefront's table slot as their work area; offsets ascend, so the
+        // last prime is the largest)
+        const u64 p = lane < n ? C.low + global_ld(C.offs + q0 + lane) : 0;
+        const int cslot = J.cubic ? cubic_classes(J, p, lane < n, readlane64(p, n - 1), tab_addr)
+                                  : -1;
+        if (lane < n) prep_record(J, p, S + lane * J.W, cslot, tab_addr);
         wave_sync_global();
 
         // BSGS, the wavefront per prime; record words 0..63 of the next prime
@@ -1517,11 +1609,17 @@ extern "C" __global__ void rs_test_row_masks(const JobDev* __restrict__ jobs,
                                              const uint64_t* __restrict__ p,
                                              uint64_t* __restrict__ rec,
                                              uint64_t* __restrict__ out, long long n) {
+    // one wavefront per block; with J.cubic its cubic work area is the
+    // block's dynamic LDS (cubic_area_bytes(nk) bytes, at address 0)
+    extern __shared__ uint32_t lds[];
     const JobDev J = uniform_load(jobs);  // before any lane leaves
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const u64 pi = i < n ? p[i] : 0;
+    // primes in any order: the exponent bound of the largest below 2^50
+    const int cslot = J.cubic ? cubic_classes(J, pi, i < n, kFpMaxP - 1, lds_addr(lds)) : -1;
     if (i >= n) return;
     u64* const R = rec + i * J.W;
-    prep_record(J, p[i], R);
+    prep_record(J, pi, R, cslot, lds_addr(lds));
     const int nw = (J.nrows + 63) >> 6;
     for (int w = 0; w < nw; ++w) out[i * nw + w] = R[J.off_j + w];
 }

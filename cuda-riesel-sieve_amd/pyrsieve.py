@@ -122,6 +122,7 @@ TILE_ROW_INDEX = 16
 TILE_GROUP4 = 32
 TILE_GROUP8 = 64
 TILE_SPARSE = 128
+TILE_CUBIC = 256
 
 class RsError(RuntimeError):
     def __init__(self, what, status):

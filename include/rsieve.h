@@ -209,6 +209,19 @@ typedef struct rs_sieve rs_sieve;
  * bucket load halves and fewer primes see a bucket overflow (whose giant
  * steps then also follow overflow chains), for more LDS. */
 #define RS_TILE_SPARSE 128
+/* Cubic-residue row test (tile flag, per-wavefront layouts).  When Q is a
+ * multiple of 3 the set-up also drops, for primes p = 1 (mod 3) below
+ * 2^50, the rows (k, r) whose k b^r is not a cube mod p (k b^n = 1 needs
+ * it, and n = r mod 3): on those primes two thirds of the rows that pass
+ * the quadratic test, for one exponentiation per distinct k and prime.  It
+ * pays only where the dropped rows shorten the giant steps by whole rounds
+ * (measured within a few percent either way), so it is off unless asked
+ * for; rs_autotune tries it at its best (m, tile).  The factor set is the
+ * same either way.  Ignored where it does not apply (Q not a multiple of
+ * 3, more than 1024 distinct k, shared-table layouts, or a table too small
+ * for the set-up's work area: 2560 + 64 * (distinct k) bytes);
+ * rs_sieve_info reports the flag only where it applies. */
+#define RS_TILE_CUBIC 256
 /* Job-shape limits (rs_sieve_create, rs_batch_create, rs_autotune and the
  * server return RS_ERR_ARG outside them; rs_plan_auto_m only picks shapes
  * inside them): with X = floor(maxN/Q) - floor(minN/Q) + 1 and T the giant
@@ -288,7 +301,11 @@ void rs_sieve_destroy(rs_sieve* s);
  * and 4, each with and without RS_TILE_ROW_INDEX (plans of at most 512
  * rows).  effort 1 is a coordinate descent (tile at the model's m, then m
  * in steps of m/4 while the time improves, then the row-index layout at
- * the best pair), typically 6-9 timings.  cand_m,
+ * the best pair), typically 6-9 timings.  Both then time the shared-table
+ * layouts where the cost model's balance of baby and giant steps lies far
+ * above what one wavefront's table holds, and, at the best pair, the
+ * sparse table (RS_TILE_SPARSE) and, where it applies, the cubic-residue
+ * test (RS_TILE_CUBIC).  cand_m,
  * cand_tile, ns_per_prime (optional, capacity cap) receive the measured
  * candidates, *ncand their number. */
 rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
@@ -396,11 +413,14 @@ rs_status rs_dev_jacobi(int device, const uint32_t* a, const uint64_t* p,
                         int32_t* out, int64_t count);
 /* The sieve kernel's own row filter (the Legendre test of :491-521, as the
  * set-up phase computes it: per-k Jacobi constants, quadratic-character
- * tables, (2/p)^z, reciprocity and, for k*b^n+1, (-1/p)) for the plan's
- * rows and odd primes 3 <= p[i] < 2^63: bit j of masks[i * NW + w],
- * NW = ceil(rows / 64), is set iff row 64 w + j, (k, r), can hold a factor
- * of p[i], i.e. c k b^r is a quadratic residue mod p[i] (c = 1 for
- * k*b^n-1, -1 for k*b^n+1).  Host buffers; for tests. */
+ * tables, (2/p)^z, reciprocity and, for k*b^n+1, (-1/p); and, when Q is a
+ * multiple of 3 and the plan has at most 1024 distinct k, the cubic-residue
+ * test for p = 1 (mod 3) below 2^50) for the plan's rows and odd primes
+ * 3 <= p[i] < 2^63: bit j of masks[i * NW + w], NW = ceil(rows / 64), is
+ * set iff row 64 w + j, (k, r), can hold a factor of p[i], i.e. c k b^r is
+ * a quadratic residue mod p[i] (c = 1 for k*b^n-1, -1 for k*b^n+1) and,
+ * where the cubic test applies, a cubic residue too.  Host buffers; for
+ * tests. */
 rs_status rs_dev_row_masks(int device, const rs_plan* pl, const uint64_t* p,
                            int64_t count, uint64_t* masks);
 

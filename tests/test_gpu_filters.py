@@ -10,6 +10,10 @@ above and below the odd part, even k of odd and even valuation, k = 1, a
 table budget that runs out), bases of every 2-adic shape, p in every class
 mod 8 on both sides of 2^50, p dividing k or b, and both forms.
 
+Where Q is a multiple of 3 and p = 1 (mod 3) lies below 2^50, the masks
+also carry the cubic-residue test (c k b^n a cube mod p), checked against
+Python's pow.
+
 Distributions: the 64-job batched set and the 24-job serving set drawn from
 the heterogeneous job distribution, each job against the oracle on an
 oracle-sized window.
@@ -111,20 +115,29 @@ def sample_primes():
     return sorted(set(ps))
 
 
-def expected_masks(plan, primes):
+def cubic_applies(plan, p):
+    """The set-up's cubic-residue test: Q a multiple of 3 (so n = r mod 3),
+    at most 1024 distinct k, and p = 1 (mod 3) below 2^50."""
+    nk = len(set(plan.rows()[0].tolist()))
+    return plan.Q % 3 == 0 and nk <= 1024 and p % 3 == 1 and p < 1 << 50
+
+
+def expected_masks(plan, primes, cubic=True):
     rk, rr, _ = plan.rows()
     nw = (plan.nrows + 63) // 64
     out = np.zeros((len(primes), nw), np.uint64)
-    # a row (k, r) can hold a factor of p iff c k b^n is a square mod p for
-    # its n = Q x + r >= 1 (c = 1 for k*b^n-1, -1 for k*b^n+1); b^Q is a
-    # square, so n = r + Q decides (and p | b fails every row)
+    # a row (k, r) can hold a factor of p only if c k b^n is a square mod p
+    # for its n = Q x + r >= 1 (c = 1 for k*b^n-1, -1 for k*b^n+1); b^Q is a
+    # square, so n = r + Q decides (and p | b fails every row); where the
+    # cubic test applies, c k b^n must be a cube too (b^Q is one, 3 | Q)
     c = 1 if plan.c == -1 else -1
     for i, p in enumerate(primes):
         br = [pow(plan.base, r + plan.Q, p) for r in range(plan.Q)]
+        cub = cubic and cubic_applies(plan, p)
         words = [0] * nw
         for j, (k, r) in enumerate(zip(rk.tolist(), rr.tolist())):
             x = c * k * br[r] % p
-            if x and pow(x, (p - 1) // 2, p) == 1:
+            if x and pow(x, (p - 1) // 2, p) == 1 and (not cub or pow(x, (p - 1) // 3, p) == 1):
                 words[j // 64] |= 1 << (j % 64)
         out[i] = words
     return out
@@ -151,9 +164,16 @@ def test_row_masks_vs_euler(rs, gpu, base, c):
     want = expected_masks(plan, primes)
     bad = np.nonzero((got != want).any(axis=1))[0]
     assert len(bad) == 0, [primes[i] for i in bad[:10]]
-    # the filter passes about half of the rows of a prime not dividing k or b
-    frac = np.unpackbits(want.view(np.uint8)).sum() / (len(primes) * plan.nrows)
+    # the quadratic test passes about half of the rows of a prime not
+    # dividing k or b, and the cubic test a third of those where it applies
+    quad = expected_masks(plan, primes, cubic=False)
+    frac = np.unpackbits(quad.view(np.uint8)).sum() / (len(primes) * plan.nrows)
     assert 0.3 < frac < 0.6
+    cub = [i for i, p in enumerate(primes) if cubic_applies(plan, p) and p > 1 << 20]
+    assert len(cub) > 100
+    kept = (np.unpackbits(want[cub].view(np.uint8)).sum() /
+            np.unpackbits(quad[cub].view(np.uint8)).sum())
+    assert 0.25 < kept < 0.42
 
 
 @pytest.mark.gpu
