@@ -261,11 +261,12 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def profile_summary(kernel, workload):
+def profile_summary(kernel, tile, workload):
     """Counter figures of the committed rocprofv3 summary for this kernel
-    build and workload (profiles/<kernel>_<workload>_pmc.txt), or {}."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
-    out = {}
+    build, tile and workload (profiles/<kernel>_t<tile>_<workload>_pmc.txt),
+    or {"source": None} when none was made for that build."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_t{tile}_{workload}_pmc.txt")
+    out = {"source": None}
     if not os.path.exists(path):
         return out
     out["source"] = os.path.relpath(path, ROOT)
@@ -273,21 +274,23 @@ def profile_summary(kernel, workload):
         key, _, val = line.partition(":")
         key = key.strip()
         if key in ("traffic_bytes_per_prime", "valu_insts_per_prime", "valu_issue_busy_frac",
-                   "lds_bank_conflict_cycles_per_prime", "salu_insts_per_prime"):
+                   "lds_bank_conflict_cycles_per_prime", "salu_insts_per_prime",
+                   "lds_active_cycles_per_prime", "lds_busy_frac", "mean_launch_ms"):
             out[key] = float(val.split()[0])
-        elif key == "config":
-            out["config"] = val.strip()
     return out
 
 
 def roofline(rs, plan, sv, tm, launches, nprimes, workload="r745", probes=True):
     """The dominant kernel's roofline line.  The kernel is bound by VALU
-    instruction issue (exact 32-bit integer and FP64 arithmetic), neither
-    HBM (4 bytes in per prime) nor MFMA.  achieved = algorithmic 32-bit
+    instruction issue (exact 32-bit integer and FP64 arithmetic) together
+    with the LDS (the baby table's random bucket reads), neither HBM (4
+    bytes in per prime) nor MFMA.  achieved = algorithmic 32-bit
     multiplies per prime x primes per launch / mean launch time (HIP events
     on the sieve's own stream); peak = the published VALU lane-operation
     rate of MI355X (78.6 TFLOP/s vector FP64 = 39.3e12 full-rate lane
-    operations/s: 1024 SIMDs x 16 lanes x 2.4 GHz)."""
+    operations/s: 1024 SIMDs x 16 lanes x 2.4 GHz).  The committed counter
+    summary of the same build and tile adds how busy the VALU issue and the
+    LDS are, as shares of the CU cycles per prime."""
     # multiplies per prime of the BSGS phase: m baby steps and Q*T Y-table
     # products at 11 (full 64x64 Montgomery), A*T giant steps at 5, A =
     # rows/2 expected Jacobi survivors (the set-up phase is not counted)
@@ -297,7 +300,7 @@ def roofline(rs, plan, sv, tm, launches, nprimes, workload="r745", probes=True):
     launch_s = tm["kernel_ms"] / 1e3 / launches
     achieved = muls * per_launch / launch_s
     published = 78.6e12 / 2
-    prof = profile_summary(sv.kernel, workload)
+    prof = profile_summary(sv.kernel, sv.tile, workload)
     traffic = prof.get("traffic_bytes_per_prime")
     out = {"bound": "valu", "achieved": achieved / 1e9, "peak": published / 1e9,
            "unit": "Gmul/s", "frac": achieved / published,
@@ -307,7 +310,9 @@ def roofline(rs, plan, sv, tm, launches, nprimes, workload="r745", probes=True):
            "muls_per_prime": muls, "primes_per_launch": per_launch,
            "mean_launch_ms": launch_s * 1e3,
            "published_valu_peak": published / 1e9,
-           "hbm_frac": (traffic * per_launch / launch_s / 8e12) if traffic else None}
+           "hbm_frac": (traffic * per_launch / launch_s / 8e12) if traffic else None,
+           "valu_issue_busy_frac": prof.get("valu_issue_busy_frac"),
+           "lds_busy_frac": prof.get("lds_busy_frac")}
     if probes:
         mad = rs.dev_mad_rate()
         copy_bytes = 1 << 30

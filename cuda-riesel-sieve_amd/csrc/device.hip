@@ -476,13 +476,20 @@ struct Tuner {
     double* ns_out;
     double best = 1e300;
     int best_m = 0, best_tile = 0;
+    struct Cand {
+        int m, tile;
+        double ns;  // mean over its timings
+        int reps;
+    };
+    std::vector<Cand> all;
 
-    // Mean kernel ns per prime of (m, tile) on the sample; RS_ERR_ARG when
-    // the layout does not fit one wavefront's LDS limit (nor will a larger
-    // m), kSkip when the shape is outside the kernel's index limits (m too
-    // small for the x-window: a larger m may fit).
+    // Kernel ns per prime of one launch of (m, tile) on the sample, or the
+    // status: RS_ERR_ARG when the layout does not fit one wavefront's LDS
+    // limit (nor will a larger m), kSkip when the shape is outside the
+    // kernel's index limits (m too small for the x-window: a larger m may
+    // fit).
     static const rs_status kSkip = 1;
-    rs_status time(int m, int tile, double* ns) {
+    rs_status measure(int m, int tile, double* ns) {
         if (!rsh::shape_ok(pl, m, std::max(1, tile & 7))) return kSkip;
         rs_sieve* s = nullptr;
         rs_status st = rs_sieve_create(device, pl, m, tile, 1, 1 << 16, &s);
@@ -503,11 +510,14 @@ struct Tuner {
         rs_sieve_destroy(s);
         if (st != RS_OK) return st;
         *ns = ms_time * 1e6 / (double)count;
-        if (n < cap) {
-            if (cand_m) cand_m[n] = m;
-            if (cand_tile) cand_tile[n] = tile;
-            if (ns_out) ns_out[n] = *ns;
-        }
+        return RS_OK;
+    }
+
+    // Times (m, tile) once and records it as a candidate.
+    rs_status time(int m, int tile, double* ns) {
+        rs_status st = measure(m, tile, ns);
+        if (st != RS_OK) return st;
+        all.push_back({m, tile, *ns, 1});
         ++n;
         if (*ns < best) {
             best = *ns;
@@ -515,6 +525,38 @@ struct Tuner {
             best_tile = tile;
         }
         return RS_OK;
+    }
+
+    // Times the `top` fastest candidates `extra` more times each and ranks
+    // them by their mean: one timing on the sample varies by a few percent
+    // from launch to launch, as much as the leading candidates differ.
+    rs_status refine(int top, int extra) {
+        std::vector<size_t> idx(all.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return all[a].ns < all[b].ns; });
+        for (size_t r = 0; r < idx.size() && (int)r < top; ++r) {
+            Cand& c = all[idx[r]];
+            for (int e = 0; e < extra; ++e) {
+                double ns = 0;
+                rs_status st = measure(c.m, c.tile, &ns);
+                if (st != RS_OK) return st;
+                c.ns = (c.ns * c.reps + ns) / (c.reps + 1);
+                ++c.reps;
+            }
+        }
+        best = 1e300;
+        for (const Cand& c : all)
+            if (c.ns < best) best = c.ns, best_m = c.m, best_tile = c.tile;
+        return RS_OK;
+    }
+
+    // The candidates (mean ns per prime) into the caller's arrays.
+    void report() const {
+        for (int i = 0; i < (int)all.size() && i < cap; ++i) {
+            if (cand_m) cand_m[i] = all[i].m;
+            if (cand_tile) cand_tile[i] = all[i].tile;
+            if (ns_out) ns_out[i] = all[i].ns;
+        }
     }
 };
 
@@ -674,6 +716,9 @@ extern "C" rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
                 return st;
         }
     }
+    rs_status st = tu.refine(4, 2);
+    if (st != RS_OK && st != RS_ERR_NOMEM) return st;
+    tu.report();
     if (ncand) *ncand = tu.n;
     *best_m = tu.best_m;
     *best_tile = tu.best_tile;

@@ -305,9 +305,10 @@ void rs_sieve_destroy(rs_sieve* s);
  * layouts where the cost model's balance of baby and giant steps lies far
  * above what one wavefront's table holds, and, at the best pair, the
  * sparse table (RS_TILE_SPARSE) and, where it applies, the cubic-residue
- * test (RS_TILE_CUBIC).  cand_m,
+ * test (RS_TILE_CUBIC).  Finally the four fastest are timed twice more
+ * and ranked by the mean of their three timings.  cand_m,
  * cand_tile, ns_per_prime (optional, capacity cap) receive the measured
- * candidates, *ncand their number. */
+ * candidates (each pair once, with its mean time), *ncand their number. */
 rs_status rs_autotune(int device, const rs_plan* pl, uint64_t low,
                       const uint32_t* offsets_dev, int64_t count, int effort, int* best_m,
                       int* best_tile, int* cand_m, int* cand_tile,

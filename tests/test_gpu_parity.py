@@ -564,6 +564,37 @@ def test_autotune_grid_and_descent(rs, orc, gpu):
     buf.free()
 
 
+def shape_ok(plan, m, tw):
+    """The job-shape limits of rsieve.h (rs_sieve_create refuses the rest)."""
+    X = plan.maxN // plan.Q - plan.minN // plan.Q + 1
+    T = max(2, -(-(-(-X // m)) // tw) * tw)
+    ys = T + (1 if T % 2 == 0 else 0)
+    return (m % 64 == 0 and 64 <= m <= 1 << 16 and T <= 65535 and plan.Q * ys < 65536
+            and min(plan.nrows, 512) * T + 4096 < 65536)
+
+
+def test_autotune_wide_plan_below_the_shape_limit(rs, gpu):
+    """A plan over 512 rows whose grid starts below the job-shape limit
+    (base 2, 500 k, n in [1e6, 3e6]: at half the cost model's m, T breaks
+    min(rows, 512) T + 4096 < 65536): both efforts skip the shapes outside
+    the limits, time the rest and return a pair that builds."""
+    plan = rs.Plan(rs.Abcd.synth(2, 500, 3, 10**6, 10**6, 3 * 10**6, 1000, seed=3,
+                                 odd_k=True))
+    assert plan.nrows > 512
+    m0 = plan.auto_m()
+    assert not shape_ok(plan, max(64, m0 // 2 // 64 * 64), 1)
+    low = 10**9 + 7
+    g = rs.GpuPrimeGen(low + (1 << 16) + 1)
+    buf = g.device_range(low, low + (1 << 16))
+    for effort in (0, 1):
+        m, tile, cands = rs.autotune(plan, low, buf, effort=effort)
+        assert m >= 64 and (m, tile) in [(cm, ct) for cm, ct, _ in cands], effort
+        assert all(shape_ok(plan, cm, max(1, ct & 7)) for cm, ct, _ in cands)
+        sv = rs.Sieve(plan, m=m, tile=tile)
+        sv.close()
+    buf.free()
+
+
 def test_hit_buffer_overflow_is_reported(rs, gpu):
     plan = rs.Plan(rs.Abcd.synth(5, 10, 3, 10**5, 2000, 60000, 200, seed=4, odd_k=False))
     low = 10**6 + 1
