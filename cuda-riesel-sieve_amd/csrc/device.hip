@@ -60,6 +60,8 @@ struct rs_sieve {
     bool consumed_valid[2] = {false, false};
     int buf = 0;
     std::vector<Timing> timings;
+    std::vector<Timing> spare;  // timing events read and free for reuse (creating
+                                // and destroying a pair per launch costs host time)
     double acc_ms = 0;
     long long acc_primes = 0;
     std::vector<HitDev> held;
@@ -87,10 +89,11 @@ static void free_sieve(rs_sieve* s) {
         if (s->copied[b]) (void)hipEventDestroy(s->copied[b]);
         if (s->consumed[b]) (void)hipEventDestroy(s->consumed[b]);
     }
-    for (auto& t : s->timings) {
-        (void)hipEventDestroy(t.a);
-        (void)hipEventDestroy(t.b);
-    }
+    for (auto* v : {&s->timings, &s->spare})
+        for (auto& t : *v) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     delete s;
@@ -160,12 +163,17 @@ extern "C" const char* rs_sieve_kernel_name(const rs_sieve* s) {
 static rs_status timed_launch(rs_sieve* s, const ChunkDev& c0, long long ngroups,
                               int64_t primes) {
     Timing t{};
-    t.primes = primes;
-    if (hipEventCreate(&t.a) != hipSuccess) return RS_ERR_HIP;
-    if (hipEventCreate(&t.b) != hipSuccess) {
-        (void)hipEventDestroy(t.a);
-        return RS_ERR_HIP;
+    if (!s->spare.empty()) {
+        t = s->spare.back();
+        s->spare.pop_back();
+    } else {
+        if (hipEventCreate(&t.a) != hipSuccess) return RS_ERR_HIP;
+        if (hipEventCreate(&t.b) != hipSuccess) {
+            (void)hipEventDestroy(t.a);
+            return RS_ERR_HIP;
+        }
     }
+    t.primes = primes;
     rs_status st = hipEventRecord(t.a, s->stream) == hipSuccess ? RS_OK : RS_ERR_HIP;
     if (st == RS_OK) st = launch(s->L, s->stream, s->job.d_self, c0, nullptr, 1, ngroups);
     if (st == RS_OK && hipEventRecord(t.b, s->stream) != hipSuccess) st = RS_ERR_HIP;
@@ -365,8 +373,7 @@ static rs_status sieve_drain(rs_sieve* s) {
         HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
         s->acc_ms += ms;
         s->acc_primes += t.primes;
-        (void)hipEventDestroy(t.a);
-        (void)hipEventDestroy(t.b);
+        s->spare.push_back(t);
     }
     s->timings.clear();
     return RS_OK;
