@@ -73,7 +73,7 @@ struct rs_primegen {
 
 namespace {
 
-const uint64_t kSegBytes = 1u << 18;   // 256 KiB: 7.86M numbers per segment
+const uint64_t kSegBytes = 1u << 20;   // 1 MiB: 31.5M numbers per segment
 const uint64_t kBlockSegs = 16;        // segments per thread block
 const uint64_t kSliceBytes = 1u << 15;  // 32 KiB slices for the small primes
 const uint32_t kSmallQ = 1u << 15;
@@ -155,6 +155,10 @@ void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbytes_total, uin
         wheel[i] = (uint8_t)kW.idx[kr];
     }
     const size_t ns = std::min(np, g->nsmall);
+    // room for the block's primes up front (about 30 nbytes / ln(base)),
+    // so the output does not move while it grows
+    const double lnb = std::log((double)std::max<uint64_t>(base + 30 * nbytes_total, 16));
+    out.reserve(out.size() + (size_t)(30.0 * (double)nbytes_total / (lnb - 1.2) * 1.05) + 4096);
     for (uint64_t done = 0; done < nbytes_total; done += kSegBytes) {
         const uint64_t nb = std::min(kSegBytes, nbytes_total - done);
         const uint64_t b0 = base + 30 * done;
@@ -198,9 +202,17 @@ void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbytes_total, uin
         }
         // survivors, in order: the words of 8 bytes (240 numbers) at both
         // ends of the window check the bounds, the test above bound2 only
-        // where the segment reaches it
+        // where the segment reaches it.  The output grows by the survivor
+        // count (a popcount pass), not by the segment's 8 nb candidates: the
+        // vector zero-fills what it grows by
+        size_t nfree = 0;
+        for (uint64_t w = 0; w < nb; w += 8) {
+            uint64_t word;
+            memcpy(&word, seg.data() + w, 8);
+            nfree += (size_t)__builtin_popcountll(~word);
+        }
         const size_t at = out.size();
-        out.resize(at + (size_t)(nb * 8) + 8);
+        out.resize(at + nfree);
         uint32_t* o = out.data() + at;
         const bool mr = b0 + 30 * nb > g->bound2;
         const uint64_t off = b0 - origin;

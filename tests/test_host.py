@@ -138,12 +138,13 @@ def test_primes_range_matches_oracle(rs, orc):
 
 @pytest.mark.parametrize("threads", [1, 4])
 def test_primes_range_across_segments_and_blocks(rs, orc, threads):
-    """A window of 32 segments of 2^21 odd numbers (blocks of 16 or 8 of
-    them, each prime's next multiple carried from segment to segment), with
-    sieving primes to 1.01e7 and an odd, ragged end, equals the oracle; so
-    do pieces of it cut at arbitrary points."""
+    """A window of 8.5 segments of 1 MiB wheel bytes (31.5M numbers each;
+    one block on one thread, four blocks of about two segments on four, each
+    prime's next multiple carried from segment to segment), with sieving
+    primes to 1.01e7 and an odd, ragged end, equals the oracle; so do
+    pieces of it cut at arbitrary points."""
     lo = 102297149770000 + 1
-    hi = lo + (1 << 27) - 77
+    hi = lo + (1 << 28) - 77
     g = rs.PrimeGen(hi, threads=threads)
     want = orc.true_primes(lo, hi)
     got = g.range(lo, hi)
