@@ -280,10 +280,13 @@ extern "C" rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* 
                                      int64_t count) {
     if (!s || count < 0 || (count > 0 && !offsets)) return RS_ERR_ARG;
     if (count == 0) return RS_OK;
-    uint32_t mx = 0;
-    for (int64_t i = 0; i < count; ++i) mx = std::max(mx, offsets[i]);
-    const uint64_t hi = low + mx;
-    if (low < 3 || hi < low || hi >= (1ull << 63)) return RS_ERR_ARG;
+    if (low < 3) return RS_ERR_ARG;
+    // the largest offset is read only when low + 2^32 - 1 could reach 2^63
+    if (low >= (1ull << 63) - 0xffffffffull) {
+        uint32_t mx = 0;
+        for (int64_t i = 0; i < count; ++i) mx = std::max(mx, offsets[i]);
+        if (low + mx < low || low + mx >= (1ull << 63)) return RS_ERR_ARG;
+    }
     HIPCHK(hipSetDevice(s->device));
     for (int64_t done = 0; done < count; done += s->max_batch) {
         const int64_t n = std::min<int64_t>(s->max_batch, count - done);

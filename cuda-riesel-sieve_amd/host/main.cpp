@@ -6,8 +6,9 @@
 // picks m, where the reference sizes its tables from the free memory,
 // :917-929; M = 2..4 runs m = 64, this build's smallest table), -p P0,
 // -P P1, -Q Q (even), -O FILE (factor file, default factors.txt), -b/-t
-// SCALE (primes per launch = 128*2^(b-1) * 128*2^(t-1), :817-818, :849-851),
-// -s SCALE (at least 4096*s sieving primes, :1274), -v (no verification:
+// SCALE (primes per batch, the reference's launch, = 128*2^(b-1) *
+// 128*2^(t-1), :817-818, :849-851; a device launch here holds up to 2^24
+// primes of whole batches), -s SCALE (at least 4096*s sieving primes, :1274), -v (no verification:
 // datless output), -g (generate the primes on the GPU; the default is the
 // CPU generator of the reference's design), -R FILE (write the input ABCD
 // with every verified factor's candidate removed; the reference only
@@ -17,7 +18,7 @@
 // clipped to [P0, P1] (the reference runs past P1 with its last batch).
 // Output lines keep the reference's formats: factors "p | k*b^n-1"
 // (:1833; "+1" for a Sierpinski file) and a progress line every 20
-// launches (:1759).
+// batches (:1759).
 #include <getopt.h>
 #include <sched.h>
 
@@ -31,7 +32,7 @@
 #include <condition_variable>
 #include <ctime>
 #include <deque>
-#include <future>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -48,8 +49,8 @@ static void usage() {
         "                If no -p and -P arguments then we default to 102297149770000 for testing.\n"
         "-Q Q     : Override subsequence value Q (even). Sieve k*b^n-1 as (k*b^d)*(b^Q)^m-1.\n"
         "-O FILE  : Write factors to a file called FILE.\n"
-        "-b SCALE : Scale the primes per launch by 2^(SCALE-1) (blocks).\n"
-        "-t SCALE : Scale the primes per launch by 2^(SCALE-1) (threads).\n"
+        "-b SCALE : Scale the primes per batch by 2^(SCALE-1) (blocks).\n"
+        "-t SCALE : Scale the primes per batch by 2^(SCALE-1) (threads).\n"
         "-s SCALE : Use at least 4096*SCALE small primes for the prime generator. Default 256.\n"
         "-v       : Don't verify factors. This implies running datless.\n"
         "-g       : Generate the primes on the GPU instead of the CPU generator.\n"
@@ -58,26 +59,25 @@ static void usage() {
         "-h       : Prints this help\n");
 }
 
-// One launch's primes: p = low + offs[i], i < n (n = -1: the p-range
-// [low, next) is generated on the device); next is the following batch's
-// low; done marks the end of the range.
-struct Batch {
-    uint64_t low = 0, next = 0;
-    long long n = 0;
-    std::vector<uint32_t> offs;
-    bool done = true;
-};
-
-// CPU-generated batches from a producer thread: windows of about 8 batches
-// (at least 2^31 wide) are sieved at once by rs_primes_range (all host
-// threads), one window ahead of the one being cut into batches of `size`
-// primes as the reference cuts its prime arrays (:1561), at most 4 batches
-// ahead of the sieve.
-class BatchFeed {
+// Windows of true primes [lo, hi) (p = lo + offs[i], i < n) sieved ahead of
+// the device by a producer thread with the generator's host threads.
+// Windows are 2^30 wide: on 15 threads at 1e14 the generator sieves one in
+// about 30 ms, the device about 40 (so after the first window the device
+// never waits), and every thread gets a block of its own.  Three buffers
+// are reused in turn (no allocation or page faults per window): one being
+// submitted, one ready, one being sieved.
+class WindowFeed {
    public:
-    BatchFeed(rs_primegen* g, uint64_t low, uint64_t high, long long size)
-        : g_(g), low_(low), high_(high), size_(size) {}
-    ~BatchFeed() {
+    struct Window {
+        uint64_t lo = 0, hi = 0;
+        const uint32_t* offs = nullptr;
+        size_t n = 0;
+        int buf = -1;  // -1: no window (the range is done)
+    };
+    WindowFeed(rs_primegen* g, uint64_t low, uint64_t high) : g_(g), low_(low), high_(high) {
+        for (int b = 0; b < kBufs; ++b) free_.push_back(b);
+    }
+    ~WindowFeed() {
         {
             std::lock_guard<std::mutex> l(mu_);
             stop_ = true;
@@ -86,126 +86,76 @@ class BatchFeed {
         if (th_.joinable()) th_.join();
     }
     void start() { th_ = std::thread([this] { run(); }); }
-    // The next batch (done = true after the last), or the generator's error.
-    rs_status pop(Batch* b) {
+    // The next window (buf = -1 after the last), or the generator's error.
+    rs_status pop(Window* w) {
         std::unique_lock<std::mutex> l(mu_);
-        cv_.wait(l, [&] { return !q_.empty() || finished_; });
-        if (!q_.empty()) {
-            *b = std::move(q_.front());
-            q_.pop_front();
-            cv_.notify_all();
+        cv_.wait(l, [&] { return !ready_.empty() || finished_; });
+        if (!ready_.empty()) {
+            *w = ready_.front();
+            ready_.pop_front();
             return RS_OK;
         }
-        b->done = true;
+        *w = Window{};
         return st_;
+    }
+    // The window's offsets have been copied out: its buffer is free again.
+    void release(const Window& w) {
+        if (w.buf < 0) return;
+        std::lock_guard<std::mutex> l(mu_);
+        free_.push_back(w.buf);
+        cv_.notify_all();
     }
 
    private:
-    void push(Batch&& b) {
-        std::unique_lock<std::mutex> l(mu_);
-        cv_.wait(l, [&] { return q_.size() < 4 || stop_; });
-        q_.push_back(std::move(b));
-        cv_.notify_all();
-    }
-    void finish(rs_status st) {
+    static const int kBufs = 3;
+    void run() {
+        const uint64_t width = 1ull << 30;
+        for (uint64_t lo = low_; lo <= high_;) {
+            int b;
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                cv_.wait(l, [&] { return !free_.empty() || stop_; });
+                if (stop_) return;
+                b = free_.front();
+                free_.pop_front();
+            }
+            const uint64_t hi = std::min<uint64_t>(lo + width, high_ + 1);
+            const double lnp = std::log((double)std::max<uint64_t>(lo, 16));
+            size_t need = (size_t)((double)(hi - lo) / std::max(1.0, lnp - 1.5) * 1.2) + 4096;
+            int64_t got = 0;
+            rs_status st = RS_OK;
+            for (int pass = 0; pass < 2; ++pass) {
+                if (cap_[b] < need) {
+                    buf_[b].reset(new uint32_t[need]);  // left uninitialised
+                    cap_[b] = need;
+                }
+                st = rs_primes_range(g_, lo, hi, buf_[b].get(), (int64_t)cap_[b], &got);
+                if (st != RS_ERR_OVERFLOW) break;
+                need = (size_t)got;
+            }
+            std::lock_guard<std::mutex> l(mu_);
+            if (st != RS_OK) {
+                st_ = st;
+                break;
+            }
+            ready_.push_back(Window{lo, hi, buf_[b].get(), (size_t)got, b});
+            cv_.notify_all();
+            lo = hi;
+        }
         std::lock_guard<std::mutex> l(mu_);
-        st_ = st;
         finished_ = true;
         cv_.notify_all();
-    }
-    // Primes of the window [lo, min(lo + width, P1 + 1)) as offsets from lo.
-    struct Window {
-        uint64_t lo = 0, hi = 0;
-        std::vector<uint32_t> offs;
-        rs_status st = RS_OK;
-    };
-    Window sieve_window(uint64_t wlo) {
-        Window w;
-        w.lo = wlo;
-        const double lnp = std::log((double)std::max<uint64_t>(wlo, 16));
-        // at least 2^31 wide, so that each generator thread sieves many
-        // segments per block
-        uint64_t width = (uint64_t)(8.0 * size_ * lnp);
-        width = std::min<uint64_t>(std::max<uint64_t>(width, 1ull << 31), 1ull << 32);
-        w.hi = std::min<uint64_t>(wlo + width, high_ + 1);
-        w.offs.resize((size_t)((w.hi - wlo) / std::max(1.0, lnp - 1.5) * 1.2) + 4096);
-        int64_t got = 0;
-        w.st = rs_primes_range(g_, wlo, w.hi, w.offs.data(), (int64_t)w.offs.size(), &got);
-        if (w.st == RS_ERR_OVERFLOW) {
-            w.offs.resize((size_t)got);
-            w.st = rs_primes_range(g_, wlo, w.hi, w.offs.data(), got, &got);
-        }
-        w.offs.resize(w.st == RS_OK ? (size_t)got : 0);
-        return w;
-    }
-    void run() {
-        uint64_t cur = low_;   // the next batch's low
-        uint64_t wbase = low_; // the current window's low: its primes are wbase + win[i]
-        std::vector<uint32_t> win;
-        size_t head = 0, cnt = 0;
-        // the next window is sieved while the current one is cut and sent
-        std::future<Window> ahead;
-        if (low_ <= high_)
-            ahead = std::async(std::launch::async, [this] { return sieve_window(low_); });
-        while (cur <= high_) {
-            {
-                std::lock_guard<std::mutex> l(mu_);
-                if (stop_) break;
-            }
-            Batch b;
-            b.low = cur;
-            b.done = false;
-            b.offs.resize((size_t)size_);
-            size_t n = 0;
-            while (n < (size_t)size_) {
-                if (head == cnt) {  // take the next window
-                    if (!ahead.valid()) break;
-                    Window w = ahead.get();
-                    if (w.st != RS_OK) return finish(w.st);
-                    if (w.hi <= high_) {
-                        const uint64_t nlo = w.hi;
-                        ahead = std::async(std::launch::async, [this, nlo] { return sieve_window(nlo); });
-                    }
-                    wbase = w.lo;
-                    win = std::move(w.offs);
-                    head = 0;
-                    cnt = win.size();
-                    continue;
-                }
-                // the window's next primes, while their offsets from the
-                // batch's low fit 32 bits (wbase + win[i] - cur < 2^32)
-                size_t take = std::min((size_t)size_ - n, cnt - head);
-                const uint64_t room = (1ull << 32) + cur - wbase;  // win[i] < room
-                if (room <= 0xffffffffull)
-                    take = std::min<size_t>(
-                        take, std::lower_bound(win.begin() + head, win.begin() + head + take,
-                                               (uint32_t)room) - (win.begin() + head));
-                if (take == 0) break;
-                const uint32_t adj = (uint32_t)(wbase - cur);  // mod 2^32
-                const uint32_t* src = win.data() + head;
-                uint32_t* dst = b.offs.data() + n;
-                for (size_t i = 0; i < take; ++i) dst[i] = src[i] + adj;
-                head += take;
-                n += take;
-                if (head < cnt && n < (size_t)size_) break;  // stopped by the 32-bit bound
-            }
-            if (n == 0) break;  // no prime left in [cur, P1]
-            b.offs.resize(n);
-            b.n = (long long)n;
-            b.next = cur + b.offs.back() + 2;
-            cur = b.next;
-            push(std::move(b));
-        }
-        finish(RS_OK);
     }
 
     rs_primegen* g_;
     uint64_t low_, high_;
-    long long size_;
+    std::unique_ptr<uint32_t[]> buf_[kBufs];
+    size_t cap_[kBufs] = {};
     std::thread th_;
     std::mutex mu_;
     std::condition_variable cv_;
-    std::deque<Batch> q_;
+    std::deque<int> free_;
+    std::deque<Window> ready_;
     bool stop_ = false, finished_ = false;
     rs_status st_ = RS_OK;
 };
@@ -280,7 +230,7 @@ int main(int argc, char* argv[]) {
     }
     const long long arraySize = 128 * bscale * 128 * tscale;
     if (arraySize > (1ll << 30)) {
-        printf("Bad input parameter : primes per launch must stay below 2^30.\n");
+        printf("Bad input parameter : primes per batch must stay below 2^30.\n");
         return 1;
     }
 
@@ -319,60 +269,40 @@ int main(int argc, char* argv[]) {
         return 1;
     }
 
+    // Device launches hold whole reference batches, several at once when a
+    // batch is small (at -b 5 a batch of 262144 primes is one 64-prime group
+    // per resident wavefront, and a launch's fixed cost is then a third of
+    // its time), up to kLaunchPrimes primes; the factors found do not depend
+    // on how batches are grouped into launches.
+    const long long kLaunchPrimes = std::max<long long>(arraySize, 1ll << 24);
+    const int64_t kHitCap = 1 << 22;
     rs_sieve* sv = nullptr;
-    if ((st = rs_sieve_create(device, plan, m, 0, arraySize, 1 << 20, &sv)) != RS_OK)
+    if ((st = rs_sieve_create(device, plan, m, 0, kLaunchPrimes, kHitCap, &sv)) != RS_OK)
         return fail("device set-up", st);
     int mUsed, T, slots, ldsw, tile;
     rs_sieve_info(sv, &mUsed, &T, &slots, &ldsw, &tile);
     printf("Each prime uses %d baby steps, %d giant steps per row, a %d-slot LDS table "
            "(%d bytes of LDS per wavefront)\n\n", mUsed, T, slots, ldsw);
-    // the generator's threads leave two of the process's CPUs to the batch
-    // cutter and this submitting thread
+    // the generator's threads leave one of the process's CPUs to this
+    // submitting thread
     int cpus = (int)std::thread::hardware_concurrency();
     cpu_set_t cset;
     if (sched_getaffinity(0, sizeof cset, &cset) == 0) cpus = CPU_COUNT(&cset);
     if (const char* e = getenv("OMP_NUM_THREADS"))
         if (atoi(e) > 0) cpus = std::min(cpus, atoi(e));
     rs_primegen* gen = nullptr;
-    if ((st = rs_primegen_create(smallPrimeScale, high + 1, std::max(1, cpus - 2), &gen)) !=
+    if ((st = rs_primegen_create(smallPrimeScale, high + 1, std::max(1, cpus - 1), &gen)) !=
         RS_OK)
         return fail("prime generator set-up", st);
 
     const unsigned long long startLow = low;
     std::vector<rs_factor> hits(1 << 20);
     std::vector<rs_factor> verified;  // candidates to remove with -R
-    // Batches as in the reference (:1537-1561): the first arraySize primes >=
-    // low (low odd), offsets from low, the next low = last prime + 2; the
-    // last batch is clipped to P1.  With the CPU generator they come from a
-    // producer thread that sieves windows of about 8 batches at once with
-    // all host threads (one window per call keeps every thread busy) and
-    // cuts them into batches, so generation overlaps the sieve; with -g each
-    // launch is a p-range sieved on the GPU (at least 2^30 wide: the device
-    // generator needs wide ranges).
-    BatchFeed feed(gen, low, high, arraySize);
-    if (!gpuPrimes) feed.start();
-    uint64_t gLow = low;
-    auto next_batch = [&](Batch* b) -> rs_status {
-        if (!gpuPrimes) return feed.pop(b);
-        if (gLow > high) {
-            b->done = true;
-            return RS_OK;
-        }
-        const double lnp = std::log((double)std::max<uint64_t>(gLow, 16));
-        uint64_t width = std::max<uint64_t>((uint64_t)(arraySize * lnp), 1ull << 30);
-        width = std::min<uint64_t>(width, 1ull << 32);
-        b->low = gLow;
-        b->next = std::min<uint64_t>(gLow + width, high + 1);
-        b->n = -1;
-        b->done = false;
-        gLow = b->next;
-        return RS_OK;
-    };
     // collects and writes the hits of everything submitted so far (the
-    // batches are consecutive in p, so the sorted hits come out in the order
-    // a collect per batch would give)
+    // launches are consecutive in p, so the sorted hits come out in the
+    // order a collect per launch would give); *raw = hits collected
     long long factorCount = 0;
-    auto drain = [&]() -> rs_status {
+    auto drain = [&](int64_t* raw) -> rs_status {
         int64_t got = 0, lost = 0;
         rs_status s2 = rs_sieve_collect(sv, hits.data(), (int64_t)hits.size(), &got, &lost);
         if (s2 == RS_ERR_OVERFLOW) {
@@ -380,6 +310,7 @@ int main(int argc, char* argv[]) {
             s2 = rs_sieve_collect(sv, hits.data(), (int64_t)hits.size(), &got, &lost);
         }
         if (s2 != RS_OK) return s2;
+        *raw = got;
         if (lost > 0) fprintf(stderr, "Warning: %" PRId64 " hits dropped (append buffer full)\n", lost);
         FILE* ff = nullptr;
         for (int64_t i = 0; i < got; ++i) {
@@ -408,51 +339,104 @@ int main(int argc, char* argv[]) {
         if (ff) fclose(ff);
         return RS_OK;
     };
-    // hits are collected every kCollect launches (the device keeps them in
-    // its append buffer meanwhile), so the host never waits for a launch it
-    // has just queued
-    const int kCollect = 8;
-    int kernelCount = 0;
+    // Hits wait in the device's append buffer between collects, and a
+    // collect waits for the device, so collects are spaced by the hits they
+    // bring: the first after one batch, then every drainEvery primes,
+    // doubled (up to 2^28) while a collect brings fewer than kHitCap/16 hits
+    // and halved (down to one batch) when it brings more than kHitCap/4.
+    long long drainEvery = arraySize, sinceDrain = 0;
+    // batches in the next launch: one after each collect, doubling after
+    // every launch, so the copy of each launch overlaps the sieve of the
+    // one before (a full-size launch right after a collect would leave the
+    // device idle while the host copies its offsets)
+    long long ramp = 1;
+    auto paced_drain = [&](long long primes, bool last) -> rs_status {
+        sinceDrain += primes;
+        if (!last && sinceDrain < drainEvery) return RS_OK;
+        int64_t raw = 0;
+        rs_status s2 = drain(&raw);
+        ramp = 1;
+        if (raw < kHitCap / 16)
+            drainEvery = std::min(drainEvery * 2, 1ll << 28);
+        else if (raw > kHitCap / 4)
+            drainEvery = std::max(drainEvery / 2, arraySize);
+        sinceDrain = 0;
+        return s2;
+    };
+    // the progress line of :1759, every 20 batches; the rate is over the
+    // p-range since the previous line
+    long long kernelCount = 0;  // batches (the reference's launches) submitted
     auto t0 = std::chrono::steady_clock::now();
-    auto tb = t0;
-    Batch cur;
-    if ((st = next_batch(&cur)) != RS_OK) return fail("prime generation", st);
-    while (!cur.done) {
-        ++kernelCount;
-        unsigned long long progress = 0;
-        if (cur.n < 0) {  // GPU-generated primes of [low, next)
-            if ((st = rs_sieve_submit_range(sv, cur.low, cur.next)) != RS_OK)
-                return fail("submit", st);
-            progress = cur.next - cur.low;
-        } else {
-            if (cur.n > 0 &&
-                (st = rs_sieve_submit(sv, cur.low, cur.offs.data(), cur.n)) != RS_OK)
-                return fail("submit", st);
-            progress = cur.n > 0 ? cur.offs[cur.n - 1] - cur.offs[0] : 0;
-        }
-        const uint64_t nLow = cur.next;
-        Batch nxt;
-        if ((st = next_batch(&nxt)) != RS_OK) return fail("prime generation", st);
-        if (nxt.done || kernelCount % kCollect == 0)
-            if ((st = drain()) != RS_OK) return fail("collect", st);
+    auto tLine = t0;
+    uint64_t pLine = startLow;
+    auto progress_line = [&](uint64_t nLow) {
         auto te = std::chrono::steady_clock::now();
-        const double dt = std::chrono::duration<double>(te - tb).count();
-        tb = te;
-        if (kernelCount % 20 == 0) {
-            const double done = 100.0 * (double)(nLow - startLow) / (double)(high - startLow + 1);
-            const double soFar = std::chrono::duration<double>(te - t0).count();
-            const double left = done > 0 ? soFar * (100.0 / done - 1.0) : 0.0;
-            time_t eta = time(nullptr) + (time_t)left;
-            char etaBuf[64];
-            strftime(etaBuf, sizeof etaBuf, "%a %b %d %H:%M:%S %Y", localtime(&eta));
-            printf("p=%llu, %.0f p/sec,  %.2f%% done, %lld factors found, ETA %s \n",
-                   (unsigned long long)nLow,
-                   progress / dt, done > 100 ? 100.0 : done, factorCount, etaBuf);
-            fflush(stdout);
+        const double dt = std::chrono::duration<double>(te - tLine).count();
+        const double done = 100.0 * (double)(nLow - startLow) / (double)(high - startLow + 1);
+        const double soFar = std::chrono::duration<double>(te - t0).count();
+        const double left = done > 0 ? soFar * (100.0 / done - 1.0) : 0.0;
+        time_t eta = time(nullptr) + (time_t)left;
+        char etaBuf[64];
+        strftime(etaBuf, sizeof etaBuf, "%a %b %d %H:%M:%S %Y", localtime(&eta));
+        printf("p=%llu, %.0f p/sec,  %.2f%% done, %lld factors found, ETA %s \n",
+               (unsigned long long)nLow, dt > 0 ? (double)(nLow - pLine) / dt : 0.0,
+               done > 100 ? 100.0 : done, factorCount, etaBuf);
+        fflush(stdout);
+        tLine = te;
+        pLine = nLow;
+    };
+    // Batches as in the reference (:1537-1561): the first arraySize primes >=
+    // low (low odd), the next batch from the last prime + 2, the last one
+    // clipped to P1.  With the CPU generator the primes come in windows from
+    // a producer thread, so generation overlaps the sieve, and each launch
+    // is a slice of a window (offsets from the window's low: no copy beyond
+    // the one into the pinned staging buffer) ending on a batch boundary or
+    // at the window's end.  With -g each launch is a p-range sieved on the
+    // GPU (at least 2^30 wide: the device generator needs wide ranges).
+    if (!gpuPrimes) {
+        WindowFeed feed(gen, low, high);
+        feed.start();
+        long long inBatch = 0;  // primes of the open batch submitted so far
+        for (;;) {
+            WindowFeed::Window w;
+            if ((st = feed.pop(&w)) != RS_OK) return fail("prime generation", st);
+            if (w.buf < 0) break;
+            for (size_t i = 0; i < w.n;) {
+                const long long k = std::min(
+                    ramp, std::max<long long>(1, std::min(kLaunchPrimes, drainEvery) / arraySize));
+                ramp = 2 * k;
+                const size_t take =
+                    (size_t)std::min<long long>(k * arraySize - inBatch, (long long)(w.n - i));
+                if ((st = rs_sieve_submit(sv, w.lo, w.offs + i, (int64_t)take)) != RS_OK)
+                    return fail("submit", st);
+                const long long ends = (inBatch + (long long)take) / arraySize;
+                inBatch = (inBatch + (long long)take) % arraySize;
+                i += take;
+                if (ends > 0) {  // the last batch completed ends at prime i - inBatch - 1
+                    const bool line = (kernelCount + ends) / 20 > kernelCount / 20;
+                    kernelCount += ends;
+                    if (line) progress_line(w.lo + w.offs[i - (size_t)inBatch - 1] + 2);
+                }
+                if ((st = paced_drain((long long)take, false)) != RS_OK)
+                    return fail("collect", st);
+            }
+            feed.release(w);
         }
-        cur = std::move(nxt);
+        if (inBatch > 0) ++kernelCount;  // the last batch, clipped to P1
+    } else {
+        for (uint64_t gLow = low; gLow <= high;) {
+            const double lnp = std::log((double)std::max<uint64_t>(gLow, 16));
+            uint64_t width = std::max<uint64_t>((uint64_t)(arraySize * lnp), 1ull << 30);
+            width = std::min<uint64_t>(width, 1ull << 32);
+            const uint64_t next = std::min<uint64_t>(gLow + width, high + 1);
+            if ((st = rs_sieve_submit_range(sv, gLow, next)) != RS_OK) return fail("submit", st);
+            if (++kernelCount % 20 == 0) progress_line(next);
+            if ((st = paced_drain((long long)((double)(next - gLow) / lnp), false)) != RS_OK)
+                return fail("collect", st);
+            gLow = next;
+        }
     }
-    if (kernelCount == 0 && (st = drain()) != RS_OK) return fail("collect", st);
+    if ((st = paced_drain(0, true)) != RS_OK) return fail("collect", st);
     const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const unsigned long long swept = high + 1 - startLow;
     printf("\nTime taken: %gs\n", total);

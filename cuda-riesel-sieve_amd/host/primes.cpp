@@ -18,10 +18,15 @@
 // per prime per block, by a precomputed reciprocal.  The multiples of 7, 11
 // and 13 come from a precomputed pattern of period 1001 bytes; primes below
 // kSmallQ mark 32 KiB slices at a time, eight wheel steps per loop turn.
+// Each block's primes go to a buffer of its own, and then, all threads
+// copying at once, to their place in the output.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <thread>
 
 #include "host.h"
@@ -57,6 +62,21 @@ struct WheelTables {
 };
 const WheelTables kW;
 
+// A growable run of offsets whose new elements are left uninitialised (a
+// vector zero-fills what it grows by, a second pass over every prime).
+struct OffsBuf {
+    std::unique_ptr<uint32_t[]> p;
+    size_t n = 0, cap = 0;
+    void reserve(size_t c) {
+        if (c <= cap) return;
+        c = std::max(c, cap + cap / 2);
+        std::unique_ptr<uint32_t[]> q(new uint32_t[c]);
+        if (n) memcpy(q.get(), p.get(), n * sizeof(uint32_t));
+        p = std::move(q);
+        cap = c;
+    }
+};
+
 }  // namespace
 
 struct rs_primegen {
@@ -69,12 +89,17 @@ struct rs_primegen {
     size_t nsmall = 0;         // sieving primes below kSmallQ
     uint64_t bound2 = 0;       // survivors below this are prime without MR
     int threads = 1;
+    // per-block outputs kept from call to call (no page faults on the next
+    // call's hundreds of MB); a call made while another holds them uses its own
+    std::mutex parts_mu;
+    std::vector<OffsBuf> parts;
 };
 
 namespace {
 
-const uint64_t kSegBytes = 1u << 20;   // 1 MiB: 31.5M numbers per segment
+const uint64_t kSegBytes = 1u << 21;   // 2 MiB: 63M numbers per segment
 const uint64_t kBlockSegs = 16;        // segments per thread block
+const uint64_t kMinBlockBytes = 1u << 19;  // the smallest block: 15.7M numbers
 const uint64_t kSliceBytes = 1u << 15;  // 32 KiB slices for the small primes
 const uint32_t kSmallQ = 1u << 15;
 
@@ -127,7 +152,7 @@ inline void mark_prime(uint8_t* seg, uint64_t end, uint32_t q, uint64_t& pos, in
 void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbytes_total, uint64_t lo,
                  uint64_t hi, uint64_t origin, std::vector<uint8_t>& seg,
                  std::vector<uint64_t>& next, std::vector<uint8_t>& wheel,
-                 std::vector<uint32_t>& out) {
+                 OffsBuf& out) {
     const uint64_t top = std::min<uint64_t>(base + 30 * nbytes_total, hi) - 1;
     // sieving primes that reach the block, and for each the first multiple
     // q k >= max(q^2, base) with k prime to 30, as a byte position and the
@@ -158,7 +183,7 @@ void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbytes_total, uin
     // room for the block's primes up front (about 30 nbytes / ln(base)),
     // so the output does not move while it grows
     const double lnb = std::log((double)std::max<uint64_t>(base + 30 * nbytes_total, 16));
-    out.reserve(out.size() + (size_t)(30.0 * (double)nbytes_total / (lnb - 1.2) * 1.05) + 4096);
+    out.reserve(out.n + (size_t)(30.0 * (double)nbytes_total / (lnb - 1.2) * 1.05) + 4096);
     for (uint64_t done = 0; done < nbytes_total; done += kSegBytes) {
         const uint64_t nb = std::min(kSegBytes, nbytes_total - done);
         const uint64_t b0 = base + 30 * done;
@@ -202,18 +227,15 @@ void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbytes_total, uin
         }
         // survivors, in order: the words of 8 bytes (240 numbers) at both
         // ends of the window check the bounds, the test above bound2 only
-        // where the segment reaches it.  The output grows by the survivor
-        // count (a popcount pass), not by the segment's 8 nb candidates: the
-        // vector zero-fills what it grows by
+        // where the segment reaches it; room for them from a popcount pass
         size_t nfree = 0;
         for (uint64_t w = 0; w < nb; w += 8) {
             uint64_t word;
             memcpy(&word, seg.data() + w, 8);
             nfree += (size_t)__builtin_popcountll(~word);
         }
-        const size_t at = out.size();
-        out.resize(at + nfree);
-        uint32_t* o = out.data() + at;
+        out.reserve(out.n + nfree);
+        uint32_t* o = out.p.get() + out.n;
         const bool mr = b0 + 30 * nb > g->bound2;
         const uint64_t off = b0 - origin;
         for (uint64_t w = 0; w < nb; w += 8) {
@@ -239,7 +261,7 @@ void sieve_block(const rs_primegen* g, uint64_t base, uint64_t nbytes_total, uin
                 }
             }
         }
-        out.resize((size_t)(o - out.data()));
+        out.n = (size_t)(o - out.p.get());
     }
 }
 
@@ -303,16 +325,26 @@ extern "C" rs_status rs_primes_range(rs_primegen* g, uint64_t lo, uint64_t hi,
     // them per thread (multiples of 8 bytes, the last one takes the rest)
     const uint64_t nt0 = (uint64_t)std::max(1, g->threads);
     const uint64_t rounds = std::max<uint64_t>(1, (nseg + nt0 * kBlockSegs - 1) / (nt0 * kBlockSegs));
-    // (each block computes every sieving prime's first multiple: blocks of
-    // at least two segments, or one block)
+    // (each block computes every sieving prime's first multiple, about 3 ms
+    // at 1e14: blocks of at least kMinBlockBytes, so that a narrow window
+    // still keeps every thread busy, or one block)
     const uint64_t nblk =
-        std::max<uint64_t>(1, std::min<uint64_t>(nt0 * rounds, nbytes / (2 * kSegBytes)));
+        std::max<uint64_t>(1, std::min<uint64_t>(nt0 * rounds, nbytes / kMinBlockBytes));
     const uint64_t bbytes = (nbytes / nblk + 7) & ~7ull;
-    std::vector<std::vector<uint32_t>> parts(nblk);
-    std::atomic<uint64_t> next(0);
-    std::vector<std::thread> th;
+    std::unique_lock<std::mutex> own(g->parts_mu, std::try_to_lock);
+    std::vector<OffsBuf> mine;
+    std::vector<OffsBuf>& parts = own.owns_lock() ? g->parts : mine;
+    if (parts.size() < nblk) parts.resize(nblk);
+    for (uint64_t k = 0; k < nblk; ++k) parts[k].n = 0;
     const int nt = (int)std::min<uint64_t>(nt0, nblk);
-    auto work = [&]() {
+    auto run = [&](const std::function<void(int)>& f) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(f, t);
+        f(0);  // the calling thread takes a share too
+        for (auto& x : th) x.join();
+    };
+    std::atomic<uint64_t> next(0);
+    run([&](int) {
         std::vector<uint8_t> seg, wheel;
         std::vector<uint64_t> carry;
         for (uint64_t k; (k = next.fetch_add(1)) < nblk;) {
@@ -321,20 +353,19 @@ extern "C" rs_status rs_primes_range(rs_primegen* g, uint64_t lo, uint64_t hi,
             const uint64_t nb = k + 1 == nblk ? nbytes - first : std::min(bbytes, nbytes - first);
             sieve_block(g, base + 30 * first, nb, lo, hi, lo, seg, carry, wheel, parts[k]);
         }
-    };
-    for (int t = 1; t < nt; ++t) th.emplace_back(work);
-    work();  // the calling thread takes blocks too
-    for (auto& x : th) x.join();
-    int64_t total = (int64_t)small.size();
-    for (auto& p : parts) total += (int64_t)p.size();
-    *count = total;
-    if (total > cap) return RS_ERR_OVERFLOW;
-    int64_t c = 0;
-    for (uint32_t v : small) offsets[c++] = v;
-    for (auto& p : parts) {
-        memcpy(offsets + c, p.data(), p.size() * sizeof(uint32_t));
-        c += (int64_t)p.size();
-    }
+    });
+    // each block's primes go to their place in the output, every thread
+    // copying its share of the blocks
+    std::vector<int64_t> at(nblk + 1);
+    at[0] = (int64_t)small.size();
+    for (uint64_t k = 0; k < nblk; ++k) at[k + 1] = at[k] + (int64_t)parts[k].n;
+    *count = at[nblk];
+    if (at[nblk] > cap) return RS_ERR_OVERFLOW;
+    for (size_t i = 0; i < small.size(); ++i) offsets[i] = small[i];
+    run([&](int t) {
+        for (uint64_t k = (uint64_t)t; k < nblk; k += (uint64_t)nt)
+            if (parts[k].n) memcpy(offsets + at[k], parts[k].p.get(), parts[k].n * sizeof(uint32_t));
+    });
     return RS_OK;
 }
 

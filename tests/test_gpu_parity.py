@@ -777,16 +777,18 @@ def test_cli_gpu_generated_primes(tmp_path, gpu):
 
 
 def test_cli_batches_across_windows(rs, tmp_path, gpu):
-    """The CLI's CPU-generated batches (16384 or 65536 primes, cut from
-    windows of about eight batches, so many batches span two windows) cover
-    every prime of [P0, P1] once: a job with thousands of factors gives the
-    same lines as the GPU-generated run and as the library's own sieve,
-    verified and in the n-lists by default, every datless hit with -v (the
-    reference's "don't verify")."""
+    """The CLI's CPU-generated batches (16384 or 65536 primes; launches of
+    whole batches sliced from prime windows of 2^24, 2^25, then 2^26
+    numbers, so batches span window ends, and collects paced by the hits
+    seen)
+    cover every prime of [P0, P1] once: a job with thousands of factors
+    gives the same lines as the GPU-generated run and as the library's own
+    sieve, verified and in the n-lists by default, every datless hit with -v
+    (the reference's "don't verify")."""
     abcd = rs.Abcd.synth(5, 10, 3, 10**5, 2000, 9000, 200, seed=4, odd_k=2)
     path = tmp_path / "in.abcd"
     abcd.write(path)
-    lo, hi = 10**6 + 1, 2 * 10**7
+    lo, hi = 10**6 + 1, 8 * 10**7
     exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
     outs = []
     runs = (["-b", "1"], ["-b", "3"], ["-g"], ["-b", "1", "-v"])

@@ -138,9 +138,10 @@ def test_primes_range_matches_oracle(rs, orc):
 
 @pytest.mark.parametrize("threads", [1, 4])
 def test_primes_range_across_segments_and_blocks(rs, orc, threads):
-    """A window of 8.5 segments of 1 MiB wheel bytes (31.5M numbers each;
-    one block on one thread, four blocks of about two segments on four, each
-    prime's next multiple carried from segment to segment), with sieving
+    """A window of 4.3 segments of 2 MiB wheel bytes (63M numbers each; one
+    block on one thread, four blocks of one segment and a piece of the next
+    on four, each prime's next multiple carried from segment to segment),
+    with sieving
     primes to 1.01e7 and an odd, ragged end, equals the oracle; so do
     pieces of it cut at arbitrary points."""
     lo = 102297149770000 + 1
