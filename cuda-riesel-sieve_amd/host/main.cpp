@@ -120,8 +120,14 @@ class WindowFeed {
                 free_.pop_front();
             }
             const uint64_t hi = std::min<uint64_t>(lo + width, high_ + 1);
+            // room for the window's primes: about (hi - lo) / ln lo, and never
+            // more than pi(hi) < 1.25506 hi / ln hi (Rosser and Schoenfeld),
+            // which bounds windows that start at small p (a short count is
+            // retried below with the exact one)
             const double lnp = std::log((double)std::max<uint64_t>(lo, 16));
-            size_t need = (size_t)((double)(hi - lo) / std::max(1.0, lnp - 1.5) * 1.2) + 4096;
+            const double lnh = std::log((double)std::max<uint64_t>(hi, 16));
+            size_t need = (size_t)std::min((double)(hi - lo) / std::max(1.0, lnp - 1.5) * 1.2,
+                                           1.25506 * (double)hi / lnh) + 4096;
             int64_t got = 0;
             rs_status st = RS_OK;
             for (int pass = 0; pass < 2; ++pass) {

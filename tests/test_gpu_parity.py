@@ -810,6 +810,33 @@ def test_cli_batches_across_windows(rs, tmp_path, gpu):
         assert got == want, (extra, len(got), len(want), sorted(set(got) ^ set(want))[:5])
 
 
+def test_cli_from_small_p(rs, orc, tmp_path, gpu):
+    """The CLI from p = 211 (just past the input's presieve bound, where a
+    2^30-wide first window holds 5% of its numbers as primes) to 3e7: every
+    listed candidate's factor, as the library's own sieve and the oracle
+    find them, with the small primes' dense hits collected without loss."""
+    abcd = rs.Abcd.synth(5, 10, 3, 10**5, 2000, 9000, 200, seed=4, odd_k=2)
+    path = tmp_path / "in.abcd"
+    abcd.write(path)
+    lo, hi = 211, 3 * 10**7
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+    out = tmp_path / "f.txt"
+    r = subprocess.run([exe, "-i", str(path), "-p", str(lo), "-P", str(hi), "-m", "8", "-b", "2",
+                        "-O", str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "dropped" not in r.stderr
+    got = sorted(out.read_text().splitlines())
+    plan = rs.Plan(abcd)
+    offs = rs.PrimeGen(hi + 1).range(lo, hi + 1)
+    hits = run_device(rs, plan, lo, offs, m=512)
+    small = np.array([o for o in offs if lo + int(o) < 10**6], np.uint32)
+    assert [h for h in hits if h[0] < 10**6] == run_oracle(orc, plan, 512, lo, small)
+    want = sorted(rs.format_factor(p, k, 5, n).strip() for p, k, n in hits
+                  if plan.contains(k, n))
+    assert len(want) > 1000
+    assert got == want, (len(got), len(want), sorted(set(got) ^ set(want))[:5])
+
+
 @pytest.mark.parametrize("quantum_ms", [0.0, 0.05])
 def test_server_results_alone_and_in_the_mix(rs, orc, gpu, quantum_ms):
     """Jobs arriving between steps are admitted as slots and budget free up;
