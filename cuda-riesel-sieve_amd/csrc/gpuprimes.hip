@@ -4,11 +4,14 @@
 //
 // Every prime p in [lo, hi) (hi - lo <= 2^32, 3 <= lo, hi <= 2^63) is
 // written as the uint32 offset p - lo, ascending.  The odd numbers of the
-// range are split into segments of SEG_BITS; one workgroup sieves a segment
+// range are split into segments (kSegSmall or kSegWide odd numbers); one
+// workgroup sieves a segment
 // in an LDS bitmap with the odd primes q <= S (S = sqrt(hi) up to 2^25,
 // else 2^22),
 // primes below SMALL_Q marked by the whole block and larger ones by one
-// thread each; survivors at or above S^2 are proved by deterministic
+// thread each (each prime's first multiple in the segment from a remainder
+// in double precision, about a third of the instructions of a 64-bit
+// integer remainder); survivors at or above S^2 are proved by deterministic
 // Miller-Rabin (bases 2, 325, 9375, 28178, 450775, 9780504, 1795265022,
 // exact for 64-bit n).  A scan of the segment counts then places each
 // segment's offsets.
@@ -22,9 +25,29 @@
 #include "gpuprimes.h"
 #include "mont.h"
 
-#define SEG_BITS (1u << 18)   // odd numbers per segment: 32 KiB of LDS bitmap
-#define SEG_WORDS (SEG_BITS / 32)
-#define SMALL_Q 4096u         // primes below this are marked block-wide
+// Segment shapes: 2^18 odd numbers (a 32 KiB LDS bitmap) per workgroup of
+// 256 threads, or, for ranges of at least 512 such wide segments (2^30
+// numbers), 2^20 (128 KiB) per workgroup of 1024.  Every sieving prime is
+// visited once per segment, so the wide shape makes a quarter of the visits
+// (2.4 ms against 4.8 per 2^30 numbers at 1e14, with the remainders below);
+// narrow ranges keep the small shape, whose segments spread over more CUs.
+constexpr u32 kSegSmall = 1u << 18, kSegWide = 1u << 20;
+#define SMALL_Q 4096u  // primes below this are marked block-wide
+
+// v mod q for any 64-bit v and q < 2^26, in double precision: with
+// v = a 2^32 + b, x = (a mod q) (2^32 mod q) + b < 2^53 is exact in a
+// double, and each quotient estimate t / q below is off by less than one.
+__device__ __forceinline__ u32 mod_small(u64 v, u32 q) {
+    const double rq = 1.0 / (double)q;
+    auto red = [&](u64 t) -> u32 {  // t < 2^53
+        long long r = (long long)t - (long long)(u64)((double)t * rq) * (long long)q;
+        if (r < 0) r += q;
+        else if (r >= (long long)q) r -= q;
+        return (u32)r;
+    };
+    const u32 c = red(1ull << 32);
+    return red((u64)red(v >> 32) * c + (v & 0xffffffffull));
+}
 
 __device__ __forceinline__ bool mr_prime(u64 n) {
     // n odd, n > 2^20 here (only survivors above S^2 are tested)
@@ -62,24 +85,26 @@ __device__ __forceinline__ bool mr_prime(u64 n) {
 
 // Sieves segment blockIdx.x of the odd numbers base + 2i, i < nodd; writes
 // the bitmap (bit set = composite or excluded) and the prime count.
-extern "C" __global__ void __launch_bounds__(256)
-rs_prime_sieve_kernel(uint64_t base, long long nodd, const uint32_t* __restrict__ sp, int np,
-                      int nsmall, uint64_t s2, uint32_t* __restrict__ bitmap,
-                      uint32_t* __restrict__ counts) {
+template <u32 SEG_BITS, int NT>
+__device__ __forceinline__ void prime_sieve(uint64_t base, long long nodd,
+                                            const uint32_t* __restrict__ sp, int np, int nsmall,
+                                            uint64_t s2, uint32_t* __restrict__ bitmap,
+                                            uint32_t* __restrict__ counts) {
+    constexpr u32 SEG_WORDS = SEG_BITS / 32;
     __shared__ uint32_t bm[SEG_WORDS];
-    __shared__ uint32_t red[256];
+    __shared__ uint32_t red[NT];
     const int tid = threadIdx.x;
     const long long i_lo = (long long)blockIdx.x * SEG_BITS;
     const u32 nbits = (u32)min((long long)SEG_BITS, nodd - i_lo);
     const u64 v_lo = base + 2ull * (u64)i_lo;
     const u64 v_hi = v_lo + 2ull * (nbits - 1);
-    for (u32 w = tid; w < SEG_WORDS; w += 256) bm[w] = 0;
+    for (u32 w = tid; w < SEG_WORDS; w += NT) bm[w] = 0;
     __syncthreads();
     // first index i with q | v_lo + 2i and v_lo + 2i >= q^2 (q odd)
     auto first = [&](u32 q) -> u64 {
         const u64 qq = (u64)q * q;
         if (qq >= v_lo) return (qq - v_lo) >> 1;
-        const u32 r = (u32)(v_lo % q);
+        const u32 r = mod_small(v_lo, q);
         const u32 d = r == 0 ? 0u : q - r;
         return (d & 1) ? (u64)((d + q) >> 1) : (u64)(d >> 1);
     };
@@ -88,11 +113,11 @@ rs_prime_sieve_kernel(uint64_t base, long long nodd, const uint32_t* __restrict_
         const u32 q = sp[k];
         if ((u64)q * q > v_hi) break;
         const u64 i0 = first(q);
-        for (u64 i = i0 + (u64)tid * q; i < nbits; i += 256ull * q)
+        for (u64 i = i0 + (u64)tid * q; i < nbits; i += (u64)NT * q)
             atomicOr(&bm[i >> 5], 1u << (i & 31));
     }
     // large primes: one thread each (at most SEG_BITS / SMALL_Q marks)
-    for (int k = nsmall + tid; k < np; k += 256) {
+    for (int k = nsmall + tid; k < np; k += NT) {
         const u32 q = sp[k];
         if ((u64)q * q > v_hi) break;
         for (u64 i = first(q); i < nbits; i += q) atomicOr(&bm[i >> 5], 1u << (i & 31));
@@ -100,7 +125,7 @@ rs_prime_sieve_kernel(uint64_t base, long long nodd, const uint32_t* __restrict_
     __syncthreads();
     // exclusions: 1, the tail past nbits, survivors >= S^2 that fail MR
     u32 cnt = 0;
-    for (u32 w = tid; w < SEG_WORDS; w += 256) {
+    for (u32 w = tid; w < SEG_WORDS; w += NT) {
         u32 x = bm[w];
         const u32 b0 = w * 32;
         if (b0 >= nbits) x = ~0u;
@@ -120,11 +145,25 @@ rs_prime_sieve_kernel(uint64_t base, long long nodd, const uint32_t* __restrict_
     }
     red[tid] = cnt;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
         if (tid < s) red[tid] += red[tid + s];
         __syncthreads();
     }
     if (tid == 0) counts[blockIdx.x] = red[0];
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+rs_prime_sieve_kernel(uint64_t base, long long nodd, const uint32_t* __restrict__ sp, int np,
+                      int nsmall, uint64_t s2, uint32_t* __restrict__ bitmap,
+                      uint32_t* __restrict__ counts) {
+    prime_sieve<kSegSmall, 256>(base, nodd, sp, np, nsmall, s2, bitmap, counts);
+}
+
+extern "C" __global__ void __launch_bounds__(1024)
+rs_prime_sieve_wide_kernel(uint64_t base, long long nodd, const uint32_t* __restrict__ sp, int np,
+                           int nsmall, uint64_t s2, uint32_t* __restrict__ bitmap,
+                           uint32_t* __restrict__ counts) {
+    prime_sieve<kSegWide, 1024>(base, nodd, sp, np, nsmall, s2, bitmap, counts);
 }
 
 // Exclusive scan of the segment counts in place (one block); total in
@@ -155,10 +194,12 @@ rs_prime_scan_kernel(uint32_t* __restrict__ counts, int nseg) {
 }
 
 // Writes the offsets of segment blockIdx.x starting at counts[seg].
-extern "C" __global__ void __launch_bounds__(256)
-rs_prime_emit_kernel(uint64_t base, uint64_t lo, const uint32_t* __restrict__ bitmap,
-                     const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
-                     long long cap) {
+template <u32 SEG_BITS>
+__device__ __forceinline__ void prime_emit(uint64_t base, uint64_t lo,
+                                           const uint32_t* __restrict__ bitmap,
+                                           const uint32_t* __restrict__ counts,
+                                           uint32_t* __restrict__ out, long long cap) {
+    constexpr u32 SEG_WORDS = SEG_BITS / 32;
     __shared__ uint32_t part[256];
     const int tid = threadIdx.x;
     const long long w0 = (long long)blockIdx.x * SEG_WORDS;
@@ -186,6 +227,20 @@ rs_prime_emit_kernel(uint64_t base, uint64_t lo, const uint32_t* __restrict__ bi
             ++at;
         }
     }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+rs_prime_emit_kernel(uint64_t base, uint64_t lo, const uint32_t* __restrict__ bitmap,
+                     const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
+                     long long cap) {
+    prime_emit<kSegSmall>(base, lo, bitmap, counts, out, cap);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+rs_prime_emit_wide_kernel(uint64_t base, uint64_t lo, const uint32_t* __restrict__ bitmap,
+                          const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
+                          long long cap) {
+    prime_emit<kSegWide>(base, lo, bitmap, counts, out, cap);
 }
 
 #define HIPCHK(x)                                 \
@@ -245,8 +300,10 @@ rs_status gpu_primes_enqueue(rs_gpu_primegen* g, uint64_t lo, uint64_t hi, uint3
         return RS_ERR_ARG;
     const uint64_t base = lo | 1;
     const long long nodd = base >= hi ? 0 : (long long)((hi - base + 1) / 2);
-    const int nseg = (int)std::max<long long>(1, (nodd + SEG_BITS - 1) / SEG_BITS);
-    const size_t words = (size_t)nseg * SEG_WORDS;
+    const bool wide = nodd >= 512ll * kSegWide;
+    const long long seg_bits = wide ? kSegWide : kSegSmall;
+    const int nseg = (int)std::max<long long>(1, (nodd + seg_bits - 1) / seg_bits);
+    const size_t words = (size_t)nseg * (size_t)(seg_bits / 32);
     // buffers are reused: earlier work on any stream must not still read them
     if (words > g->bitmap_words || nseg + 1 > g->counts_cap) HIPCHK(hipDeviceSynchronize());
     if (words > g->bitmap_words) {
@@ -265,14 +322,22 @@ rs_status gpu_primes_enqueue(rs_gpu_primegen* g, uint64_t lo, uint64_t hi, uint3
         hipLaunchKernelGGL(rs_zero_u32, dim3(1), dim3(64), 0, stream, g->d_counts + nseg, 1);
         HIPCHK(hipGetLastError());
     } else {
-        hipLaunchKernelGGL(rs_prime_sieve_kernel, dim3(nseg), dim3(256), 0, stream, base, nodd,
-                           g->d_sp, g->np, g->nsmall, g->s2, g->d_bitmap, g->d_counts);
+        if (wide)
+            hipLaunchKernelGGL(rs_prime_sieve_wide_kernel, dim3(nseg), dim3(1024), 0, stream, base,
+                               nodd, g->d_sp, g->np, g->nsmall, g->s2, g->d_bitmap, g->d_counts);
+        else
+            hipLaunchKernelGGL(rs_prime_sieve_kernel, dim3(nseg), dim3(256), 0, stream, base, nodd,
+                               g->d_sp, g->np, g->nsmall, g->s2, g->d_bitmap, g->d_counts);
         HIPCHK(hipGetLastError());
         hipLaunchKernelGGL(rs_prime_scan_kernel, dim3(1), dim3(1024), 0, stream, g->d_counts,
                            nseg);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(rs_prime_emit_kernel, dim3(nseg), dim3(256), 0, stream, base, lo,
-                           g->d_bitmap, g->d_counts, out, (long long)cap);
+        if (wide)
+            hipLaunchKernelGGL(rs_prime_emit_wide_kernel, dim3(nseg), dim3(256), 0, stream, base,
+                               lo, g->d_bitmap, g->d_counts, out, (long long)cap);
+        else
+            hipLaunchKernelGGL(rs_prime_emit_kernel, dim3(nseg), dim3(256), 0, stream, base, lo,
+                               g->d_bitmap, g->d_counts, out, (long long)cap);
         HIPCHK(hipGetLastError());
     }
     *count_dev = g->d_counts + nseg;

@@ -736,9 +736,13 @@ def test_batch_large_chunk_single_launch_equals_plain(rs, gpu):
 
 def test_gpu_primes_equal_cpu_generator(rs, gpu):
     """GPU segmented sieve (+ Miller-Rabin above the sieving bound) vs the
-    host generator, including tiny ranges, 1e12 and ranges near 2^62/2^63."""
+    host generator, including tiny ranges, 1e12, 1e14 (sieving primes to
+    1e7, ragged ends; the second range is wide enough for the wide segment
+    shape, 2^20 odd numbers per workgroup) and ranges near 2^62/2^63."""
     cases = [(3, 1 << 20, 1 << 22), (10**9, 10**9 + (1 << 24), 10**12),
              (10**12, 10**12 + (1 << 26), 10**12 + (1 << 27)),
+             (10**14 + 12345, 10**14 + 12345 + (1 << 28) + 777, 10**14 + (1 << 29)),
+             (10**14 + 999, 10**14 + 999 + (1 << 30) + 12345, 10**14 + (1 << 31)),
              ((1 << 62) - (1 << 20), 1 << 62, 1 << 62),
              ((1 << 63) - (1 << 20), 1 << 63, 1 << 63)]
     for lo, hi, hmax in cases:
