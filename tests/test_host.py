@@ -136,12 +136,12 @@ def test_primes_range_matches_oracle(rs, orc):
         assert np.array_equal(got, want), (lo, hi)
 
 
-@pytest.mark.parametrize("threads", [1, 4])
+@pytest.mark.parametrize("threads", [1, 4, 16])
 def test_primes_range_across_segments_and_blocks(rs, orc, threads):
     """A window of 4.3 segments of 2 MiB wheel bytes (63M numbers each; one
     block on one thread, four blocks of one segment and a piece of the next
-    on four, each prime's next multiple carried from segment to segment),
-    with sieving
+    on four, sixteen blocks of about a quarter segment on sixteen, each
+    prime's next multiple carried from segment to segment), with sieving
     primes to 1.01e7 and an odd, ragged end, equals the oracle; so do
     pieces of it cut at arbitrary points."""
     lo = 102297149770000 + 1
@@ -153,6 +153,29 @@ def test_primes_range_across_segments_and_blocks(rs, orc, threads):
     cuts = [lo, lo + 12345, lo + (1 << 22) + 1, lo + (5 << 22) - 3, hi]
     parts = [c0 - lo + g.range(c0, c1).astype(np.int64) for c0, c1 in zip(cuts, cuts[1:])]
     assert np.array_equal(np.concatenate(parts), want.astype(np.int64))
+
+
+def test_primes_range_concurrent_calls(rs, orc):
+    """Calls on one generator from several threads at once (the ctypes calls
+    release the GIL): one call keeps the generator's per-block buffers, the
+    others use their own; every result equals the oracle."""
+    import threading
+    lo = 10**12 + 1
+    g = rs.PrimeGen(lo + 2**26, threads=4)
+    wins = [(lo + i * 2**23 + 17 * i, lo + (i + 1) * 2**23) for i in range(8)]
+    got = [None] * len(wins)
+
+    def work(j):
+        for _ in range(3):
+            got[j] = g.range(*wins[j])
+
+    th = [threading.Thread(target=work, args=(j,)) for j in range(len(wins))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for (a, b), offs in zip(wins, got):
+        assert np.array_equal(offs, orc.true_primes(a, b)), (a, b)
 
 
 def test_primes_batch_is_reference_layout(rs, orc):
