@@ -123,20 +123,27 @@ __device__ __forceinline__ uint2 lds_ld2(u32 a) {
     return make_uint2(v.x, v.y);
 }
 
-// True iff one of the 8 16-bit slots of e equals the fingerprint of the
-// low word lo (its high half): the slots minus the fingerprint (packed
-// 16-bit subtractions whose second operand takes the high half of lo for
-// both halves, so the fingerprint needs no instruction of its own), their
-// packed minimum, folded with its halves swapped, has a zero low half.
-__device__ __forceinline__ bool bucket_match(uint4 e, u32 lo) {
+// The 8 16-bit slots of e minus the fingerprint of the low word lo (its
+// high half; packed 16-bit subtractions whose second operand takes the high
+// half of lo for both halves, so the fingerprint needs no instruction of its
+// own), reduced to their packed minimum: a zero half means a slot matched.
+__device__ __forceinline__ u16x2 bucket_min(uint4 e, u32 lo) {
     const u16x2 l = __builtin_bit_cast(u16x2, lo);
     const u16x2 f = __builtin_shufflevector(l, l, 1, 1);
     const u16x2 a = __builtin_bit_cast(u16x2, e.x) - f, b = __builtin_bit_cast(u16x2, e.y) - f;
     const u16x2 c = __builtin_bit_cast(u16x2, e.z) - f, d = __builtin_bit_cast(u16x2, e.w) - f;
-    const u16x2 mn = __builtin_elementwise_min(__builtin_elementwise_min(a, b),
-                                               __builtin_elementwise_min(c, d));
+    return __builtin_elementwise_min(__builtin_elementwise_min(a, b),
+                                     __builtin_elementwise_min(c, d));
+}
+// Whether a packed minimum has a zero half: folded with its halves
+// swapped, its low half is zero.
+__device__ __forceinline__ bool has_zero_half(u16x2 mn) {
     const u16x2 g = __builtin_elementwise_min(mn, __builtin_shufflevector(mn, mn, 1, 0));
     return g.x == 0;
+}
+// True iff one of the 8 16-bit slots of e equals the fingerprint of lo.
+__device__ __forceinline__ bool bucket_match(uint4 e, u32 lo) {
+    return has_zero_half(bucket_min(e, lo));
 }
 
 // Whether one of the first c slots of bucket e (the slots its inserts have
@@ -305,6 +312,9 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
     auto entry_at = [&](u32 x) { return idx ? rows_lds[lds_u16(x)] : lds_ld2(x); };
     uint2 rown = make_uint2(0, 0);
     if (PF && A > 0) rown = entry_at(min(la, llast));
+    // EXACT: each lane keeps its step t0, so its Y run starts at a fixed
+    // offset into every row's Y entries
+    const char* const ybase = Yb + 8 * t0;
     for (int base = 0; base < total; base += 64 * TW) {
         // the row's entry: in the list, or (idx) behind its row number
         uint2 row;
@@ -324,7 +334,7 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
         // (EXACT: tested on the advanced address, so lcur need not be kept)
         const bool ok = PF ? (EXACT ? la <= llast + lstep : lcur <= llast) : a < A;
         const u64 okm = lanes(ok);
-        const u64* const yp = (const u64*)(Yb + row.y + 8 * tcur);
+        const u64* const yp = (const u64*)((EXACT ? ybase : Yb + 8 * tcur) + row.y);
         uint4 e[TW];
         u32 lw[TW], ba[TW];
 #pragma unroll
@@ -342,20 +352,32 @@ __device__ __forceinline__ void giant_steps(const JobDev& J, const JobDev* __res
         for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
         u64 mm[TW], full[TW];
         u64 anyf = 0, anym = 0;
+        if (CHAIN) {
 #pragma unroll
-        for (int u = 0; u < TW; ++u) {
-            mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
-            anym |= mm[u];
-            if (CHAIN) {
+            for (int u = 0; u < TW; ++u) {
+                mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
+                anym |= mm[u];
                 full[u] = lanes(cnt[(ba[u] - tab_addr) >> 4] >= 8) & okm;
                 anyf |= full[u];
             }
+        } else {
+            // one test for the TW items together (a match anywhere is rare):
+            // the packed minimum over all their slots has a zero half; the
+            // items are told apart below, only when one matched
+            u16x2 acc = bucket_min(e[0], lw[0]);
+#pragma unroll
+            for (int u = 1; u < TW; ++u)
+                acc = __builtin_elementwise_min(acc, bucket_min(e[u], lw[u]));
+            anym = lanes(has_zero_half(acc)) & okm;
+#pragma unroll
+            for (int u = 0; u < TW; ++u) mm[u] = 0;
         }
         if (PF && idx) rown = rows_lds[rnum];
         // rare: keep only the matches in filled slots
         if (anym) {
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
+                if (!CHAIN) mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
                 u32 c = 0;
                 if (in_mask(mm[u])) c = cnt[(ba[u] - tab_addr) >> 4];
                 mm[u] &= lanes(bucket_match_live(e[u], lw[u], c));
