@@ -1238,12 +1238,18 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
         }
 #pragma unroll
         for (int u = 0; u < TW; ++u) e[u] = lds_bucket(ba[u]);
+        // one fingerprint test for the TW items together (a match anywhere is
+        // rare): the packed minimum over all their slots has a zero half; the
+        // items are told apart below, only when one matched
         u64 mm[TW], full[TW];
-        u64 anyf = 0, anym = 0;
+        u64 anyf = 0;
+        u16x2 acc = bucket_min(e[0], lw[0]);
+#pragma unroll
+        for (int u = 1; u < TW; ++u) acc = __builtin_elementwise_min(acc, bucket_min(e[u], lw[u]));
+        const u64 anym = lanes(has_zero_half(acc)) & okm;
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
-            anym |= mm[u];
+            mm[u] = 0;
             if (CHAIN) {
                 full[u] = lanes(cnt[ba[u] >> 4] >= 8) & okm;
                 anyf |= full[u];
@@ -1253,6 +1259,7 @@ __device__ __forceinline__ void giant_steps_group(const JobDev& J, const JobDev*
         if (anym) {
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
+                mm[u] = lanes(bucket_match(e[u], lw[u])) & okm;
                 u32 c = 0;
                 if (in_mask(mm[u])) c = cnt[ba[u] >> 4];
                 mm[u] &= lanes(bucket_match_live(e[u], lw[u], c));
