@@ -841,6 +841,35 @@ def test_cli_from_small_p(rs, orc, tmp_path, gpu):
     assert got == want, (len(got), len(want), sorted(set(got) ^ set(want))[:5])
 
 
+def test_cli_edge_ranges(rs, tmp_path, gpu):
+    """The CLI on ranges at the edges of its batching: a prime gap (no prime
+    in [1328, 1360]: no launch, no factor, a clean summary), one prime (the
+    reference's debug prime alone, datless), and 2^22 numbers below 2^62
+    (host primes proved by Miller-Rabin above the sieving bound) with CPU
+    and with GPU primes, against the library's own sieve."""
+    exe = os.path.join(ROOT, "cuda-riesel-sieve_amd", "rsieve")
+
+    def run(*flags):
+        out = tmp_path / f"f{len(list(tmp_path.iterdir()))}.txt"
+        r = subprocess.run([exe, "-i", R745, "-m", "8", "-O", str(out)] + [str(f) for f in flags],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, (flags, r.stderr)
+        return r.stdout, (sorted(out.read_text().splitlines()) if out.exists() else [])
+
+    stdout, lines = run("-p", 1328, "-P", 1360)
+    assert "Factors found: 0" in stdout and not lines
+    stdout, lines = run("-p", KNOWN_P, "-P", KNOWN_P, "-v")
+    assert lines == [f"{KNOWN_P} | 18300*745^242670-1"]
+    lo, hi = (1 << 62) - (1 << 22), (1 << 62) - 1
+    got_cpu = run("-p", lo, "-P", hi, "-v")[1]
+    got_gpu = run("-p", lo, "-P", hi, "-v", "-g")[1]
+    plan = rs.Plan(rs.Abcd.read(R745))
+    offs = rs.PrimeGen(hi + 1).range(lo, hi + 1)
+    hits = run_device(rs, plan, lo, offs, m=512)
+    want = sorted(rs.format_factor(p, k, 745, n).strip() for p, k, n in hits)
+    assert got_cpu == got_gpu == want
+
+
 @pytest.mark.parametrize("quantum_ms", [0.0, 0.05])
 def test_server_results_alone_and_in_the_mix(rs, orc, gpu, quantum_ms):
     """Jobs arriving between steps are admitted as slots and budget free up;
