@@ -145,7 +145,9 @@ typedef struct rs_primegen rs_primegen;
 rs_status rs_primegen_create(int small_prime_scale, uint64_t hi, int threads,
                              rs_primegen** out);
 /* Every prime in [lo, hi) as offsets p - lo (hi - lo <= 2^32).  If cap is
- * too small returns RS_ERR_OVERFLOW with *count = the number needed. */
+ * too small returns RS_ERR_OVERFLOW with *count = the number needed.
+ * Threads may call it on one generator at once: one call at a time reuses
+ * the generator's per-block buffers, the others allocate their own. */
 rs_status rs_primes_range(rs_primegen* g, uint64_t lo, uint64_t hi,
                           uint32_t* offsets, int64_t cap, int64_t* count);
 /* generateGPUPrimes (:1838-1987) replacement: the first count primes >= low
