@@ -334,6 +334,25 @@ def test_large_p_near_2_62_and_2_63(rs, orc, gpu):
         assert got == want
 
 
+def test_submit_rejects_p_past_2_63(rs, orc, gpu):
+    """rs_sieve_submit refuses a call with any p = low + offset at or past
+    2^63 (it reads the offsets for their largest only where low + 2^32 - 1
+    could reach 2^63), or with low below 3, and sieves nothing for it: the
+    sieve then still gives the oracle's hits for valid primes."""
+    plan = rs.Plan(rs.Abcd.synth(2, 12, 3, 10**9, 1000000, 1003000, 1000, seed=3))
+    sv = rs.Sieve(plan, m=1024)
+    low = (1 << 63) - (1 << 19)
+    offs = rs.PrimeGen(low + (1 << 18), threads=16).range(low, low + (1 << 18))
+    for bad_low, bad in ((low, np.append(offs[:5], np.uint32((1 << 19) + 1))),
+                         (2, np.array([1], np.uint32))):
+        with pytest.raises(rs.RsError) as e:
+            sv.submit(bad_low, bad)
+        assert e.value.status == rs.RS_ERR_ARG
+    sv.submit(low, offs)
+    assert sv.collect() == run_oracle(orc, plan, 1024, low, offs)
+    sv.close()
+
+
 def planted_pairs(b, p, nmin, nmax):
     """(k, n) with 1 < k < 2^31 and p | k*b^n - 1, n in [nmin, nmax]: k is
     b^-n mod p whenever that is small (about one n in p/2^31)."""
