@@ -34,19 +34,27 @@
 constexpr u32 kSegSmall = 1u << 18, kSegWide = 1u << 20;
 #define SMALL_Q 4096u  // primes below this are marked block-wide
 
-// v mod q for any 64-bit v and q < 2^26, in double precision: with
-// v = a 2^32 + b, x = (a mod q) (2^32 mod q) + b < 2^53 is exact in a
-// double, and each quotient estimate t / q below is off by less than one.
+// v mod q for any 64-bit v and an odd q < 2^26, in double precision: with
+// v = a 2^32 + b, every quotient below is under 2^32 (x = (a mod q)
+// (2^32 mod q) + b < q^2 + 2^32), so a reciprocal within 2^-48 of 1/q makes
+// each estimate off by less than one, and the remainders, in (-q, 2q), are
+// exact in 32-bit arithmetic.  The reciprocal is the hardware estimate with
+// two Newton steps (each squares the relative error).
 __device__ __forceinline__ u32 mod_small(u64 v, u32 q) {
-    const double rq = 1.0 / (double)q;
-    auto red = [&](u64 t) -> u32 {  // t < 2^53
-        long long r = (long long)t - (long long)(u64)((double)t * rq) * (long long)q;
-        if (r < 0) r += q;
-        else if (r >= (long long)q) r -= q;
+    const double qd = (double)q;
+    double rq = __builtin_amdgcn_rcp(qd);
+    rq = __builtin_fma(__builtin_fma(-qd, rq, 1.0), rq, rq);
+    rq = __builtin_fma(__builtin_fma(-qd, rq, 1.0), rq, rq);
+    auto fix = [&](u32 t, u32 qe) -> u32 {  // t - qe q, taken into [0, q)
+        int r = (int)(t - qe * q);
+        if (r < 0) r += (int)q;
+        else if (r >= (int)q) r -= (int)q;
         return (u32)r;
     };
-    const u32 c = red(1ull << 32);
-    return red((u64)red(v >> 32) * c + (v & 0xffffffffull));
+    auto red32 = [&](u32 t) { return fix(t, (u32)((double)t * rq)); };
+    const u32 c = red32(0u - q);  // 2^32 mod q = (2^32 - q) mod q
+    const u64 x = (u64)red32((u32)(v >> 32)) * c + (v & 0xffffffffull);
+    return fix((u32)x, (u32)((double)x * rq));
 }
 
 __device__ __forceinline__ bool mr_prime(u64 n) {
