@@ -33,6 +33,7 @@
 // narrow ranges keep the small shape, whose segments spread over more CUs.
 constexpr u32 kSegSmall = 1u << 18, kSegWide = 1u << 20;
 #define SMALL_Q 4096u  // primes below this are marked block-wide
+constexpr int kMaxSmall = 1024;  // room for them (563 odd primes below SMALL_Q)
 
 // v mod q for any 64-bit v and an odd q < 2^26, in double precision: with
 // v = a 2^32 + b, every quotient below is under 2^32 (x = (a mod q)
@@ -101,6 +102,7 @@ __device__ __forceinline__ void prime_sieve(uint64_t base, long long nodd,
     constexpr u32 SEG_WORDS = SEG_BITS / 32;
     __shared__ uint32_t bm[SEG_WORDS];
     __shared__ uint32_t red[NT];
+    __shared__ uint32_t i0s[kMaxSmall];  // first index of each small prime
     const int tid = threadIdx.x;
     const long long i_lo = (long long)blockIdx.x * SEG_BITS;
     const u32 nbits = (u32)min((long long)SEG_BITS, nodd - i_lo);
@@ -116,19 +118,37 @@ __device__ __forceinline__ void prime_sieve(uint64_t base, long long nodd,
         const u32 d = r == 0 ? 0u : q - r;
         return (d & 1) ? (u64)((d + q) >> 1) : (u64)(d >> 1);
     };
-    // small primes: the whole block marks each one
-    for (int k = 0; k < nsmall; ++k) {
+    // small primes: each one's first index once per block (a thread per
+    // prime; ~0u where q^2 is past the segment, and so for every later
+    // prime), then the whole block marks each one
+    for (int k = tid; k < nsmall; k += NT) {
         const u32 q = sp[k];
-        if ((u64)q * q > v_hi) break;
-        const u64 i0 = first(q);
+        i0s[k] = (u64)q * q > v_hi ? ~0u : (u32)min(first(q), (u64)nbits);
+    }
+    __syncthreads();
+    for (int k = 0; k < nsmall; ++k) {
+        const u32 i0 = i0s[k];
+        if (i0 == ~0u) break;
+        const u32 q = sp[k];
         for (u64 i = i0 + (u64)tid * q; i < nbits; i += (u64)NT * q)
             atomicOr(&bm[i >> 5], 1u << (i & 31));
     }
-    // large primes: one thread each (at most SEG_BITS / SMALL_Q marks)
-    for (int k = nsmall + tid; k < np; k += NT) {
-        const u32 q = sp[k];
-        if ((u64)q * q > v_hi) break;
-        for (u64 i = first(q); i < nbits; i += q) atomicOr(&bm[i >> 5], 1u << (i & 31));
+    // large primes: one thread each (at most SEG_BITS / SMALL_Q marks), four
+    // of a thread's primes loaded together so their loads overlap
+    bool done = false;
+    for (int k0 = nsmall + tid; k0 < np && !done; k0 += 4 * NT) {
+        u32 qv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) qv[j] = k0 + j * NT < np ? sp[k0 + j * NT] : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const u32 q = qv[j];
+            if (done || q == 0 || (u64)q * q > v_hi) {
+                done = true;
+                continue;
+            }
+            for (u64 i = first(q); i < nbits; i += q) atomicOr(&bm[i >> 5], 1u << (i & 31));
+        }
     }
     __syncthreads();
     // exclusions: 1, the tail past nbits, survivors >= S^2 that fail MR
@@ -202,22 +222,22 @@ rs_prime_scan_kernel(uint32_t* __restrict__ counts, int nseg) {
 }
 
 // Writes the offsets of segment blockIdx.x starting at counts[seg].
-template <u32 SEG_BITS>
+template <u32 SEG_BITS, int NT>
 __device__ __forceinline__ void prime_emit(uint64_t base, uint64_t lo,
                                            const uint32_t* __restrict__ bitmap,
                                            const uint32_t* __restrict__ counts,
                                            uint32_t* __restrict__ out, long long cap) {
     constexpr u32 SEG_WORDS = SEG_BITS / 32;
-    __shared__ uint32_t part[256];
+    __shared__ uint32_t part[NT];
     const int tid = threadIdx.x;
     const long long w0 = (long long)blockIdx.x * SEG_WORDS;
-    // each thread owns SEG_WORDS / 256 consecutive words
-    const int per = SEG_WORDS / 256;
+    // each thread owns SEG_WORDS / NT consecutive words
+    constexpr int per = SEG_WORDS / NT;
     u32 c = 0;
     for (int k = 0; k < per; ++k) c += __popc(~bitmap[w0 + tid * per + k]);
     part[tid] = c;
     __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
+    for (int off = 1; off < NT; off <<= 1) {
         const u32 v = tid >= off ? part[tid - off] : 0;
         __syncthreads();
         part[tid] += v;
@@ -241,14 +261,14 @@ extern "C" __global__ void __launch_bounds__(256)
 rs_prime_emit_kernel(uint64_t base, uint64_t lo, const uint32_t* __restrict__ bitmap,
                      const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
                      long long cap) {
-    prime_emit<kSegSmall>(base, lo, bitmap, counts, out, cap);
+    prime_emit<kSegSmall, 256>(base, lo, bitmap, counts, out, cap);
 }
 
 extern "C" __global__ void __launch_bounds__(256)
 rs_prime_emit_wide_kernel(uint64_t base, uint64_t lo, const uint32_t* __restrict__ bitmap,
                           const uint32_t* __restrict__ counts, uint32_t* __restrict__ out,
                           long long cap) {
-    prime_emit<kSegWide>(base, lo, bitmap, counts, out, cap);
+    prime_emit<kSegWide, 256>(base, lo, bitmap, counts, out, cap);
 }
 
 #define HIPCHK(x)                                 \
@@ -288,6 +308,10 @@ extern "C" rs_status rs_gpu_primegen_create(int device, uint64_t hi_max, rs_gpu_
     std::vector<uint32_t> odd(ps.begin() + 1, ps.end());
     g->np = (int)odd.size();
     g->nsmall = (int)(std::lower_bound(odd.begin(), odd.end(), SMALL_Q) - odd.begin());
+    if (g->nsmall > kMaxSmall) {  // the sieve kernel keeps one LDS word per small prime
+        rs_gpu_primegen_free(g);
+        return RS_ERR_ARG;
+    }
     const uint64_t last = odd.empty() ? 3 : odd.back();
     g->s2 = last * last;
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
