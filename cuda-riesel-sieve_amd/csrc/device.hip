@@ -279,7 +279,16 @@ extern "C" rs_status rs_sieve_submit_device_noalloc(rs_sieve* s, uint64_t low,
 extern "C" rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* offsets,
                                      int64_t count) {
     if (!s || count < 0 || (count > 0 && !offsets)) return RS_ERR_ARG;
-    if (count == 0) return RS_OK;
+    if (count == 0) {  // only the staging buffers, made ready before the first primes
+        HIPCHK(hipSetDevice(s->device));
+        for (int b = 0; b < 2; ++b)
+            if (!s->h_pin[b]) {
+                HIPCHK(hipHostMalloc(&s->h_pin[b], (size_t)s->max_batch * 4,
+                                     hipHostMallocDefault));
+                HIPCHK(hipMalloc(&s->d_offs[b], (size_t)s->max_batch * 4));
+            }
+        return RS_OK;
+    }
     if (low < 3) return RS_ERR_ARG;
     // the largest offset is read only when low + 2^32 - 1 could reach 2^63
     if (low >= (1ull << 63) - 0xffffffffull) {

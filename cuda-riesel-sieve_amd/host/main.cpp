@@ -419,6 +419,9 @@ int main(int argc, char* argv[]) {
     if (!gpuPrimes) {
         WindowFeed feed(gen, low, high);
         feed.start();
+        // the pinned staging buffers (64 MB each), allocated while the first
+        // window of primes is sieved rather than at the first launches
+        if ((st = rs_sieve_submit(sv, low, nullptr, 0)) != RS_OK) return fail("submit", st);
         long long inBatch = 0;  // primes of the open batch submitted so far
         for (;;) {
             WindowFeed::Window w;

@@ -254,7 +254,9 @@ const char* rs_sieve_kernel_name(const rs_sieve* s);
 /* Primes p = low + offsets[i] from HOST memory (p odd, 3 <= p < 2^63, p ∤ b).
  * Copied through pinned double buffers on a copy stream, then sieved on the
  * compute stream; returns once the copy of this call's data is issued.
- * Replaces :1543-1578. */
+ * count = 0 only allocates the pinned buffers (max_batch primes each) if
+ * they are not there yet, so a caller can have them ready before its first
+ * primes (offsets may then be NULL).  Replaces :1543-1578. */
 rs_status rs_sieve_submit(rs_sieve* s, uint64_t low, const uint32_t* offsets,
                           int64_t count);
 /* Same, with offsets already in device memory (HBM-resident input): one
