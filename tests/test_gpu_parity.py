@@ -353,6 +353,24 @@ def test_submit_rejects_p_past_2_63(rs, orc, gpu):
     sv.close()
 
 
+def test_zero_count_submit_prepares_staging(rs, orc, gpu):
+    """rs_sieve_submit with no primes only makes the pinned staging ready:
+    it sieves nothing, and later submits (twice, so both buffers are used)
+    give the oracle's hits."""
+    plan = rs.Plan(rs.Abcd.synth(745, 6, 3, 10**6, 20000, 30000, 1000, seed=5))
+    low = 10**9 + 1
+    offs = rs.PrimeGen(low + (1 << 20)).range(low, low + (1 << 20))
+    sv = rs.Sieve(plan, m=512, max_batch=1 << 15)
+    sv.submit(low, np.empty(0, np.uint32))
+    assert sv.collect() == []
+    sv.submit(low, offs)
+    sv.submit(low, offs[: len(offs) // 3])
+    got = sv.collect()
+    sv.close()
+    want = run_oracle(orc, plan, 512, low, offs)
+    assert got == sorted(want + [h for h in want if h[0] - low <= int(offs[len(offs) // 3 - 1])])
+
+
 def planted_pairs(b, p, nmin, nmax):
     """(k, n) with 1 < k < 2^31 and p | k*b^n - 1, n in [nmin, nmax]: k is
     b^-n mod p whenever that is small (about one n in p/2^31)."""
